@@ -1,0 +1,224 @@
+/*
+ * specmc_hip.h -- C ABI of the MI355X (gfx950) kernels behind SpecMC's
+ * time-step hot path.
+ *
+ * The reference (edwardaud/SpecMC) is pure numpy/scipy and has no FFI; the
+ * drop-in boundary is its Python model protocol (step / cfl_dt / stokes /
+ * diagnostics, see specmc/api.py:147-190 and core/rbc_galerkin.py:258-497).
+ * The host side of that protocol lives in the Python package specmc_amd; every
+ * numerical operation it performs per step goes through the entry points
+ * below.  Each declaration names the reference code whose arithmetic it
+ * replaces.
+ *
+ * Conventions
+ *  - All entry points return 0 on success and a nonzero status otherwise;
+ *    smx_last_error() returns a message for the calling thread.  No C++
+ *    exception crosses the ABI.
+ *  - Pointers are device pointers (HBM), owned by the caller; sizes are
+ *    element counts.  `stream` is a hipStream_t passed as void*.
+ *  - `prec` selects the arithmetic type: 0 = double (FP64), 1 = float (FP32).
+ *  - Fields are real: with the mirror (box) symmetry every theta-type field
+ *    has real x-mode coefficients and every stream-function-type field purely
+ *    imaginary ones; the latter are stored by their imaginary part.
+ *  - `active` (int[B], may be NULL) masks runs: entries != 0 are processed.
+ */
+#ifndef SPECMC_HIP_H
+#define SPECMC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* library identification and error text */
+int smx_version(void);
+const char *smx_last_error(void);
+
+/* General strided batched GEMM
+ *   C[z](i,j) = alpha * rs(i) * sum_k A[z](i,k) B[z](k,j) + beta * C[z](i,j)
+ * with A(i,k) = A[i*ars + k*acs], B(k,j) = B[k*brs + j*bcs],
+ * C(i,j) = C[i*crs + j*ccs]; batch z = z1*nz2 + z2 offsets
+ * (sa1*z1 + sa2*z2) etc.; rs(i) = rscale[i % rperiod] (rscale may be NULL).
+ * Masking: mask_mode 0 none, 1 run = i / rows_per_run, 2 run = z1, 3 run = z2.
+ * Replaces the dense products of the reference: the z-transforms V@, Vi@,
+ * D@, D2@ (rbc_galerkin.py:322-339, implicit_theta.py:296-304), the x-mode
+ * DFTs of np.fft.rfft/irfft on the mirror-symmetric half grid
+ * (rbc_galerkin.py:322-339, coupled_model.py:440-449), the Green applies
+ * (stokes_coupled.py:318-325, lagged_stokes.py:195-200) and the operator
+ * builds (vv_stokes.py:266-307, bfbt.py:157-180). */
+int smx_gemm(int prec, int M, int N, int K, double alpha, double beta,
+             const void *A, long ars, long acs, long sa1, long sa2,
+             const void *B, long brs, long bcs, long sb1, long sb2,
+             void *C, long crs, long ccs, long sc1, long sc2,
+             int nz1, int nz2, const void *rscale, int rperiod,
+             const int *active, int mask_mode, int rows_per_run,
+             void *stream);
+
+/* Batched LU with partial pivoting, in place, column-major N x N matrices
+ * (LAPACK getrf semantics).  Replaces scipy.linalg.lu_factor in
+ * ChebTau._factors_for (rbc_galerkin.py:119-144), VVStokesReduced
+ * (vv_stokes.py:245-278) and SpectralBFBTPrecond._build (bfbt.py:157-181).
+ * Matrix m of batch belongs to run m / per_run. */
+int smx_lu_factor(int prec, int nmat, int N, void *LU, int *piv,
+                  int per_run, const int *active, void *stream);
+
+/* Batched LU solve (getrs, one right-hand side per matrix):
+ *   x = LU^{-1} ( scale * b ) with b[N-2] = b[N-1] = 0 when tau_zero,
+ * matrix m, rhs at rhs + m*N; run = m / per_run; the scale is
+ * rhs_scale * kscale[m % per_run] * bscale[run] (either may be NULL).  With
+ * shared != 0 the factors of matrix m are those of m % per_run (one set of
+ * factors serves every run, e.g. the isoviscous lam = 0 operators).
+ * Replaces sla.lu_solve in ChebTau.solve (rbc_galerkin.py:171-185),
+ * VVStokesReduced.solve (vv_stokes.py:309-321) and
+ * SpectralBFBTPrecond.apply (bfbt.py:185-191).  In-place allowed. */
+int smx_lu_solve(int prec, int nmat, int N, const void *LU, const int *piv,
+                 const void *rhs, void *x, int tau_zero, double rhs_scale,
+                 const void *kscale, const void *bscale, int per_run,
+                 int shared, const int *active, void *stream);
+
+/* Chebyshev-tau operators, column-major, per (run, mode):
+ *   kind 0: D2 - (k^2 + lam_b) I, rows N-2, N-1 = (-1)^n, 1
+ *           (ChebTau._assemble, rbc_galerkin.py:164-169);
+ *   kind 1: L_k M_b + 2 k^2 C_b,k with rows N-2, N-1 = V[0], V[-1] where
+ *           LM = D2@M_b is given per run (lm) and C = Comm_b @ G_k per
+ *           (run, mode) (cg): VVStokesReduced._assemble (vv_stokes.py:302-307)
+ *           and the BFBT block (bfbt.py:157-180). */
+int smx_tau_assemble(int prec, int kind, int B, int Nk, int N,
+                     const void *D2, const void *kx, const void *lam,
+                     const void *Mb, const void *lm, const void *cg,
+                     const void *V0, const void *V1, void *out,
+                     const int *active, void *stream);
+
+/* Exact truncated Chebyshev multiplication matrices M[b] (row-major N x N)
+ * from coefficient vectors a[b] (cheb_mult_matrix, vv_stokes.py:81-109). */
+int smx_cheb_mult(int prec, int B, int N, const void *a, void *M,
+                  const int *active, void *stream);
+
+/* The viscosity law on the padded half grid (coupled_model.py:333-399):
+ * T = 1 - z + theta, counters of T outside [0,1] / [0,2], T clipped to
+ * [0,2], eta = exp(-ln(deta) T) (law 0) or exp(-ln(deta)(1+Ts) T/(T+Ts))
+ * (law 1), then the window [lo, hi] with the 1e-12 dead-band
+ * (eta_bounds.py:360-379) or the C^2 smooth clip (eta_bounds.py:411-480).
+ * One block per run; grid rows carry mirror weights (row 0 and row nrow-1
+ * count once, the others twice).  law params per run: [ln_deta, Ts, lo, hi,
+ * smooth_w, law]; counters per run (double[4]): n_T_out, n_clip_T,
+ * n_clip_eta, max_T_excursion (accumulated). */
+int smx_eta_law(int prec, int B, int nrow, int N, const void *theta,
+                const void *z, const double *params, void *eta,
+                double *counters, const int *active, void *stream);
+
+/* Positivity guard and BFBT profile on the unpadded half grid
+ * (coupled_model.py:451-496, bfbt.py:99-108): if min(eta) <= 0 the window is
+ * re-applied (counted in counters[4]); then prof(z) = 1/mean_x(1/eta)
+ * (profile 0, 'harm'), mean_x(eta) (1, 'diag') or exp(mean_x(log eta))
+ * (2, 'geom') over the full Nx grid. */
+int smx_eta_guard_profile(int prec, int B, int Nx, int N, void *eta,
+                          const double *params, void *prof, int profile,
+                          double *counters, const int *active, void *stream);
+
+/* Pseudo-spectral eta product of the coupled Schur operator
+ * (FreeSchurOperator.mult_eta, lagged_stokes.py:162-193, both halves of the
+ * one-sided convolution, partner sign -1, the cross term flipped): for each
+ * run, field f (0..nf-1) and padded z node q: synthesise the x-modes of
+ * y[b][f][:, q] on the mirror half of the 2Nx grid (odd parity for f < nodd,
+ * even otherwise), multiply by eta[b][:, q] and analyse back to Nk modes.
+ * y/out layout [B][Nk][nf][Q]; fscale [nf][Nk] (may be NULL) scales the
+ * input modes (the 2 k^2 and k factors inside the convolution). */
+int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
+              const void *y, const void *fscale, const void *eta, void *out,
+              const void *So, const void *Ce, const void *Ao, const void *Ae,
+              const int *active, void *stream);
+
+/* Advection product of the implicit theta operator
+ * (ImplicitThetaSolver.advect, implicit_theta.py:300-304):
+ *   a = Ae * ( -(ux * So (ks0 y0) + uz * Ce y1) [+ e1] [+ h_b] [+ e2] )
+ * per run and z node; y [B][Nk][2][N] holds z-values of the odd (k v) and
+ * even (D v) inputs; e1/e2 extra grid terms and h a per-run constant (the
+ * explicit right-hand side of RBC.explicit_rhs, rbc_galerkin.py:396-423). */
+int smx_advect_prod(int prec, int B, int Nk, int nrow, int N,
+                    const void *y, const void *ks0, const void *ux,
+                    const void *uz,
+                    const void *e1, const void *h, const void *e2,
+                    void *out, const void *So, const void *Ce,
+                    const void *Ae, const int *active, void *stream);
+
+/* Elementwise / small kernels of the step.
+ * axpby: out = (a*ab[b]) x + (b*bb[b]) y per run (ab, bb, y may be NULL).
+ * rowscale: out[b][k][j] = (alpha*ab[b]) ks[k] x[b][k][j], tau entries zeroed
+ *   when tau_zero -- the Stokes right-hand sides i k Ra theta
+ *   (stokes_coupled.py:292-302, vv_stokes.py:311-312, rbc_galerkin.py:349).
+ * tau_rows: the Dirichlet rows v(0), v(1) of an operator output.
+ * stokes_combine: Lm(eta v) + D2(eta 2k^2 G v) + 2k^2 eta D2 G v
+ *   - 4k D(eta k D G v) from the truncated products t[B][Nk][4][N] and
+ *   dt = (D2 t0, D2 t1, D t3), tau rows from v (lagged_stokes.py:202-220).
+ * theta_combine: a0 v - (D2 v - k^2 v) - adv, tau rows from v
+ *   (implicit_theta.py:354-383).
+ * theta_rhs: (b0 th - b1 thp) + k psi, tau entries zeroed, + H at (0, 0),
+ *   + src on the equation rows (implicit_theta.py:385-449).
+ * phi_grid: 2 eta ((gxx^2 + gzz^2 + 2 gxz^2) - tr^2/3), gzz = -gxx, times
+ *   scale[b] (dissipation.py:313-362).
+ * diag: Nu_top, Nu_bot, Nu_energy, urms, umax, amp (rbc_galerkin.py:439-453);
+ *   dth0 = D theta_0 of run b at dth0 + b*ldd.
+ * maxabs: per-run max |x|. */
+int smx_axpby(int prec, long n_per_run, int B, double a, const void *ab,
+              const void *x, double b, const void *bb, const void *y,
+              void *out, const int *active, void *stream);
+int smx_rowscale(int prec, int B, int Nk, int N, const void *x,
+                 const void *ks, double alpha, const void *ab, int tau_zero,
+                 void *out, const int *active, void *stream);
+int smx_tau_rows(int prec, int B, int Nk, int N, const void *v,
+                 const void *V0, const void *V1, void *out,
+                 const int *active, void *stream);
+int smx_stokes_combine(int prec, int B, int Nk, int N, const void *t,
+                       const void *dt, const void *kx, const void *v,
+                       const void *V0, const void *V1, void *out,
+                       const int *active, void *stream);
+int smx_theta_combine(int prec, int B, int Nk, int N, const void *v,
+                      const void *d2v, const void *adv, const void *kx,
+                      const void *a0, const void *V0, const void *V1,
+                      void *out, const int *active, void *stream);
+int smx_theta_rhs(int prec, int B, int Nk, int N, const void *th,
+                  const void *thp, const void *psi, const void *kx,
+                  const void *b0, const void *b1, const void *H,
+                  const void *src, void *out, const int *active,
+                  void *stream);
+int smx_phi_grid(int prec, int B, long np, const void *gxx, const void *gxz,
+                 const void *eta, const void *scale, void *out,
+                 const int *active, void *stream);
+int smx_diag(int prec, int B, int Nx, int N, const void *ux, const void *uz,
+             const void *th, const void *dth0, long ldd, const void *V0,
+             const void *V1, const void *wq, double *out,
+             const int *active, void *stream);
+int smx_maxabs(int prec, int B, long n_per_run, const void *x, double *out,
+               const int *active, void *stream);
+
+/* FGMRES state machine (krylov_fgmres.py:60-179), one solver instance per
+ * run.  The per-run control block ctl (int[B][8]) and scalars sc
+ * (double[B][8]) live in device memory; see the implementation for the
+ * layout.  fg_init: x = 0, r = b, beta = ||b||, tol = max(atol, rtol*beta).
+ * fg_cycle_end: y = H^{-1} g, x += Z y, then the inputs of this round's
+ * preconditioner (V_j) and operator (x) apply.  fg_post: Z_j store, MGS +
+ * Givens + the stop tests, or the restart from the true residual.  The input
+ * pointer of the next operator apply (Z_j or x).  fg_count: number of runs
+ * not yet finished (written to *out).  ctl is int[10][B] and sc double[8][B]
+ * (struct of arrays); ctl + 5*B / ctl + 6*B are the preconditioner and
+ * operator masks of the current round. */
+int smx_fg_init(int prec, int B, long n, int m, const void *b, void *x,
+                void *Vb, void *g, int *ctl, double *sc, double rtol,
+                double atol, int maxiter, const int *active, void *stream);
+int smx_fg_cycle_end(int prec, int B, long n, int m, void *x, const void *Zb,
+                     const void *Vb, void *H, void *g, void *y, void *vin,
+                     void *zout, int *ctl, void *stream);
+int smx_fg_post(int prec, int B, long n, int m, const void *b, void *w,
+                const void *zin, void *Zb, void *Vb, void *H, void *cs,
+                void *sn, void *g, int *ctl, double *sc, void *stream);
+int smx_fg_count(int B, const int *ctl, int *out, void *stream);
+
+/* CFL dt (rbc_galerkin.py:465-471, cfl_policy.py:64-74): per run
+ * dt = min(dtmax, clip(safety * h / max(umax, 1e-12), dtmin, dtmax)). */
+int smx_cfl_dt(int B, const double *umax, const double *params, double *dt,
+               const int *active, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,162 @@
+"""Public API: config -> device model -> march (the surface of specmc/api.py).
+
+``build_model(cfg)`` returns the device-backed class of ``cfg.viscosity_model``
+and checks it, ``march`` / ``march_adaptive`` run the reference's loops
+unchanged (api.py:147-190), and the introspection helpers keep their
+meaning.  ``precond='legacy'`` (the lagged-LU route) is served by the BFBT
+FGMRES route on the device: both solve the same coupled Schur system to the
+same relative tolerance, only the Krylov path differs; a ``UserWarning`` says
+so once per process.
+"""
+from __future__ import annotations
+
+import os
+import warnings
+
+import numpy as np
+
+from . import _numerics as NUM
+from .config import (ConvectionConfig, normalize_compressibility,
+                     normalize_viscosity_model)
+from .core.cfl_policy import cfl_dt as _policy_dt
+
+__all__ = ["MODEL_CLASSES", "numerics", "model_spec", "model_kwargs",
+           "build_model", "describe", "production_dt", "load_checkpoint",
+           "march", "march_adaptive"]
+
+_BUILDERS = {"const": "RBC", "eta_z": "RBCImplicitTheta", "eta_T": "RBCCoupled",
+             "arrhenius": "RBCCoupled"}
+_warned_legacy = [False]
+
+
+def _classes():
+    from . import models
+    return {k: getattr(models, v) for k, v in _BUILDERS.items()}
+
+
+class _LazyClasses(dict):
+    def __missing__(self, key):
+        self.update(_classes())
+        return dict.__getitem__(self, key)
+
+    def items(self):
+        self.update(_classes())
+        return dict.items(self)
+
+
+MODEL_CLASSES = _LazyClasses()
+
+
+def numerics():
+    return NUM.as_dict()
+
+
+def _kwargs(cfg, vm):
+    base = dict(Ra=float(cfg.Ra), Nx=int(cfg.Nx), Nz=int(cfg.Nz), Lx=float(cfg.Lx),
+                internal_heating=float(cfg.internal_heating),
+                dissipation_number=float(cfg.dissipation_number))
+    if vm == "const":
+        return base
+    base.update(deta_T=float(cfg.deta_T), eta_min=float(cfg.eta_min),
+                eta_max=float(cfg.eta_max), eta_smooth=float(cfg.eta_smooth),
+                symmetrize=bool(cfg.symmetrize))
+    if vm == "eta_z":
+        return base
+    base.update(arrhenius_Ts=float(cfg.arrhenius_Ts),
+                precond=str(cfg.precond).strip().lower(), lag=int(cfg.lag),
+                K=int(cfg.K), max_K=int(cfg.max_K),
+                defect_tol=(None if float(cfg.defect_tol) <= 0.0 else float(cfg.defect_tol)))
+    return base
+
+
+def model_spec(cfg: ConvectionConfig):
+    """-> the construction recipe for ``cfg`` (api.py:73-94)."""
+    if not isinstance(cfg, ConvectionConfig):
+        raise TypeError("expected ConvectionConfig, got %r" % (type(cfg),))
+    cfg.validate()
+    vm = normalize_viscosity_model(cfg.viscosity_model)
+    comp = normalize_compressibility(cfg.compressibility)
+    if vm not in _BUILDERS:
+        raise NotImplementedError("viscosity_model=%r is not implemented" % (vm,))
+    return dict(model=vm, compressibility=comp, builder=_BUILDERS[vm],
+                cls=MODEL_CLASSES[vm], kwargs=_kwargs(cfg, vm))
+
+
+def model_kwargs(cfg):
+    return dict(model_spec(cfg)["kwargs"])
+
+
+def build_model(cfg: ConvectionConfig):
+    """Build the device model for ``cfg`` and check its class (api.py:102-110)."""
+    spec = model_spec(cfg)
+    kw = dict(spec["kwargs"])
+    if kw.get("precond") == "legacy" and not _warned_legacy[0]:
+        _warned_legacy[0] = True
+        warnings.warn("precond='legacy' runs the BFBT-preconditioned FGMRES route on "
+                      "the device (same coupled Schur system and tolerance; the "
+                      "lagged-LU Krylov path is not carried)", UserWarning, stacklevel=2)
+    model = spec["cls"](**kw)
+    if type(model) is not spec["cls"]:
+        raise AssertionError("internal mapping error: viscosity_model=%r built %s, "
+                             "expected %s" % (spec["model"], type(model).__name__,
+                                              spec["cls"].__name__))
+    return model
+
+
+def describe(cfg):
+    spec = model_spec(cfg)
+    return dict(config=cfg.canonical(), model=spec["model"],
+                model_class=spec["cls"].__name__, builder=spec["builder"],
+                ctor_kwargs=dict(spec["kwargs"]), numerics=numerics())
+
+
+def production_dt(model, that):
+    return float(_policy_dt(model, that, safety=NUM.CFL_SAFETY, dtmax=NUM.DTMAX))
+
+
+def load_checkpoint(path):
+    """-> ``(theta_coefficients, meta)`` from a driver checkpoint ``.npz``
+    (io/checkpoint.py:80-103): fields ``that`` and optionally ``t, n, fac``."""
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    with np.load(path, allow_pickle=False) as z:
+        that = np.array(z["that"])
+        meta = {k: (z[k].item() if z[k].shape == () else np.array(z[k]))
+                for k in z.files if k != "that"}
+    return that, meta
+
+
+def march(model, that, dt, nsteps, keep_states=False, on_step=None):
+    """Fixed-dt march (api.py:147-166)."""
+    dt = float(dt)
+    nsteps = int(nsteps)
+    s = np.array(that, copy=True)
+    states = np.empty((nsteps,) + s.shape, dtype=complex) if keep_states else None
+    for n in range(nsteps):
+        s = model.step(s, dt)
+        if states is not None:
+            states[n] = s
+        if on_step is not None:
+            on_step(n + 1, (n + 1) * dt, s)
+    info = dict(steps=nsteps, t=nsteps * dt, dt=dt)
+    if states is not None:
+        info["states"] = states
+    return s, info
+
+
+def march_adaptive(model, that, tstop, safety=None, dtmax=None, on_step=None):
+    """CFL-limited march to ``tstop`` (api.py:169-190)."""
+    safety = NUM.CFL_SAFETY if safety is None else float(safety)
+    dtmax = NUM.DTMAX if dtmax is None else float(dtmax)
+    s = np.array(that, copy=True)
+    t = 0.0
+    n = 0
+    dt = float("nan")
+    while t < float(tstop):
+        dt = float(_policy_dt(model, s, safety=safety, dtmax=dtmax))
+        s = model.step(s, dt)
+        t += dt
+        n += 1
+        if on_step is not None:
+            on_step(n, t, s)
+    return s, dict(steps=n, t=t, dt=dt, safety=safety, dtmax=dtmax)

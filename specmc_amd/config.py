@@ -1,0 +1,221 @@
+"""``ConvectionConfig``: the public scenario description (config.py:239-668 of the
+reference), with the same fields, defaults, aliases and error behaviour.
+
+Fields whose physics this build does not carry raise ``NotImplementedError``
+with a message containing "not implemented": the pressure-dependent (pV)
+Arrhenius branch (``arrhenius_V > 0``) and the TALA / ALA compressibility
+models, in addition to the reference's own reserved fields.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import warnings
+from dataclasses import dataclass, replace
+from typing import Any, Dict, Tuple
+
+__all__ = ["ConvectionConfig", "IMPLEMENTED_VISCOSITY_MODELS",
+           "RESERVED_VISCOSITY_MODELS", "IMPLEMENTED_COMPRESSIBILITY",
+           "RESERVED_COMPRESSIBILITY", "normalize_viscosity_model",
+           "normalize_compressibility", "arrhenius_effective_E"]
+
+IMPLEMENTED_VISCOSITY_MODELS = ("const", "eta_z", "eta_T", "arrhenius")
+RESERVED_VISCOSITY_MODELS = ("eta_Tz",)
+IMPLEMENTED_COMPRESSIBILITY = ("boussinesq",)
+RESERVED_COMPRESSIBILITY = ("tala", "ala")
+
+_VISC = {"const": "const", "constant": "const", "isoviscous": "const",
+         "eta_z": "eta_z", "etaz": "eta_z", "eta_t": "eta_T", "etat": "eta_T",
+         "eta_tz": "eta_Tz", "etatz": "eta_Tz", "arrhenius": "arrhenius"}
+_COMP = {"boussinesq": "boussinesq", "incompressible": "boussinesq",
+         "tala": "tala", "truncated_anelastic": "tala",
+         "ala": "ala", "anelastic": "ala"}
+
+
+def normalize_viscosity_model(name):
+    """Canonical viscosity model name (case / alias tolerant, config.py:181-194)."""
+    key = str(name).strip().lower()
+    if key in _VISC:
+        return _VISC[key]
+    raise ValueError("unknown viscosity_model %r; implemented: %s; reserved "
+                     "(planned): %s" % (name, ", ".join(IMPLEMENTED_VISCOSITY_MODELS),
+                                        ", ".join(RESERVED_VISCOSITY_MODELS)))
+
+
+def normalize_compressibility(name):
+    key = str(name).strip().lower()
+    if key in _COMP:
+        return _COMP[key]
+    raise ValueError("unknown compressibility %r; implemented: %s"
+                     % (name, ", ".join(IMPLEMENTED_COMPRESSIBILITY)))
+
+
+def _not_impl(what):
+    return NotImplementedError("%s is not implemented in specmc_amd" % (what,))
+
+
+def arrhenius_E(deta_T, Ts):
+    """``E = ln(deta_T) Ts (1 + Ts)`` (arrhenius.py:208-223)."""
+    D, s = float(deta_T), float(Ts)
+    if not (math.isfinite(D) and D > 0.0):
+        raise ValueError("deta_T must be positive and finite (got %r)" % (deta_T,))
+    if not (math.isfinite(s) and s > 0.0):
+        raise ValueError("arrhenius_Ts must be positive and finite (got %r)" % (Ts,))
+    return float(math.log(D) * s * (1.0 + s))
+
+
+def arrhenius_effective_E(cfg):
+    Ts = float(cfg.arrhenius_Ts)
+    return 0.0 if Ts <= 0.0 else arrhenius_E(cfg.deta_T, Ts)
+
+
+@dataclass(frozen=True)
+class ConvectionConfig:
+    """Physical scenario + scenario-dependent numerical parameters (frozen)."""
+
+    viscosity_model: str = "eta_T"
+    deta_T: float = 1000.0
+    eta_z_coeff: float = 0.0
+    depth_dependence: float = 0.0
+    arrhenius_E: float = 0.0
+    arrhenius_V: float = 0.0
+    reference_anchor: str = "D"
+    arrhenius_Ts: float = 0.0
+    eta_min: float = 1e-3
+    eta_max: float = 1e3
+    eta_smooth: float = 0.0
+    precond: str = "legacy"
+    internal_heating: float = 0.0
+    dissipation_number: float = 0.0
+    compressibility: str = "boussinesq"
+    Ra: float = 1e4
+    Nx: int = 64
+    Nz: int = 96
+    Lx: float = 2.0
+    lag: int = 16
+    K: int = 3
+    defect_tol: float = 1e-8
+    max_K: int = 8
+    symmetrize: bool = True
+
+    def __post_init__(self):
+        self.validate()
+
+    def validate(self):
+        """Raise ``ValueError`` / ``NotImplementedError`` for unusable values;
+        warn (``UserWarning``) for ``symmetrize=False`` on a variable-viscosity
+        model (config.py:426-647)."""
+        vm = normalize_viscosity_model(self.viscosity_model)
+        if vm in RESERVED_VISCOSITY_MODELS:
+            raise _not_impl("viscosity_model=%r" % (self.viscosity_model,))
+        comp = normalize_compressibility(self.compressibility)
+        if comp in RESERVED_COMPRESSIBILITY:
+            raise _not_impl("compressibility=%r" % (self.compressibility,))
+        if not math.isfinite(float(self.internal_heating)):
+            raise ValueError("internal_heating must be finite (got %r)"
+                             % (self.internal_heating,))
+        Di = float(self.dissipation_number)
+        if not math.isfinite(Di):
+            raise ValueError("dissipation_number must be finite (got %r)"
+                             % (self.dissipation_number,))
+        if Di < 0.0:
+            raise ValueError("dissipation_number is Di = alpha0 g0 d / cP0, so it "
+                             "cannot be negative (got %r)" % (self.dissipation_number,))
+        if Di > 0.0 and not (math.isfinite(float(self.Ra)) and float(self.Ra) > 0.0):
+            raise ValueError("dissipation_number=%r makes the source (Di/Ra) Phi, so "
+                             "Ra must be finite and positive (got Ra=%r)"
+                             % (self.dissipation_number, self.Ra))
+        for name in ("eta_z_coeff", "depth_dependence"):
+            if float(getattr(self, name)) != 0.0:
+                raise _not_impl("%s=%r" % (name, getattr(self, name)))
+        Ts, E, V = float(self.arrhenius_Ts), float(self.arrhenius_E), float(self.arrhenius_V)
+        if not (math.isfinite(Ts) and math.isfinite(E) and math.isfinite(V)):
+            raise ValueError("arrhenius_E/arrhenius_Ts/arrhenius_V must be finite "
+                             "(got %r, %r, %r)" % (self.arrhenius_E, self.arrhenius_Ts,
+                                                   self.arrhenius_V))
+        if Ts < 0.0:
+            raise ValueError("arrhenius_Ts is a temperature offset, so it cannot be "
+                             "negative (got %r)" % (self.arrhenius_Ts,))
+        if V < 0.0:
+            raise ValueError("arrhenius_V cannot be negative (got %r)" % (self.arrhenius_V,))
+        if V > 0.0:
+            raise _not_impl("arrhenius_V=%r (the pressure-dependent pV Arrhenius law)"
+                            % (self.arrhenius_V,))
+        if str(self.reference_anchor) != "D":
+            raise ValueError("reference_anchor=%r has no effect while arrhenius_V == 0, "
+                             "so it must stay at its default 'D'" % (self.reference_anchor,))
+        if Ts == 0.0 and E != 0.0:
+            raise ValueError("arrhenius_E=%r is set but arrhenius_Ts is 0, which means "
+                             "'Arrhenius off'; the activation would be silently ignored"
+                             % (self.arrhenius_E,))
+        if Ts > 0.0:
+            if vm != "arrhenius":
+                raise ValueError("arrhenius_Ts=%r selects the Arrhenius law, which needs "
+                                 "viscosity_model='arrhenius' (got %r)"
+                                 % (self.arrhenius_Ts, self.viscosity_model))
+            Eref = arrhenius_E(self.deta_T, Ts)
+            if E != 0.0 and abs(E - Eref) > 1e-12 * abs(Eref):
+                raise ValueError("arrhenius_E=%r contradicts the calibrated value "
+                                 "E = ln(deta_T) Ts (1+Ts) = %r" % (self.arrhenius_E, Eref))
+        lo, hi = float(self.eta_min), float(self.eta_max)
+        if not (math.isfinite(lo) and math.isfinite(hi)):
+            raise ValueError("eta bounds must be finite (got %r, %r)" % (lo, hi))
+        if lo <= 0.0:
+            raise ValueError("eta_min must be positive (got %r)" % (lo,))
+        if hi <= lo:
+            raise ValueError("eta_max must exceed eta_min (got %r, %r)" % (lo, hi))
+        w = float(self.eta_smooth)
+        if not math.isfinite(w):
+            raise ValueError("eta_smooth must be finite (got %r)" % (self.eta_smooth,))
+        if w < 0.0:
+            raise ValueError("eta_smooth must be >= 0 (got %r)" % (w,))
+        if w > 0.5 * (math.log(hi) - math.log(lo)):
+            raise ValueError("eta_smooth=%r is too wide for the window [%r, %r]"
+                             % (w, lo, hi))
+        if str(self.precond).strip().lower() not in ("legacy", "bfbt"):
+            raise ValueError("unknown precond %r; use 'legacy' or 'bfbt'" % (self.precond,))
+        if vm == "const" and not (lo <= 1.0 <= hi):
+            raise ValueError("viscosity_model='const' has eta == 1 everywhere, so the "
+                             "requested eta window [%r, %r] would clip the whole field"
+                             % (lo, hi))
+        if float(self.Ra) <= 0.0:
+            raise ValueError("Ra must be positive (got %r)" % (self.Ra,))
+        if int(self.Nx) != self.Nx or int(self.Nx) < 4:
+            raise ValueError("Nx must be an integer >= 4 (got %r)" % (self.Nx,))
+        if int(self.Nz) != self.Nz or int(self.Nz) < 4:
+            raise ValueError("Nz must be an integer >= 4 (got %r)" % (self.Nz,))
+        if float(self.Lx) <= 0.0:
+            raise ValueError("Lx must be positive (got %r)" % (self.Lx,))
+        if float(self.deta_T) <= 0.0:
+            raise ValueError("deta_T must be positive (got %r)" % (self.deta_T,))
+        if int(self.lag) != self.lag or int(self.lag) < 1:
+            raise ValueError("lag must be an integer >= 1 (got %r)" % (self.lag,))
+        if int(self.max_K) != self.max_K or int(self.max_K) < 1:
+            raise ValueError("max_K must be an integer >= 1 (got %r)" % (self.max_K,))
+        if int(self.K) != self.K or int(self.K) < 0:
+            raise ValueError("K must be an integer >= 0 (got %r)" % (self.K,))
+        if float(self.defect_tol) <= 0.0 and int(self.K) < 1:
+            raise ValueError("defect_tol <= 0 selects the fixed-K route, so K must be "
+                             ">= 1 (got K=%r)" % (self.K,))
+        if self.K > self.max_K:
+            raise ValueError("K=%r must not exceed max_K=%r" % (self.K, self.max_K))
+        if vm != "const" and not bool(self.symmetrize):
+            warnings.warn("symmetrize=False on viscosity_model=%r removes the box "
+                          "projection" % (self.viscosity_model,), UserWarning,
+                          stacklevel=2)
+
+    def canonical(self) -> Dict[str, Any]:
+        d = self.as_dict()
+        d["viscosity_model"] = normalize_viscosity_model(self.viscosity_model)
+        d["compressibility"] = normalize_compressibility(self.compressibility)
+        return d
+
+    def as_dict(self) -> Dict[str, Any]:
+        return {f.name: getattr(self, f.name) for f in dataclasses.fields(self)}
+
+    def with_overrides(self, **kwargs) -> "ConvectionConfig":
+        return replace(self, **kwargs)
+
+    @classmethod
+    def field_names(cls) -> Tuple[str, ...]:
+        return tuple(f.name for f in dataclasses.fields(cls))

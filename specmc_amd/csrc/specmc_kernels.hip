@@ -1,0 +1,1464 @@
+// specmc_kernels.hip -- gfx950 kernels of SpecMC's time-step hot path.
+//
+// Every field is real (mirror-symmetric box): theta-type fields carry real
+// x-mode coefficients, stream-function-type fields purely imaginary ones
+// stored by their imaginary part.  All arrays are run-major; one launch
+// serves a bucket of B runs of one (Nx, Nz) shape.  The C ABI is declared in
+// include/specmc_hip.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#define SMX_VERSION 1
+
+static thread_local char g_err[512] = "";
+
+static int fail(const char *where, hipError_t e) {
+  snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+  return (int)e == 0 ? -1 : (int)e;
+}
+static int fail_msg(const char *msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return -1;
+}
+#define CHECK_LAUNCH(name)                                                   \
+  do {                                                                       \
+    hipError_t _e = hipGetLastError();                                       \
+    if (_e != hipSuccess) return fail(name, _e);                             \
+  } while (0)
+#define STREAM(s) ((hipStream_t)(s))
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// block reductions (wavefront shuffles, then LDS across wavefronts)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// sum over the block; every thread receives the result
+template <typename T>
+__device__ T block_sum(T v, T *red) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+template <typename T>
+__device__ T block_max(T v, T *red) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = red[0];
+  for (int i = 1; i < nw; ++i) s = fmax(s, red[i]);
+  return s;
+}
+template <typename T>
+__device__ T block_min(T v, T *red) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_min(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = red[0];
+  for (int i = 1; i < nw; ++i) s = fmin(s, red[i]);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// general strided batched GEMM (64x64 tile, K step 16, 256 threads, 4x4/thread)
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+  int M, N, K;
+  double alpha, beta;
+  const void *A; long ars, acs, sa1, sa2;
+  const void *B; long brs, bcs, sb1, sb2;
+  void *C; long crs, ccs, sc1, sc2;
+  int nz2;
+  const void *rscale; int rperiod;
+  const int *active; int mask_mode, rows_per_run;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ T As[16][65];
+  __shared__ T Bs[16][65];
+  const int z = blockIdx.z, z1 = z / g.nz2, z2 = z % g.nz2;
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  if (g.active) {
+    if (g.mask_mode == 2 && !g.active[z1]) return;
+    if (g.mask_mode == 3 && !g.active[z2]) return;
+    if (g.mask_mode == 1) {
+      int ilast = min(i0 + 63, g.M - 1);
+      int r0 = i0 / g.rows_per_run, r1 = ilast / g.rows_per_run;
+      bool any = false;
+      for (int r = r0; r <= r1; ++r) any |= (g.active[r] != 0);
+      if (!any) return;
+    }
+  }
+  const T *A = (const T *)g.A + z1 * g.sa1 + z2 * g.sa2;
+  const T *Bm = (const T *)g.B + z1 * g.sb1 + z2 * g.sb2;
+  T *C = (T *)g.C + z1 * g.sc1 + z2 * g.sc2;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0;
+  const bool a_kfast = (g.acs == 1), b_jfast = (g.bcs == 1);
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+#pragma unroll
+    for (int e0 = 0; e0 < 1024; e0 += 256) {
+      int e = e0 + tid, ii, kk;
+      if (a_kfast) { ii = e >> 4; kk = e & 15; } else { kk = e >> 6; ii = e & 63; }
+      int i = i0 + ii, k = k0 + kk;
+      As[kk][ii] = (i < g.M && k < g.K) ? A[(long)i * g.ars + (long)k * g.acs] : T(0);
+      int jj, kb;
+      if (b_jfast) { kb = e >> 6; jj = e & 63; } else { jj = e >> 4; kb = e & 15; }
+      int j = j0 + jj, kq = k0 + kb;
+      Bs[kb][jj] = (j < g.N && kq < g.K) ? Bm[(long)kq * g.brs + (long)j * g.bcs] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      T a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { a[q] = As[kk][ty + 16 * q]; b[q] = Bs[kk][tx + 16 * q]; }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = fma(a[p], b[q], acc[p][q]);
+    }
+    __syncthreads();
+  }
+  const T alpha = (T)g.alpha, beta = (T)g.beta;
+  const T *rs = (const T *)g.rscale;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    int i = i0 + ty + 16 * p;
+    if (i >= g.M) continue;
+    if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
+    T al = rs ? alpha * rs[i % g.rperiod] : alpha;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int j = j0 + tx + 16 * q;
+      if (j >= g.N) continue;
+      long o = (long)i * g.crs + (long)j * g.ccs;
+      T v = al * acc[p][q];
+      if (beta != T(0)) v += beta * C[o];
+      C[o] = v;
+    }
+  }
+}
+
+extern "C" int smx_version(void) { return SMX_VERSION; }
+extern "C" const char *smx_last_error(void) { return g_err; }
+
+extern "C" int smx_gemm(int prec, int M, int N, int K, double alpha, double beta,
+                        const void *A, long ars, long acs, long sa1, long sa2,
+                        const void *B, long brs, long bcs, long sb1, long sb2,
+                        void *C, long crs, long ccs, long sc1, long sc2,
+                        int nz1, int nz2, const void *rscale, int rperiod,
+                        const int *active, int mask_mode, int rows_per_run,
+                        void *stream) {
+  if (M <= 0 || N <= 0 || nz1 <= 0 || nz2 <= 0) return 0;
+  if (K < 0) return fail_msg("smx_gemm: negative K");
+  if (rscale && rperiod <= 0) return fail_msg("smx_gemm: rperiod must be > 0");
+  if (mask_mode == 1 && rows_per_run <= 0) return fail_msg("smx_gemm: rows_per_run");
+  if ((long)nz1 * nz2 > 65535L * 65535L) return fail_msg("smx_gemm: batch too large");
+  GemmArgs g{M, N, K, alpha, beta, A, ars, acs, sa1, sa2, B, brs, bcs, sb1, sb2,
+             C, crs, ccs, sc1, sc2, nz2, rscale, rperiod, active, mask_mode,
+             rows_per_run};
+  dim3 grid(cdiv(N, 64), cdiv(M, 64), nz1 * nz2);
+  if (grid.z > 65535) return fail_msg("smx_gemm: batch > 65535");
+  if (prec == 0) hipLaunchKernelGGL(k_gemm<double>, grid, dim3(256), 0, STREAM(stream), g);
+  else hipLaunchKernelGGL(k_gemm<float>, grid, dim3(256), 0, STREAM(stream), g);
+  CHECK_LAUNCH("smx_gemm");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batched LU (column-major, partial pivoting, LAPACK getrf semantics)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_factor(T *LU, int *piv, int N, int per_run,
+                                                   const int *active) {
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  T *A = LU + (long)m * N * N;
+  int *p = piv + (long)m * N;
+  __shared__ T rv[8];
+  __shared__ int ri[8];
+  __shared__ int s_piv;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int c = 0; c < N; ++c) {
+    // pivot: first index of max |a| over rows c..N-1 (idamax)
+    T best = T(-1);
+    int bi = N;
+    for (int r = c + tid; r < N; r += nt) {
+      T v = fabs(A[(long)c * N + r]);
+      if (v > best) { best = v; bi = r; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      T ov = __shfl_xor(best, o, 64);
+      int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      T b = rv[0];
+      int i = ri[0];
+      for (int w = 1; w < (nt >> 6); ++w)
+        if (rv[w] > b || (rv[w] == b && ri[w] < i)) { b = rv[w]; i = ri[w]; }
+      if (i >= N) i = c;
+      s_piv = i;
+      p[c] = i;
+    }
+    __syncthreads();
+    const int pr = s_piv;
+    if (pr != c)
+      for (int col = tid; col < N; col += nt) {
+        T t = A[(long)col * N + c];
+        A[(long)col * N + c] = A[(long)col * N + pr];
+        A[(long)col * N + pr] = t;
+      }
+    __syncthreads();
+    const T d = A[(long)c * N + c];
+    if (d != T(0)) {
+      const T inv = T(1) / d;
+      for (int r = c + 1 + tid; r < N; r += nt) A[(long)c * N + r] *= inv;
+    }
+    __syncthreads();
+    const int w = N - c - 1;
+    for (int e = tid; e < w * w; e += nt) {
+      int q = c + 1 + e / w, r = c + 1 + e % w;
+      A[(long)q * N + r] -= A[(long)c * N + r] * A[(long)q * N + c];
+    }
+    __syncthreads();
+  }
+}
+
+extern "C" int smx_lu_factor(int prec, int nmat, int N, void *LU, int *piv,
+                             int per_run, const int *active, void *stream) {
+  if (nmat <= 0) return 0;
+  if (N <= 0 || per_run <= 0) return fail_msg("smx_lu_factor: bad sizes");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_lu_factor<double>, dim3(nmat), dim3(256), 0, STREAM(stream),
+                       (double *)LU, piv, N, per_run, active);
+  else
+    hipLaunchKernelGGL(k_lu_factor<float>, dim3(nmat), dim3(256), 0, STREAM(stream),
+                       (float *)LU, piv, N, per_run, active);
+  CHECK_LAUNCH("smx_lu_factor");
+  return 0;
+}
+
+// one wavefront per system; LU column-major
+#define LUS_WAVES 4
+#define LUS_MAXN 512
+template <typename T>
+__global__ void __launch_bounds__(64 * LUS_WAVES)
+k_lu_solve(const T *LU, const int *piv, const T *rhs, T *x, int nmat, int N, int tau_zero,
+           T scale, const T *kscale, const T *bscale, int per_run, int shared,
+           const int *active) {
+  __shared__ T buf[LUS_WAVES][LUS_MAXN];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m = blockIdx.x * LUS_WAVES + wv;
+  const bool live = (m < nmat) && !(active && !active[m / per_run]);
+  T *y = buf[wv];
+  const long mi = live ? (shared ? (m % per_run) : m) : 0;
+  const T *A = LU + mi * N * N;
+  if (live) {
+    T s = kscale ? scale * kscale[m % per_run] : scale;
+    if (bscale) s = s * bscale[m / per_run];
+    for (int r = lane; r < N; r += 64) {
+      T v = rhs[(long)m * N + r] * s;
+      if (tau_zero && r >= N - 2) v = T(0);
+      y[r] = v;
+    }
+  }
+  __syncthreads();
+  if (live && lane == 0) {
+    const int *p = piv + mi * N;
+    for (int c = 0; c < N; ++c) {
+      int pc = p[c];
+      if (pc != c) { T t = y[c]; y[c] = y[pc]; y[pc] = t; }
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < N - 1; ++c) {
+    if (live) {
+      T yc = y[c];
+      for (int r = c + 1 + lane; r < N; r += 64) y[r] -= A[(long)c * N + r] * yc;
+    }
+    __syncthreads();
+  }
+  for (int c = N - 1; c >= 0; --c) {
+    if (live && lane == 0) y[c] = y[c] / A[(long)c * N + c];
+    __syncthreads();
+    if (live) {
+      T xc = y[c];
+      for (int r = lane; r < c; r += 64) y[r] -= A[(long)c * N + r] * xc;
+    }
+    __syncthreads();
+  }
+  if (live)
+    for (int r = lane; r < N; r += 64) x[(long)m * N + r] = y[r];
+}
+
+extern "C" int smx_lu_solve(int prec, int nmat, int N, const void *LU, const int *piv,
+                            const void *rhs, void *x, int tau_zero, double rhs_scale,
+                            const void *kscale, const void *bscale, int per_run, int shared,
+                            const int *active, void *stream) {
+  if (nmat <= 0) return 0;
+  if (N <= 0 || N > LUS_MAXN || per_run <= 0) return fail_msg("smx_lu_solve: bad sizes");
+  dim3 grid(cdiv(nmat, LUS_WAVES));
+  if (prec == 0)
+    hipLaunchKernelGGL(k_lu_solve<double>, grid, dim3(64 * LUS_WAVES), 0, STREAM(stream),
+                       (const double *)LU, piv, (const double *)rhs, (double *)x, nmat, N,
+                       tau_zero, rhs_scale, (const double *)kscale,
+                       (const double *)bscale, per_run, shared, active);
+  else
+    hipLaunchKernelGGL(k_lu_solve<float>, grid, dim3(64 * LUS_WAVES), 0, STREAM(stream),
+                       (const float *)LU, piv, (const float *)rhs, (float *)x, nmat, N,
+                       tau_zero, (float)rhs_scale, (const float *)kscale,
+                       (const float *)bscale, per_run, shared, active);
+  CHECK_LAUNCH("smx_lu_solve");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// operator assembly (column-major out[b][k][col*N + row])
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_tau_assemble(int kind, int Nk, int N, const T *D2, const T *kx, const T *lam,
+                               const T *Mb, const T *lm, const T *cg, const T *V0, const T *V1,
+                               T *out, const int *active) {
+  const int bk = blockIdx.y, b = bk / Nk, k = bk % Nk;
+  if (active && !active[b]) return;
+  const long NN = (long)N * N;
+  T *o = out + (long)bk * NN;
+  const T kk = kx[k] * kx[k];
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < NN; e += (long)gridDim.x * blockDim.x) {
+    int row = (int)(e % N), col = (int)(e / N);
+    T v;
+    if (kind == 0) {
+      if (row == N - 2) v = (col & 1) ? T(-1) : T(1);
+      else if (row == N - 1) v = T(1);
+      else v = D2[(long)row * N + col] - (row == col ? (kk + lam[b]) : T(0));
+    } else {
+      if (row == N - 2) v = V0[col];
+      else if (row == N - 1) v = V1[col];
+      else {
+        long rc = (long)row * N + col;
+        v = (lm[(long)b * NN + rc] - kk * Mb[(long)b * NN + rc]) + T(2) * kk * cg[(long)bk * NN + rc];
+      }
+    }
+    o[e] = v;
+  }
+}
+
+extern "C" int smx_tau_assemble(int prec, int kind, int B, int Nk, int N, const void *D2,
+                                const void *kx, const void *lam, const void *Mb,
+                                const void *lm, const void *cg, const void *V0,
+                                const void *V1, void *out, const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if ((long)B * Nk > 65535) return fail_msg("smx_tau_assemble: B*Nk > 65535");
+  dim3 grid(cdiv((long)N * N, 256 * 4), B * Nk);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_tau_assemble<double>, grid, dim3(256), 0, STREAM(stream), kind, Nk, N,
+                       (const double *)D2, (const double *)kx, (const double *)lam,
+                       (const double *)Mb, (const double *)lm, (const double *)cg,
+                       (const double *)V0, (const double *)V1, (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_tau_assemble<float>, grid, dim3(256), 0, STREAM(stream), kind, Nk, N,
+                       (const float *)D2, (const float *)kx, (const float *)lam,
+                       (const float *)Mb, (const float *)lm, (const float *)cg,
+                       (const float *)V0, (const float *)V1, (float *)out, active);
+  CHECK_LAUNCH("smx_tau_assemble");
+  return 0;
+}
+
+template <typename T>
+__global__ void k_cheb_mult(int N, const T *a, T *M, const int *active) {
+  const int b = blockIdx.y;
+  if (active && !active[b]) return;
+  const T *ab = a + (long)b * N;
+  T *Mo = M + (long)b * N * N;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)N * N;
+       e += (long)gridDim.x * blockDim.x) {
+    int p = (int)(e / N), n = (int)(e % N);
+    T s = T(0);
+    if (p >= n) s += T(0.5) * ab[p - n];
+    if (p + n < N) s += T(0.5) * ab[p + n];
+    if (p > 0 && n >= p) s += T(0.5) * ab[n - p];
+    Mo[e] = s;
+  }
+}
+
+extern "C" int smx_cheb_mult(int prec, int B, int N, const void *a, void *M,
+                             const int *active, void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(cdiv((long)N * N, 256 * 4), B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_cheb_mult<double>, grid, dim3(256), 0, STREAM(stream), N,
+                       (const double *)a, (double *)M, active);
+  else
+    hipLaunchKernelGGL(k_cheb_mult<float>, grid, dim3(256), 0, STREAM(stream), N,
+                       (const float *)a, (float *)M, active);
+  CHECK_LAUNCH("smx_cheb_mult");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// viscosity law on the padded half grid (one block per run)
+// params per run (double[6]): ln_deta, Ts, lo, hi, smooth_w, law
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double smooth5(double t) {
+  t = fmin(fmax(t, 0.0), 1.0);
+  return t * t * t * (t * (6.0 * t - 15.0) + 10.0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_eta_law(int nrow, int N, const T *theta, const T *z,
+                                                  const double *params, T *eta, double *counters,
+                                                  const int *active) {
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  __shared__ double red[16];
+  const double *pp = params + 6 * b;
+  const double L = pp[0], Ts = pp[1], lo = pp[2], hi = pp[3], w = pp[4];
+  const int law = (int)pp[5];
+  const double elo = lo * (1.0 - 1e-12), ehi = hi * (1.0 + 1e-12);
+  const long np = (long)nrow * N;
+  const T *th = theta + (long)b * np;
+  T *et = eta + (long)b * np;
+  double n_out = 0, n_clipT = 0, n_clipE = 0, exc = 0;
+  for (long e = threadIdx.x; e < np; e += blockDim.x) {
+    int j = (int)(e / N), q = (int)(e % N);
+    double wgt = (j == 0 || j == nrow - 1) ? 1.0 : 2.0;
+    double T_ = (1.0 - (double)z[q]) + (double)th[e];
+    exc = fmax(exc, fmax(0.0 - T_, T_ - 1.0));
+    if (T_ < 0.0 || T_ > 1.0) n_out += wgt;
+    if (T_ < 0.0 || T_ > 2.0) n_clipT += wgt;
+    double Tc = fmin(fmax(T_, 0.0), 2.0);
+    double v;
+    if (law == 1) v = exp(-L * (1.0 + Ts) * Tc / (Tc + Ts));
+    else v = exp(-L * Tc);
+    if (v < elo || v > ehi) n_clipE += wgt;
+    et[e] = (T)v;
+  }
+  n_out = block_sum(n_out, red);
+  n_clipT = block_sum(n_clipT, red);
+  n_clipE = block_sum(n_clipE, red);
+  exc = block_max(exc, red);
+  if (w > 0.0) {
+    // C^2 smooth clamp in ln(eta); n_clip_eta keeps the hard-clip count
+    const double ylo = log(lo), yhi = log(hi);
+    for (long e = threadIdx.x; e < np; e += blockDim.x) {
+      double y = log((double)et[e]);
+      double Y = ylo + (y - ylo) * smooth5((y - ylo) / w);
+      double yo = yhi - (yhi - Y) * smooth5((yhi - Y) / w);
+      et[e] = (T)exp(yo);
+    }
+  } else if (n_clipE > 0.0) {
+    for (long e = threadIdx.x; e < np; e += blockDim.x)
+      et[e] = (T)fmin(fmax((double)et[e], lo), hi);
+  }
+  if (threadIdx.x == 0) {
+    double *c = counters + 8 * b;
+    c[0] += n_out;
+    c[1] += n_clipT;
+    c[2] += n_clipE;
+    c[3] = fmax(c[3], fmax(exc, 0.0));
+  }
+}
+
+extern "C" int smx_eta_law(int prec, int B, int nrow, int N, const void *theta,
+                           const void *z, const double *params, void *eta,
+                           double *counters, const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if (prec == 0)
+    hipLaunchKernelGGL(k_eta_law<double>, dim3(B), dim3(1024), 0, STREAM(stream), nrow, N,
+                       (const double *)theta, (const double *)z, params, (double *)eta,
+                       counters, active);
+  else
+    hipLaunchKernelGGL(k_eta_law<float>, dim3(B), dim3(1024), 0, STREAM(stream), nrow, N,
+                       (const float *)theta, (const float *)z, params, (float *)eta, counters,
+                       active);
+  CHECK_LAUNCH("smx_eta_law");
+  return 0;
+}
+
+// guard + profile on the unpadded half grid (nrow = Nx/2 + 1)
+template <typename T>
+__global__ void __launch_bounds__(1024) k_eta_guard(int Nx, int N, T *eta, const double *params,
+                                                    T *prof, int profile, double *counters,
+                                                    const int *active) {
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  __shared__ double red[16];
+  const int nrow = Nx / 2 + 1;
+  const long np = (long)nrow * N;
+  T *et = eta + (long)b * np;
+  const double *pp = params + 6 * b;
+  const double lo = pp[2], hi = pp[3], w = pp[4];
+  double mn = 1e300;
+  for (long e = threadIdx.x; e < np; e += blockDim.x) mn = fmin(mn, (double)et[e]);
+  mn = block_min(mn, red);
+  if (!(mn > 0.0)) {
+    const double elo = lo * (1.0 - 1e-12), ehi = hi * (1.0 + 1e-12);
+    double nc = 0;
+    for (long e = threadIdx.x; e < np; e += blockDim.x) {
+      int j = (int)(e / N);
+      double wgt = (j == 0 || j == nrow - 1) ? 1.0 : 2.0;
+      double v = (double)et[e];
+      if (v < elo || v > ehi) nc += wgt;
+    }
+    nc = block_sum(nc, red);
+    __syncthreads();
+    if (w > 0.0) {
+      const double ylo = log(lo), yhi = log(hi);
+      for (long e = threadIdx.x; e < np; e += blockDim.x) {
+        double y = log((double)et[e]);
+        double Y = ylo + (y - ylo) * smooth5((y - ylo) / w);
+        double yo = yhi - (yhi - Y) * smooth5((yhi - Y) / w);
+        et[e] = (T)exp(yo);
+      }
+    } else if (nc > 0.0) {
+      for (long e = threadIdx.x; e < np; e += blockDim.x)
+        et[e] = (T)fmin(fmax((double)et[e], lo), hi);
+    }
+    if (threadIdx.x == 0) counters[8 * b + 4] += nc;
+    __syncthreads();
+  }
+  // profile per z node: reduce over the full Nx grid (mirror weights)
+  if (prof) {
+    for (int q = threadIdx.x; q < N; q += blockDim.x) {
+      double s = 0.0;
+      for (int j = 0; j < nrow; ++j) {
+        double v = (double)et[(long)j * N + q];
+        double f = (profile == 0) ? 1.0 / v : (profile == 1 ? v : log(v));
+        s += ((j == 0 || j == nrow - 1) ? 1.0 : 2.0) * f;
+      }
+      double mean = s / (double)Nx;
+      double r = (profile == 0) ? 1.0 / mean : (profile == 1 ? mean : exp(mean));
+      prof[(long)b * N + q] = (T)r;
+    }
+  }
+}
+
+extern "C" int smx_eta_guard_profile(int prec, int B, int Nx, int N, void *eta,
+                                     const double *params, void *prof, int profile,
+                                     double *counters, const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if (prec == 0)
+    hipLaunchKernelGGL(k_eta_guard<double>, dim3(B), dim3(1024), 0, STREAM(stream), Nx, N,
+                       (double *)eta, params, (double *)prof, profile, counters, active);
+  else
+    hipLaunchKernelGGL(k_eta_guard<float>, dim3(B), dim3(1024), 0, STREAM(stream), Nx, N,
+                       (float *)eta, params, (float *)prof, profile, counters, active);
+  CHECK_LAUNCH("smx_eta_guard_profile");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// pseudo-spectral x products (synthesis on the mirror half of the padded
+// grid, pointwise product, analysis), one block per (run, field, 16 columns)
+// y/out layout [B][Nk][nf][Q]; eta [B][nrow][Q]; So/Ce [nrow][Nk]; Ao/Ae [Nk][nrow]
+// ---------------------------------------------------------------------------
+#define XC 16
+template <typename T>
+__global__ void __launch_bounds__(256) k_xprod(int nf, int nodd, int Nk, int nrow, int Q,
+                                               const T *y, const T *fscale, const T *eta, T *out,
+                                               const T *So, const T *Ce, const T *Ao,
+                                               const T *Ae, const int *active) {
+  extern __shared__ unsigned char smem_raw[];
+  T *Ys = (T *)smem_raw;        // [Nk][XC]
+  T *Gs = Ys + Nk * XC;         // [nrow][XC]
+  const int b = blockIdx.z, f = blockIdx.y, q0 = blockIdx.x * XC;
+  if (active && !active[b]) return;
+  const bool odd = f < nodd;
+  const T *S = odd ? So : Ce;
+  const T *An = odd ? Ao : Ae;
+  const long rs = (long)nf * Q;  // stride between modes
+  const T *yb = y + (long)b * Nk * rs + (long)f * Q;
+  for (int e = threadIdx.x; e < Nk * XC; e += blockDim.x) {
+    int m = e / XC, c = e % XC, q = q0 + c;
+    T v = (q < Q) ? yb[(long)m * rs + q] : T(0);
+    Ys[e] = fscale ? fscale[(long)f * Nk + m] * v : v;
+  }
+  __syncthreads();
+  const T *eb = eta + (long)b * nrow * Q;
+  for (int e = threadIdx.x; e < nrow * XC; e += blockDim.x) {
+    int j = e / XC, c = e % XC, q = q0 + c;
+    const T *Sr = S + (long)j * Nk;
+    T s = 0;
+    for (int m = 0; m < Nk; ++m) s = fma(Sr[m], Ys[m * XC + c], s);
+    Gs[e] = (q < Q) ? s * eb[(long)j * Q + q] : T(0);
+  }
+  __syncthreads();
+  T *ob = out + (long)b * Nk * rs + (long)f * Q;
+  for (int e = threadIdx.x; e < Nk * XC; e += blockDim.x) {
+    int m = e / XC, c = e % XC, q = q0 + c;
+    if (q >= Q) continue;
+    const T *Ar = An + (long)m * nrow;
+    T s = 0;
+    for (int j = 0; j < nrow; ++j) s = fma(Ar[j], Gs[j * XC + c], s);
+    ob[(long)m * rs + q] = s;
+  }
+}
+
+extern "C" int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
+                         const void *y, const void *fscale, const void *eta, void *out,
+                         const void *So,
+                         const void *Ce, const void *Ao, const void *Ae, const int *active,
+                         void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(cdiv(Q, XC), nf, B);
+  size_t sh = (size_t)(Nk + nrow) * XC * (prec == 0 ? 8 : 4);
+  if (sh > 64 * 1024) return fail_msg("smx_xprod: grid too large for LDS");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_xprod<double>, grid, dim3(256), sh, STREAM(stream), nf, nodd, Nk,
+                       nrow, Q, (const double *)y, (const double *)fscale,
+                       (const double *)eta, (double *)out,
+                       (const double *)So, (const double *)Ce, (const double *)Ao,
+                       (const double *)Ae, active);
+  else
+    hipLaunchKernelGGL(k_xprod<float>, grid, dim3(256), sh, STREAM(stream), nf, nodd, Nk, nrow,
+                       Q, (const float *)y, (const float *)fscale, (const float *)eta,
+                       (float *)out,
+                       (const float *)So, (const float *)Ce, (const float *)Ao,
+                       (const float *)Ae, active);
+  CHECK_LAUNCH("smx_xprod");
+  return 0;
+}
+
+// advect product: y [B][Nk][2][N] (f=0 odd: k v, f=1 even: D v, values in z),
+// ux/uz [B][nrow][N]; out [B][Nk][N] = Ae( -(ux So y0 + uz Ce y1) [+ e1] [+ h] [+ e2] )
+template <typename T>
+__global__ void __launch_bounds__(256) k_advect(int Nk, int nrow, int N, const T *y,
+                                                const T *ks0, const T *ux,
+                                                const T *uz, const T *e1, const T *h,
+                                                const T *e2, T *out, const T *So, const T *Ce,
+                                                const T *Ae, const int *active) {
+  extern __shared__ unsigned char smem_raw[];
+  T *Y0 = (T *)smem_raw;     // [Nk][XC]
+  T *Y1 = Y0 + Nk * XC;      // [Nk][XC]
+  T *P = Y1 + Nk * XC;       // [nrow][XC]
+  const int b = blockIdx.y, q0 = blockIdx.x * XC;
+  if (active && !active[b]) return;
+  const T *yb = y + (long)b * Nk * 2 * N;
+  for (int e = threadIdx.x; e < Nk * XC; e += blockDim.x) {
+    int m = e / XC, c = e % XC, q = q0 + c;
+    T v0 = (q < N) ? yb[(long)m * 2 * N + q] : T(0);
+    Y0[e] = ks0 ? ks0[m] * v0 : v0;
+    Y1[e] = (q < N) ? yb[(long)m * 2 * N + N + q] : T(0);
+  }
+  __syncthreads();
+  const long gb = (long)b * nrow * N;
+  for (int e = threadIdx.x; e < nrow * XC; e += blockDim.x) {
+    int j = e / XC, c = e % XC, q = q0 + c;
+    T vx = 0, vz = 0;
+    const T *Sr = So + (long)j * Nk;
+    const T *Cr = Ce + (long)j * Nk;
+    for (int m = 0; m < Nk; ++m) {
+      vx = fma(Sr[m], Y0[m * XC + c], vx);
+      vz = fma(Cr[m], Y1[m * XC + c], vz);
+    }
+    T p = 0;
+    if (q < N) {
+      long o = gb + (long)j * N + q;
+      p = -(ux[o] * vx + uz[o] * vz);
+      if (e1) p = p + e1[o];
+      if (h) p = p + h[b];
+      if (e2) p = p + e2[o];
+    }
+    P[e] = p;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < Nk * XC; e += blockDim.x) {
+    int m = e / XC, c = e % XC, q = q0 + c;
+    if (q >= N) continue;
+    const T *Ar = Ae + (long)m * nrow;
+    T s = 0;
+    for (int j = 0; j < nrow; ++j) s = fma(Ar[j], P[j * XC + c], s);
+    out[(long)b * Nk * N + (long)m * N + q] = s;
+  }
+}
+
+extern "C" int smx_advect_prod(int prec, int B, int Nk, int nrow, int N, const void *y,
+                               const void *ks0, const void *ux, const void *uz, const void *e1, const void *h,
+                               const void *e2, void *out, const void *So, const void *Ce,
+                               const void *Ae, const int *active, void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(cdiv(N, XC), B);
+  size_t sh = (size_t)(2 * Nk + nrow) * XC * (prec == 0 ? 8 : 4);
+  if (sh > 64 * 1024) return fail_msg("smx_advect_prod: grid too large for LDS");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_advect<double>, grid, dim3(256), sh, STREAM(stream), Nk, nrow, N,
+                       (const double *)y, (const double *)ks0, (const double *)ux,
+                       (const double *)uz,
+                       (const double *)e1, (const double *)h, (const double *)e2,
+                       (double *)out, (const double *)So, (const double *)Ce,
+                       (const double *)Ae, active);
+  else
+    hipLaunchKernelGGL(k_advect<float>, grid, dim3(256), sh, STREAM(stream), Nk, nrow, N,
+                       (const float *)y, (const float *)ks0, (const float *)ux,
+                       (const float *)uz,
+                       (const float *)e1, (const float *)h, (const float *)e2, (float *)out,
+                       (const float *)So, (const float *)Ce, (const float *)Ae, active);
+  CHECK_LAUNCH("smx_advect_prod");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// elementwise
+// ---------------------------------------------------------------------------
+// out = (a * a_b) x + (b * b_b) y per run (a_b, b_b, y may be NULL)
+template <typename T>
+__global__ void k_axpby(long n, T a, const T *ab, const T *x, T b, const T *bb, const T *y,
+                        T *out, const int *active) {
+  const int r = blockIdx.y;
+  if (active && !active[r]) return;
+  const long off = (long)r * n;
+  const T ca = ab ? a * ab[r] : a;
+  const T cb = bb ? b * bb[r] : b;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    T v = ca * x[off + e];
+    if (y) v = v + cb * y[off + e];
+    out[off + e] = v;
+  }
+}
+extern "C" int smx_axpby(int prec, long n_per_run, int B, double a, const void *ab,
+                         const void *x, double b, const void *bb, const void *y, void *out,
+                         const int *active, void *stream) {
+  if (B <= 0 || n_per_run <= 0) return 0;
+  dim3 grid(min(cdiv(n_per_run, 256), 1024), B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_axpby<double>, grid, dim3(256), 0, STREAM(stream), n_per_run, a,
+                       (const double *)ab, (const double *)x, b, (const double *)bb,
+                       (const double *)y, (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, grid, dim3(256), 0, STREAM(stream), n_per_run, (float)a,
+                       (const float *)ab, (const float *)x, (float)b, (const float *)bb,
+                       (const float *)y, (float *)out, active);
+  CHECK_LAUNCH("smx_axpby");
+  return 0;
+}
+
+// out[b][k][j] = (alpha * ab[b]) * ks[k] * x[b][k][j]; tau entries zeroed when tau_zero
+template <typename T>
+__global__ void k_rowscale(int Nk, int N, const T *x, const T *ks, T alpha, const T *ab,
+                           int tau_zero, T *out, const int *active) {
+  const int b = blockIdx.y;
+  if (active && !active[b]) return;
+  const long n = (long)Nk * N;
+  const T al = ab ? alpha * ab[b] : alpha;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    int k = (int)(e / N), j = (int)(e % N);
+    T v = (tau_zero && j >= N - 2) ? T(0) : (ks ? ks[k] * al : al) * x[(long)b * n + e];
+    out[(long)b * n + e] = v;
+  }
+}
+extern "C" int smx_rowscale(int prec, int B, int Nk, int N, const void *x, const void *ks,
+                            double alpha, const void *ab, int tau_zero, void *out,
+                            const int *active, void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(cdiv((long)Nk * N, 256), B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_rowscale<double>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const double *)x, (const double *)ks, alpha, (const double *)ab,
+                       tau_zero, (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_rowscale<float>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const float *)x, (const float *)ks, (float)alpha, (const float *)ab,
+                       tau_zero, (float *)out, active);
+  CHECK_LAUNCH("smx_rowscale");
+  return 0;
+}
+
+// theta right-hand side (implicit_theta.py:385-449, heating.py:219-230,
+// dissipation.py:474-487): b = (b0 th - b1 thp) + k psi; tau entries zeroed;
+// b[0][0] += H; b[:, :N-2] += src
+template <typename T>
+__global__ void k_theta_rhs(int Nk, int N, const T *th, const T *thp, const T *psi,
+                            const T *kx, const T *b0, const T *b1, const T *H, const T *src,
+                            T *out, const int *active) {
+  const int b = blockIdx.y;
+  if (active && !active[b]) return;
+  const long n = (long)Nk * N, o = (long)b * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    int k = (int)(e / N), j = (int)(e % N);
+    T v = b0[b] * th[o + e];
+    if (b1[b] != T(0)) v = v - b1[b] * thp[o + e];
+    v = v + kx[k] * psi[o + e];
+    if (j >= N - 2) v = T(0);
+    if (e == 0 && H && H[b] != T(0)) v = v + H[b];
+    if (src && j < N - 2) v = v + src[o + e];
+    out[o + e] = v;
+  }
+}
+extern "C" int smx_theta_rhs(int prec, int B, int Nk, int N, const void *th, const void *thp,
+                             const void *psi, const void *kx, const void *b0, const void *b1,
+                             const void *H, const void *src, void *out, const int *active,
+                             void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(cdiv((long)Nk * N, 256), B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_theta_rhs<double>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const double *)th, (const double *)thp, (const double *)psi,
+                       (const double *)kx, (const double *)b0, (const double *)b1,
+                       (const double *)H, (const double *)src, (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_theta_rhs<float>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const float *)th, (const float *)thp, (const float *)psi,
+                       (const float *)kx, (const float *)b0, (const float *)b1,
+                       (const float *)H, (const float *)src, (float *)out, active);
+  CHECK_LAUNCH("smx_theta_rhs");
+  return 0;
+}
+
+// viscous dissipation on the padded half grid (dissipation.py:336-362):
+// phi = scale_b * 2 * eta * ((gxx^2 + gzz^2 + 2 gxz^2) - (1/3) tr^2), gzz = -gxx, tr = 0
+template <typename T>
+__global__ void k_phi_grid(long np, const T *gxx, const T *gxz, const T *eta, const T *scale,
+                           T *out, const int *active) {
+  const int b = blockIdx.y;
+  if (active && !active[b]) return;
+  const long o = (long)b * np;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+    T xx = gxx[o + e], zz = -xx, xz = gxz[o + e], tr = xx + zz;
+    T dev = (xx * xx + zz * zz + T(2) * xz * xz) - (T(1) / T(3)) * tr * tr;
+    T ph = eta ? T(2) * eta[o + e] * dev : T(2) * dev;
+    out[o + e] = scale ? scale[b] * ph : ph;
+  }
+}
+extern "C" int smx_phi_grid(int prec, int B, long np, const void *gxx, const void *gxz,
+                            const void *eta, const void *scale, void *out, const int *active,
+                            void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(min(cdiv(np, 256), 1024), B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_phi_grid<double>, grid, dim3(256), 0, STREAM(stream), np,
+                       (const double *)gxx, (const double *)gxz, (const double *)eta,
+                       (const double *)scale, (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_phi_grid<float>, grid, dim3(256), 0, STREAM(stream), np,
+                       (const float *)gxx, (const float *)gxz, (const float *)eta,
+                       (const float *)scale, (float *)out, active);
+  CHECK_LAUNCH("smx_phi_grid");
+  return 0;
+}
+
+// diagnostics (rbc_galerkin.py:439-453) from the unpadded half grids
+// ux, uz, th [B][Nx/2+1][N] and dth0 = D theta_0 [B][N]:
+// out[b] = Nu_top, Nu_bot, Nu_energy, urms, umax, amp
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag(int Nx, int N, const T *ux, const T *uz, const T *th,
+                                              const T *dth0, long ldd, const T *V0, const T *V1,
+                                              const T *wq, double *out, const int *active) {
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  __shared__ double red[16];
+  const int nrow = Nx / 2 + 1;
+  const long o = (long)b * nrow * N;
+  double ek = 0, en = 0, um = 0, am = 0;
+  for (int q = threadIdx.x; q < N; q += blockDim.x) {
+    double sk = 0, sn = 0;
+    for (int j = 0; j < nrow; ++j) {
+      double w = (j == 0 || j == nrow - 1) ? 1.0 : 2.0;
+      double a = (double)ux[o + (long)j * N + q], c = (double)uz[o + (long)j * N + q];
+      double t = (double)th[o + (long)j * N + q];
+      sk += w * (a * a + c * c);
+      sn += w * (c * t);
+      um = fmax(um, fmax(fabs(a), fabs(c)));
+      am = fmax(am, fabs(t));
+    }
+    ek += (double)wq[q] * (sk / Nx);
+    en += (double)wq[q] * (sn / Nx);
+  }
+  ek = block_sum(ek, red);
+  en = block_sum(en, red);
+  um = block_max(um, red);
+  am = block_max(am, red);
+  double t0 = 0, t1 = 0;
+  for (int q = threadIdx.x; q < N; q += blockDim.x) {
+    double d = (double)dth0[(long)b * ldd + q];
+    t1 += (double)V1[q] * d;
+    t0 += (double)V0[q] * d;
+  }
+  t0 = block_sum(t0, red);
+  t1 = block_sum(t1, red);
+  if (threadIdx.x == 0) {
+    double *r = out + 6 * b;
+    r[0] = 1.0 - t1;
+    r[1] = 1.0 - t0;
+    r[2] = 1.0 + en;
+    r[3] = sqrt(fmax(ek, 0.0));
+    r[4] = um;
+    r[5] = am;
+  }
+}
+extern "C" int smx_diag(int prec, int B, int Nx, int N, const void *ux, const void *uz,
+                        const void *th, const void *dth0, long ldd, const void *V0,
+                        const void *V1,
+                        const void *wq, double *out, const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if (prec == 0)
+    hipLaunchKernelGGL(k_diag<double>, dim3(B), dim3(256), 0, STREAM(stream), Nx, N,
+                       (const double *)ux, (const double *)uz, (const double *)th,
+                       (const double *)dth0, ldd, (const double *)V0, (const double *)V1,
+                       (const double *)wq, out, active);
+  else
+    hipLaunchKernelGGL(k_diag<float>, dim3(B), dim3(256), 0, STREAM(stream), Nx, N,
+                       (const float *)ux, (const float *)uz, (const float *)th,
+                       (const float *)dth0, ldd, (const float *)V0, (const float *)V1,
+                       (const float *)wq, out, active);
+  CHECK_LAUNCH("smx_diag");
+  return 0;
+}
+
+// out[b][k][N-2] = v[b][k] . V0, out[b][k][N-1] = v[b][k] . V1
+template <typename T>
+__global__ void k_tau_rows(int Nk, int N, const T *v, const T *V0, const T *V1, T *out,
+                           const int *active) {
+  const int b = blockIdx.y;
+  if (active && !active[b]) return;
+  const int k = blockIdx.x;
+  const T *vr = v + ((long)b * Nk + k) * N;
+  T s0 = 0, s1 = 0;
+  for (int j = threadIdx.x; j < N; j += 64) { s0 += vr[j] * V0[j]; s1 += vr[j] * V1[j]; }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (threadIdx.x == 0) {
+    out[((long)b * Nk + k) * N + N - 2] = s0;
+    out[((long)b * Nk + k) * N + N - 1] = s1;
+  }
+}
+extern "C" int smx_tau_rows(int prec, int B, int Nk, int N, const void *v, const void *V0,
+                            const void *V1, void *out, const int *active, void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(Nk, B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_tau_rows<double>, grid, dim3(64), 0, STREAM(stream), Nk, N,
+                       (const double *)v, (const double *)V0, (const double *)V1,
+                       (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_tau_rows<float>, grid, dim3(64), 0, STREAM(stream), Nk, N,
+                       (const float *)v, (const float *)V0, (const float *)V1, (float *)out,
+                       active);
+  CHECK_LAUNCH("smx_tau_rows");
+  return 0;
+}
+
+// Schur matvec combine (lagged_stokes.py:212-218) from truncated products
+// t [B][Nk][4][N] and the derivative products dt [B][Nk][3][N]:
+// dt0 = D2 t0, dt1 = D2 t1, dt3 = D t3.
+// out = (dt0 - k^2 t0) + dt1 + 2k^2 t2 - 4k dt3 ; tau rows from v.
+template <typename T>
+__global__ void k_stokes_combine(int Nk, int N, const T *t, const T *dt, const T *kx, const T *v,
+                                 const T *V0, const T *V1, T *out, const int *active) {
+  const int b = blockIdx.y, k = blockIdx.x;
+  if (active && !active[b]) return;
+  const T kk = kx[k] * kx[k], kv = kx[k];
+  const T *tr = t + ((long)b * Nk + k) * 4 * N;
+  const T *dr = dt + ((long)b * Nk + k) * 3 * N;
+  T *o = out + ((long)b * Nk + k) * N;
+  for (int j = threadIdx.x; j < N - 2; j += blockDim.x) {
+    T r = dr[j] - kk * tr[j];
+    r = r + dr[N + j];
+    r = r + T(2) * kk * tr[2 * N + j];
+    r = r - T(4) * kv * dr[2 * N + j];
+    o[j] = r;
+  }
+  const T *vr = v + ((long)b * Nk + k) * N;
+  T s0 = 0, s1 = 0;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) { s0 += vr[j] * V0[j]; s1 += vr[j] * V1[j]; }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (threadIdx.x == 0) { o[N - 2] = s0; o[N - 1] = s1; }
+}
+extern "C" int smx_stokes_combine(int prec, int B, int Nk, int N, const void *t,
+                                  const void *dt, const void *kx, const void *v,
+                                  const void *V0, const void *V1, void *out,
+                                  const int *active, void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(Nk, B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_stokes_combine<double>, grid, dim3(64), 0, STREAM(stream), Nk, N,
+                       (const double *)t, (const double *)dt, (const double *)kx,
+                       (const double *)v, (const double *)V0, (const double *)V1,
+                       (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_stokes_combine<float>, grid, dim3(64), 0, STREAM(stream), Nk, N,
+                       (const float *)t, (const float *)dt, (const float *)kx,
+                       (const float *)v, (const float *)V0, (const float *)V1, (float *)out,
+                       active);
+  CHECK_LAUNCH("smx_stokes_combine");
+  return 0;
+}
+
+// theta matvec combine (implicit_theta.py:354-383):
+// out = (a0 v - (d2v - k^2 v)) - adv ; tau rows from v.
+template <typename T>
+__global__ void k_theta_combine(int Nk, int N, const T *v, const T *d2v, const T *adv,
+                                const T *kx, const T *a0, const T *V0, const T *V1, T *out,
+                                const int *active) {
+  const int b = blockIdx.y, k = blockIdx.x;
+  if (active && !active[b]) return;
+  const T kk = kx[k] * kx[k], a = a0[b];
+  const long off = ((long)b * Nk + k) * N;
+  for (int j = threadIdx.x; j < N - 2; j += blockDim.x) {
+    T lap = d2v[off + j] - kk * v[off + j];
+    out[off + j] = (a * v[off + j] - lap) - adv[off + j];
+  }
+  T s0 = 0, s1 = 0;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) { s0 += v[off + j] * V0[j]; s1 += v[off + j] * V1[j]; }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (threadIdx.x == 0) { out[off + N - 2] = s0; out[off + N - 1] = s1; }
+}
+extern "C" int smx_theta_combine(int prec, int B, int Nk, int N, const void *v,
+                                 const void *d2v, const void *adv, const void *kx,
+                                 const void *a0, const void *V0, const void *V1, void *out,
+                                 const int *active, void *stream) {
+  if (B <= 0) return 0;
+  dim3 grid(Nk, B);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_theta_combine<double>, grid, dim3(64), 0, STREAM(stream), Nk, N,
+                       (const double *)v, (const double *)d2v, (const double *)adv,
+                       (const double *)kx, (const double *)a0, (const double *)V0,
+                       (const double *)V1, (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_theta_combine<float>, grid, dim3(64), 0, STREAM(stream), Nk, N,
+                       (const float *)v, (const float *)d2v, (const float *)adv,
+                       (const float *)kx, (const float *)a0, (const float *)V0,
+                       (const float *)V1, (float *)out, active);
+  CHECK_LAUNCH("smx_theta_combine");
+  return 0;
+}
+
+// per-run max |x| (written as double)
+template <typename T>
+__global__ void __launch_bounds__(1024) k_maxabs(long n, const T *x, double *out, const int *active) {
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  __shared__ double red[16];
+  double m = 0.0;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) m = fmax(m, fabs((double)x[(long)b * n + e]));
+  m = block_max(m, red);
+  if (threadIdx.x == 0) out[b] = m;
+}
+extern "C" int smx_maxabs(int prec, int B, long n_per_run, const void *x, double *out,
+                          const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if (prec == 0)
+    hipLaunchKernelGGL(k_maxabs<double>, dim3(B), dim3(1024), 0, STREAM(stream), n_per_run,
+                       (const double *)x, out, active);
+  else
+    hipLaunchKernelGGL(k_maxabs<float>, dim3(B), dim3(1024), 0, STREAM(stream), n_per_run,
+                       (const float *)x, out, active);
+  CHECK_LAUNCH("smx_maxabs");
+  return 0;
+}
+
+// dt = min(dtmax, clip(safety * h / max(umax, 1e-12), dtmin, dtmax));
+// params per run: safety, h, dtmin, dtmax
+__global__ void k_cfl(int B, const double *umax, const double *params, double *dt,
+                      const int *active) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || (active && !active[b])) return;
+  const double *p = params + 4 * b;
+  double v = fmax(umax[b], 1e-12);
+  double d = p[0] * p[1] / v;
+  d = fmin(fmax(d, p[2]), p[3]);
+  dt[b] = fmin(p[3], d);
+}
+extern "C" int smx_cfl_dt(int B, const double *umax, const double *params, double *dt,
+                          const int *active, void *stream) {
+  if (B <= 0) return 0;
+  hipLaunchKernelGGL(k_cfl, dim3(cdiv(B, 256)), dim3(256), 0, STREAM(stream), B, umax, params,
+                     dt, active);
+  CHECK_LAUNCH("smx_cfl_dt");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FGMRES state machine (krylov_fgmres.py:60-179), one solver per run.
+// ctl: int[10][B] (struct of arrays)
+//   0 mode (0 arnoldi, 1 cycle end, 2 true residual, 3 final residual, 4 done)
+//   1 j, 2 kend, 3 total, 4 info, 5 act_pre, 6 act_mv, 7 maxiter, 8 cycles
+// sc: double[8][B]: 0 bnorm, 1 tol, 2 beta, 3 res (true relative residual),
+//   4 last quasi-residual
+// H: [B][(m+1)*m] row-major (i, j) -> i*m + j
+// ---------------------------------------------------------------------------
+enum { FG_ARN = 0, FG_CYC = 1, FG_RES = 2, FG_FIN = 3, FG_DONE = 4 };
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_fg_init(int B, long n, int m, const T *b, T *x, T *Vb,
+                                                  T *g, int *ctl, double *sc, double rtol,
+                                                  double atol, int maxiter, const int *active) {
+  const int r = blockIdx.x;
+  __shared__ double red[16];
+  if (active && !active[r]) {
+    if (threadIdx.x == 0) { ctl[0 * B + r] = FG_DONE; ctl[5 * B + r] = 0; ctl[6 * B + r] = 0; }
+    return;
+  }
+  const T *br = b + (long)r * n;
+  double s = 0;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) { double v = (double)br[e]; s += v * v; }
+  s = block_sum(s, red);
+  const double beta = sqrt(s);
+  const double tol = fmax(atol, rtol * beta);
+  T *xr = x + (long)r * n;
+  T *v0 = Vb + (long)r * (m + 1) * n;
+  const bool done = (beta <= tol);
+  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+    xr[e] = T(0);
+    if (!done) v0[e] = (T)((double)br[e] / beta);
+  }
+  T *gr = g + (long)r * (m + 1);
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) gr[i] = T(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    gr[0] = (T)beta;
+    ctl[0 * B + r] = done ? FG_DONE : FG_ARN;
+    ctl[1 * B + r] = 0;
+    ctl[2 * B + r] = 0;
+    ctl[3 * B + r] = 0;
+    ctl[4 * B + r] = done ? 0 : 1;
+    ctl[5 * B + r] = 0;
+    ctl[6 * B + r] = 0;
+    ctl[7 * B + r] = maxiter;
+    ctl[8 * B + r] = 0;
+    sc[0 * B + r] = beta;
+    sc[1 * B + r] = tol;
+    sc[2 * B + r] = beta;
+    sc[3 * B + r] = done ? 0.0 : 1.0;
+    sc[4 * B + r] = beta;
+  }
+}
+
+extern "C" int smx_fg_init(int prec, int B, long n, int m, const void *b, void *x, void *Vb,
+                           void *g, int *ctl, double *sc, double rtol, double atol,
+                           int maxiter, const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if (prec == 0)
+    hipLaunchKernelGGL(k_fg_init<double>, dim3(B), dim3(1024), 0, STREAM(stream), B, n, m,
+                       (const double *)b, (double *)x, (double *)Vb, (double *)g, ctl, sc,
+                       rtol, atol, maxiter, active);
+  else
+    hipLaunchKernelGGL(k_fg_init<float>, dim3(B), dim3(1024), 0, STREAM(stream), B, n, m,
+                       (const float *)b, (float *)x, (float *)Vb, (float *)g, ctl, sc, rtol,
+                       atol, maxiter, active);
+  CHECK_LAUNCH("smx_fg_init");
+  return 0;
+}
+
+// cycle end: y = H[:k,:k]^{-1} g[:k] (upper triangular), x += Z^T y; then the
+// mode of every run decides the input of this round's operator apply:
+// arnoldi -> vin = V_j (preconditioner input); residual -> zout = x.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_fg_cycle(int B, long n, int m, T *x, const T *Zb,
+                                                   const T *Vb, T *H, T *g, T *y, T *vin,
+                                                   T *zout, int *ctl) {
+  const int r = blockIdx.x;
+  __shared__ T ys[1024];
+  int mode = ctl[0 * B + r];
+  if (mode == FG_CYC) {
+    const int k = ctl[2 * B + r];
+    T *Hr = H + (long)r * (m + 1) * m;
+    const T *gr = g + (long)r * (m + 1);
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < k; ++i) ys[i] = gr[i];
+      for (int c = k - 1; c >= 0; --c) {
+        ys[c] = ys[c] / Hr[c * m + c];
+        T yc = ys[c];
+        for (int i = 0; i < c; ++i) ys[i] -= yc * Hr[i * m + c];
+      }
+      for (int i = 0; i < k; ++i) y[(long)r * m + i] = ys[i];
+    }
+    __syncthreads();
+    T *xr = x + (long)r * n;
+    const T *Zr = Zb + (long)r * m * n;
+    for (long e = threadIdx.x; e < n; e += blockDim.x) {
+      T s = 0;
+      for (int i = 0; i < k; ++i) s = fma(Zr[(long)i * n + e], ys[i], s);
+      xr[e] = xr[e] + s;
+    }
+    const int info = ctl[4 * B + r];
+    mode = (info == 0 || info == 2) ? FG_FIN : FG_RES;
+    __syncthreads();
+    if (threadIdx.x == 0) ctl[0 * B + r] = mode;
+  }
+  const int act_pre = (mode == FG_ARN);
+  const int act_mv = (mode == FG_ARN || mode == FG_RES || mode == FG_FIN);
+  if (threadIdx.x == 0) { ctl[5 * B + r] = act_pre; ctl[6 * B + r] = act_mv; }
+  if (mode == FG_ARN) {
+    const int j = ctl[1 * B + r];
+    const T *vj = Vb + ((long)r * (m + 1) + j) * n;
+    for (long e = threadIdx.x; e < n; e += blockDim.x) vin[(long)r * n + e] = vj[e];
+  } else if (mode == FG_RES || mode == FG_FIN) {
+    const T *xr = x + (long)r * n;
+    for (long e = threadIdx.x; e < n; e += blockDim.x) zout[(long)r * n + e] = xr[e];
+  }
+}
+
+extern "C" int smx_fg_cycle_end(int prec, int B, long n, int m, void *x, const void *Zb,
+                                const void *Vb, void *H, void *g, void *y, void *vin,
+                                void *zout, int *ctl, void *stream) {
+  if (B <= 0) return 0;
+  if (m > 1024) return fail_msg("smx_fg_cycle_end: restart > 1024");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_fg_cycle<double>, dim3(B), dim3(1024), 0, STREAM(stream), B, n, m,
+                       (double *)x, (const double *)Zb, (const double *)Vb, (double *)H,
+                       (double *)g, (double *)y, (double *)vin, (double *)zout, ctl);
+  else
+    hipLaunchKernelGGL(k_fg_cycle<float>, dim3(B), dim3(1024), 0, STREAM(stream), B, n, m,
+                       (float *)x, (const float *)Zb, (const float *)Vb, (float *)H,
+                       (float *)g, (float *)y, (float *)vin, (float *)zout, ctl);
+  CHECK_LAUNCH("smx_fg_cycle_end");
+  return 0;
+}
+
+// post: arnoldi -> store Z_j, MGS, Givens, stop tests; residual -> restart
+// or finish; final residual -> record ||b - A x|| / ||b||.
+// E > 0: the operator output w lives in registers (E elements per thread);
+// E == 0: w is updated in place in global memory (long vectors).
+template <typename T, int E>
+struct WVec {
+  T v[E > 0 ? E : 1];
+  T *g;
+  long n;
+  int per, tid, nt;
+  __device__ void load(T *w, long n_, int tid_, int nt_) {
+    g = w; n = n_; tid = tid_; nt = nt_;
+    per = (int)((n + nt - 1) / nt);
+    if constexpr (E > 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        long e = tid + (long)i * nt;
+        v[i] = (i < per && e < n) ? w[e] : T(0);
+      }
+    }
+  }
+  __device__ __forceinline__ T get(int i, long e) const {
+    if constexpr (E > 0) return v[i]; else return g[e];
+  }
+  __device__ __forceinline__ void set(int i, long e, T x) {
+    if constexpr (E > 0) v[i] = x; else g[e] = x;
+  }
+  __device__ __forceinline__ int count() const { return E > 0 ? E : per; }
+};
+
+template <typename T, int E>
+__global__ void __launch_bounds__(1024) k_fg_post(int B, long n, int m, const T *bvec, T *w,
+                                                  const T *zin, T *Zb, T *Vb, T *H, T *cs,
+                                                  T *sn, T *g, int *ctl, double *sc) {
+  const int r = blockIdx.x;
+  const int mode = ctl[0 * B + r];
+  if (mode != FG_ARN && mode != FG_RES && mode != FG_FIN) return;
+  __shared__ double red[16];
+  __shared__ int s_next;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  WVec<T, E> W;
+  W.load(w + (long)r * n, n, tid, nt);
+  const int per = W.per;
+  const int cnt = W.count();
+  if (mode == FG_ARN) {
+    const int j = ctl[1 * B + r];
+    T *Vr = Vb + (long)r * (m + 1) * n;
+    T *Hr = H + (long)r * (m + 1) * m;
+    {
+      const T *zr = zin + (long)r * n;
+      T *Zj = Zb + ((long)r * m + j) * n;
+      for (long e = tid; e < n; e += nt) Zj[e] = zr[e];
+    }
+    for (int i = 0; i <= j; ++i) {
+      const T *vi = Vr + (long)i * n;
+      double s = 0;
+#pragma unroll
+      for (int q = 0; q < cnt; ++q) {
+        long e = tid + (long)q * nt;
+        if (q < per && e < n) s += (double)vi[e] * (double)W.get(q, e);
+      }
+      s = block_sum(s, red);
+      const T h = (T)s;
+#pragma unroll
+      for (int q = 0; q < cnt; ++q) {
+        long e = tid + (long)q * nt;
+        if (q < per && e < n) W.set(q, e, W.get(q, e) - h * vi[e]);
+      }
+      if (tid == 0) Hr[i * m + j] = h;
+    }
+    double s2 = 0;
+#pragma unroll
+    for (int q = 0; q < cnt; ++q) {
+      long e = tid + (long)q * nt;
+      if (q < per && e < n) { double x = (double)W.get(q, e); s2 += x * x; }
+    }
+    s2 = block_sum(s2, red);
+    const double hn = sqrt(s2);
+    if (tid == 0) {
+      T *csr = cs + (long)r * m, *snr = sn + (long)r * m, *gr = g + (long)r * (m + 1);
+      Hr[(j + 1) * m + j] = (T)hn;
+      for (int i = 0; i < j; ++i) {
+        T a = Hr[i * m + j], bb = Hr[(i + 1) * m + j];
+        T t = csr[i] * a + snr[i] * bb;
+        Hr[(i + 1) * m + j] = -snr[i] * a + csr[i] * bb;
+        Hr[i * m + j] = t;
+      }
+      // Givens (krylov_fgmres.py:42-57) for real entries
+      T a = Hr[j * m + j], bb = Hr[(j + 1) * m + j];
+      T aa = fabs(a), ab = fabs(bb), c, s, rr;
+      if (ab == T(0)) { c = T(1); s = T(0); rr = a; }
+      else if (aa == T(0)) { c = T(0); s = bb / ab; rr = ab; }
+      else {
+        T den = sqrt(aa * aa + ab * ab);
+        c = aa / den;
+        s = c * bb / a;
+        rr = c * (aa * aa + ab * ab) / a;
+      }
+      csr[j] = c;
+      snr[j] = s;
+      Hr[j * m + j] = rr;
+      Hr[(j + 1) * m + j] = T(0);
+      T t = c * gr[j] + s * gr[j + 1];
+      gr[j + 1] = -s * gr[j] + c * gr[j + 1];
+      gr[j] = t;
+      int total = ctl[3 * B + r] + 1;
+      ctl[3 * B + r] = total;
+      ctl[2 * B + r] = j + 1;
+      double rn = fabs((double)gr[j + 1]);
+      sc[4 * B + r] = rn;
+      int next;
+      if (rn <= sc[1 * B + r]) { ctl[4 * B + r] = 0; next = FG_CYC; }
+      else if (hn == 0.0) { ctl[4 * B + r] = 2; next = FG_CYC; }
+      else {
+        next = FG_ARN;
+        ctl[1 * B + r] = j + 1;
+        if (j + 1 == m) next = FG_CYC;
+        else if (total >= ctl[7 * B + r]) { ctl[4 * B + r] = 1; next = FG_CYC; }
+      }
+      ctl[0 * B + r] = next;
+      s_next = (rn <= sc[1 * B + r] || hn == 0.0) ? 0 : 1;
+    }
+    __syncthreads();
+    if (s_next) {
+      T *vn = Vr + (long)(j + 1) * n;
+      const T hv = (T)hn;
+#pragma unroll
+      for (int q = 0; q < cnt; ++q) {
+        long e = tid + (long)q * nt;
+        if (q < per && e < n) vn[e] = W.get(q, e) / hv;
+      }
+    }
+    return;
+  }
+  // residual modes: r = b - w
+  const T *br = bvec + (long)r * n;
+  double s2 = 0;
+#pragma unroll
+  for (int q = 0; q < cnt; ++q) {
+    long e = tid + (long)q * nt;
+    if (q < per && e < n) {
+      T x = br[e] - W.get(q, e);
+      W.set(q, e, x);
+      s2 += (double)x * (double)x;
+    }
+  }
+  s2 = block_sum(s2, red);
+  const double beta = sqrt(s2);
+  const double bn = fmax(sc[0 * B + r], 1e-300);
+  if (mode == FG_FIN) {
+    if (tid == 0) { sc[3 * B + r] = beta / bn; ctl[0 * B + r] = FG_DONE; }
+    return;
+  }
+  if (tid == 0) {
+    sc[2 * B + r] = beta;
+    sc[3 * B + r] = beta / bn;
+    int next;
+    if (beta <= sc[1 * B + r]) { ctl[4 * B + r] = 0; next = FG_DONE; }
+    else if (ctl[3 * B + r] >= ctl[7 * B + r]) next = FG_DONE;
+    else next = FG_ARN;
+    ctl[0 * B + r] = next;
+    s_next = (next == FG_ARN);
+  }
+  __syncthreads();
+  if (s_next) {
+    T *v0 = Vb + (long)r * (m + 1) * n;
+#pragma unroll
+    for (int q = 0; q < cnt; ++q) {
+      long e = tid + (long)q * nt;
+      if (q < per && e < n) v0[e] = (T)((double)W.get(q, e) / beta);
+    }
+    T *gr = g + (long)r * (m + 1);
+    for (int i = tid; i <= m; i += nt) gr[i] = T(0);
+    __syncthreads();
+    if (tid == 0) {
+      gr[0] = (T)beta;
+      ctl[1 * B + r] = 0;
+      ctl[8 * B + r] += 1;
+    }
+  }
+}
+
+template <typename T>
+static void launch_fg_post(int B, long n, int m, const void *b, void *w, const void *zin,
+                           void *Zb, void *Vb, void *H, void *cs, void *sn, void *g, int *ctl,
+                           double *sc, hipStream_t st) {
+#define FGP(E)                                                                              \
+  hipLaunchKernelGGL((k_fg_post<T, E>), dim3(B), dim3(1024), 0, st, B, n, m, (const T *)b,  \
+                     (T *)w, (const T *)zin, (T *)Zb, (T *)Vb, (T *)H, (T *)cs, (T *)sn,    \
+                     (T *)g, ctl, sc)
+  if (n <= 4L * 1024) FGP(4);
+  else if (n <= 8L * 1024) FGP(8);
+  else FGP(0);
+#undef FGP
+}
+
+extern "C" int smx_fg_post(int prec, int B, long n, int m, const void *b, void *w,
+                           const void *zin, void *Zb, void *Vb, void *H, void *cs, void *sn,
+                           void *g, int *ctl, double *sc, void *stream) {
+  if (B <= 0) return 0;
+  if (prec == 0) launch_fg_post<double>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, STREAM(stream));
+  else launch_fg_post<float>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, STREAM(stream));
+  CHECK_LAUNCH("smx_fg_post");
+  return 0;
+}
+
+__global__ void k_fg_count(int B, const int *ctl, int *out) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (int b = threadIdx.x; b < B; b += blockDim.x)
+    if (ctl[b] != FG_DONE) atomicAdd(&cnt, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = cnt;
+}
+extern "C" int smx_fg_count(int B, const int *ctl, int *out, void *stream) {
+  hipLaunchKernelGGL(k_fg_count, dim3(1), dim3(256), 0, STREAM(stream), B, ctl, out);
+  CHECK_LAUNCH("smx_fg_count");
+  return 0;
+}

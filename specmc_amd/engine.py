@@ -1,0 +1,754 @@
+"""Batched device engine: B runs of one ``(Nx, Nz, Lx)`` shape advance together.
+
+Every numerical operation of a step is a kernel of ``libspecmc_hip.so``; this
+module only sequences launches, owns the HBM buffers (torch tensors used as
+plain device memory) and reads back the few per-run scalars that decide
+control flow (FGMRES completion, CFL dt, memo hits).
+
+Real arithmetic on the mirror-symmetric subspace
+    With the box symmetry the theta coefficients are real and the stream
+    function, vorticity and Stokes unknowns purely imaginary
+    (stokes_coupled.py:78-112): ``b = i k Ra theta`` with real operators keeps
+    the whole Stokes Krylov process in ``i R^n``.  Stream-function-type
+    vectors are therefore stored by their imaginary part (the "s-part") and
+    every FGMRES inner product equals the reference's ``vdot`` exactly.
+
+Layouts (row-major, run-major)
+    spectral vector      [B][Nk][Nz]          (n = Nk*Nz per run)
+    padded half grid     [B][Nx+1][Nz]        (x = j*Lx/(2Nx), j = 0..Nx)
+    unpadded half grid   [B][Nx/2+1][Nz]
+    Stokes product grid  [B][Nx+1][Q]         (Q z-padded CGL nodes)
+    Stokes z fields      [B][Nk][4][Q]
+    Krylov bases         [B][m+1][n], [B][m][n]
+    LU factors           [B][Nk][Nz*Nz] column-major, pivots [B][Nk][Nz]
+"""
+import numpy as np
+import torch
+
+from . import runtime
+from .basis import shape_ops
+
+KIND_CONST, KIND_DEPTH, KIND_COUPLED = "const", "eta_z", "coupled"
+PROFILES = {"harm": 0, "diag": 1, "geom": 2}
+
+MEMO_RTOL = 1e-12          # coupled_model.py:327, 749-765
+
+
+class Solver:
+    """Device FGMRES workspace for B independent systems of size n
+    (krylov_fgmres.py:60-179 per run, driven in lockstep rounds)."""
+
+    def __init__(self, eng, n, m, maxiter, rtol):
+        self.e = eng
+        B, dt, dev = eng.B, eng.dtype, eng.dev
+        self.n, self.m, self.maxiter, self.rtol = int(n), int(m), int(maxiter), float(rtol)
+        z = lambda *s: torch.zeros(*s, dtype=dt, device=dev)
+        self.x, self.b, self.w = z(B, n), z(B, n), z(B, n)
+        self.vin, self.zbuf = z(B, n), z(B, n)
+        self.V, self.Z = z(B, m + 1, n), z(B, m, n)
+        self.H, self.cs, self.sn = z(B, (m + 1) * m), z(B, m), z(B, m)
+        self.g, self.y = z(B, m + 1), z(B, m)
+        self.ctl = torch.zeros(10, B, dtype=torch.int32, device=dev)
+        self.sc = torch.zeros(8, B, dtype=torch.float64, device=dev)
+        self.cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.poll = 1
+        self.rounds = 0
+
+    def act_pre(self):
+        return self.ctl[5]
+
+    def act_mv(self):
+        return self.ctl[6]
+
+    def solve(self, precond, op, active=None, maxiter=None):
+        """Run FGMRES on every active run; ``precond(vin, zout, mask)`` and
+        ``op(zin, wout, mask)`` apply M^{-1} and A to the masked runs."""
+        e, L = self.e, self.e.L
+        B, n, m, P = e.B, self.n, self.m, e.prec
+        s = e.stream()
+        mi = self.maxiter if maxiter is None else int(maxiter)
+        L.fg_init(P, B, n, m, e.p(self.b), e.p(self.x), e.p(self.V), e.p(self.g),
+                  e.p(self.ctl), e.p(self.sc), self.rtol, 0.0, mi, e.p(active), s)
+        rounds = 0
+        while True:
+            for _ in range(self.poll):
+                L.fg_cycle_end(P, B, n, m, e.p(self.x), e.p(self.Z), e.p(self.V),
+                               e.p(self.H), e.p(self.g), e.p(self.y), e.p(self.vin),
+                               e.p(self.zbuf), e.p(self.ctl), s)
+                precond(self.vin, self.zbuf, self.act_pre())
+                op(self.zbuf, self.w, self.act_mv())
+                L.fg_post(P, B, n, m, e.p(self.b), e.p(self.w), e.p(self.zbuf),
+                          e.p(self.Z), e.p(self.V), e.p(self.H), e.p(self.cs),
+                          e.p(self.sn), e.p(self.g), e.p(self.ctl), e.p(self.sc), s)
+                rounds += 1
+            L.fg_count(B, e.p(self.ctl), e.p(self.cnt), s)
+            if int(self.cnt.item()) == 0:
+                break
+        self.rounds += rounds
+        return rounds
+
+    def results(self):
+        """-> (iters[B], info[B], true_rel_res[B]) on the host."""
+        c = self.ctl.cpu().numpy()
+        r = self.sc.cpu().numpy()
+        return c[3].copy(), c[4].copy(), r[3].copy()
+
+
+class Engine:
+    """B runs of one kind and one shape.  ``params`` is a list of per-run
+    dicts with keys Ra, deta_T, Ts, eta_min, eta_max, eta_smooth,
+    internal_heating, dissipation_number (missing keys take the defaults)."""
+
+    def __init__(self, kind, Nx, Nz, Lx, params, prec=0, device=None,
+                 profile="harm", stokes_rtol=1e-8, stokes_restart=80,
+                 stokes_maxiter=800, theta_rtol=1e-10, theta_restart=40,
+                 theta_maxiter=200):
+        self.dev = runtime.require_device() if device is None else device
+        self.L = runtime.load()
+        self.kind = kind
+        self.prec = int(prec)
+        self.dtype = torch.float64 if self.prec == 0 else torch.float32
+        self.esize = 8 if self.prec == 0 else 4
+        self.ops = shape_ops(Nx, Nz, Lx)
+        o = self.ops
+        self.Nx, self.Nz, self.Nk, self.Q = o.Nx, o.Nz, o.Nk, o.Q
+        self.nrp, self.nru = o.nrp, o.nru
+        self.n = self.Nk * self.Nz
+        self.B = B = len(params)
+        self.t = o.device_tensors(self.dtype, self.dev)
+        self.profile = PROFILES[profile]
+        dt, dev, N, Nk, n = self.dtype, self.dev, self.Nz, self.Nk, self.n
+        z = lambda *s: torch.zeros(*s, dtype=dt, device=dev)
+        # per-run physics
+        self.params = [dict(p) for p in params]
+        P = self.params
+        self.Ra = torch.tensor([float(p.get("Ra", 1e4)) for p in P], dtype=dt, device=dev)
+        self.Hh = np.array([float(p.get("internal_heating", 0.0)) for p in P])
+        self.Dih = np.array([float(p.get("dissipation_number", 0.0)) for p in P])
+        self.H = torch.tensor(self.Hh, dtype=dt, device=dev)
+        self.diss_scale = torch.tensor(self.Dih / np.array([float(p.get("Ra", 1e4)) for p in P]),
+                                       dtype=dt, device=dev)
+        law = np.zeros((B, 6))
+        for i, p in enumerate(P):
+            Ts = float(p.get("Ts", 0.0))
+            law[i] = (np.log(float(p.get("deta_T", 1000.0))), Ts,
+                      float(p.get("eta_min", 1e-3)), float(p.get("eta_max", 1e3)),
+                      float(p.get("eta_smooth", 0.0)), 1.0 if Ts > 0.0 else 0.0)
+        self.lawh = law
+        self.law = torch.tensor(law, dtype=torch.float64, device=dev)
+        self.counters = torch.zeros(B, 8, dtype=torch.float64, device=dev)
+        # state
+        self.theta, self.theta_prev = z(B, n), z(B, n)
+        self.psi, self.psi_prev, self.vsol = z(B, n), z(B, n), z(B, n)
+        self.omega = z(B, n)
+        self.has_hist = np.zeros(B, dtype=bool)
+        self.dt_prev = np.zeros(B)
+        # scratch
+        self.zv = z(B, n * 4)
+        self.zv2 = z(B, n * 4)
+        self.gp1, self.gp2 = z(B, self.nrp * N), z(B, self.nrp * N)
+        self.gp3, self.gp4 = z(B, self.nrp * N), z(B, self.nrp * N)
+        self.ux_p, self.uz_p = z(B, self.nrp * N), z(B, self.nrp * N)
+        self.ux_u, self.uz_u = z(B, self.nru * N), z(B, self.nru * N)
+        self.th_u = z(B, self.nru * N)
+        self.s1, self.s2, self.s3 = z(B, n), z(B, n), z(B, n)
+        self.umax = torch.zeros(B, dtype=torch.float64, device=dev)
+        self.umax2 = torch.zeros(B, dtype=torch.float64, device=dev)
+        self.dtv = torch.zeros(B, dtype=torch.float64, device=dev)
+        self.cflp = torch.zeros(B, 4, dtype=torch.float64, device=dev)
+        self.diag_out = torch.zeros(B, 6, dtype=torch.float64, device=dev)
+        self.a0 = z(B)
+        self.b0 = z(B)
+        self.b1 = z(B)
+        self.hlam = z(B)
+        self._a0_fact = np.full(B, np.nan)
+        self.all_on = torch.ones(B, dtype=torch.int32, device=dev)
+        self.n_steps = np.zeros(B, dtype=np.int64)
+        self.n_stokes = np.zeros(B, dtype=np.int64)
+        # theta machinery (implicit BDF2) for eta_z / coupled; the explicit
+        # isoviscous step for const
+        if kind == KIND_CONST:
+            self._init_const()
+        else:
+            self.helm = z(B, Nk, N * N)
+            self.helm_piv = torch.zeros(B, Nk, N, dtype=torch.int32, device=dev)
+            self.tsol = Solver(self, n, theta_restart, theta_maxiter, theta_rtol)
+            self.d2v, self.adv = z(B, n), z(B, n)
+            self.src = z(B, n)
+            if kind == KIND_DEPTH:
+                self._init_depth()
+            elif kind == KIND_COUPLED:
+                self._init_coupled(stokes_rtol, stokes_restart, stokes_maxiter)
+            else:
+                raise ValueError("unknown engine kind %r" % (kind,))
+
+    # ------------------------------------------------------------- plumbing
+    def p(self, t, off=0):
+        if t is None:
+            return None
+        return t.data_ptr() + off * t.element_size()
+
+    def stream(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def gemm(self, M, N, K, A, ars, acs, Bm, brs, bcs, C, crs, ccs,
+             alpha=1.0, beta=0.0, nz1=1, nz2=1, sa=(0, 0), sb=(0, 0), sc=(0, 0),
+             aoff=0, boff=0, coff=0, rscale=None, rperiod=1, active=None,
+             mask_mode=0, rpr=1):
+        self.L.gemm(self.prec, M, N, K, alpha, beta,
+                    self.p(A, aoff), ars, acs, sa[0], sa[1],
+                    self.p(Bm, boff), brs, bcs, sb[0], sb[1],
+                    self.p(C, coff), crs, ccs, sc[0], sc[1],
+                    nz1, nz2, self.p(rscale), rperiod, self.p(active),
+                    mask_mode if active is not None else 0, rpr, self.stream())
+
+    def zrows(self, src, Mat, dst, K, O, src_ld, dst_ld, alpha=1.0, beta=0.0,
+              rscale=None, active=None, soff=0, doff=0):
+        """dst[(b,k)][o] = alpha rs(k) sum_i src[(b,k)][i] Mat[o][i] over all
+        B*Nk mode rows (a z-transform of every x-mode row)."""
+        self.gemm(self.B * self.Nk, O, K, src, src_ld, 1, Mat, 1, K, dst, dst_ld, 1,
+                  alpha=alpha, beta=beta, aoff=soff, coff=doff, rscale=rscale,
+                  rperiod=self.Nk, active=active, mask_mode=1, rpr=self.Nk)
+
+    def xleft(self, S, R, Kin, src, src_ld, src_bs, dst, dst_ld, dst_bs, C,
+              soff=0, doff=0, active=None, beta=0.0):
+        """per run b: dst[b][r][c] = sum_m S[r][m] src[b][m][c] (x-DFT)."""
+        self.gemm(R, C, Kin, S, Kin, 1, src, src_ld, 1, dst, dst_ld, 1, beta=beta,
+                  nz1=self.B, sb=(src_bs, 0), sc=(dst_bs, 0), boff=soff, coff=doff,
+                  active=active, mask_mode=2)
+
+    def green(self, src, dst, active=None):
+        """dst[b][k] = G_k src[b][k] (stokes_coupled.py:318-325)."""
+        N, n = self.Nz, self.n
+        self.gemm(self.B, N, N, src, n, 1, self.t["G"], 1, N, dst, n, 1,
+                  nz1=1, nz2=self.Nk, sa=(0, N), sb=(0, N * N), sc=(0, N),
+                  active=active, mask_mode=1, rpr=1)
+
+    # ------------------------------------------------------------ state I/O
+    def set_theta(self, th, idx=None):
+        """Upload real coefficient arrays ``th`` (B', Nk, Nz) (runs ``idx``)."""
+        a = torch.from_numpy(np.ascontiguousarray(np.real(th), dtype=np.float64))
+        a = a.reshape(-1, self.n).to(device=self.dev, dtype=self.dtype)
+        if idx is None:
+            self.theta.copy_(a)
+        else:
+            self.theta[torch.as_tensor(idx, device=self.dev)] = a
+
+    def get(self, t):
+        return t.detach().to("cpu", torch.float64).numpy().reshape(self.B, self.Nk, self.Nz)
+
+    def reset_history(self, idx=None):
+        if idx is None:
+            self.has_hist[:] = False
+        else:
+            self.has_hist[idx] = False
+
+    # ------------------------------------------------------- shared pieces
+    def project(self, src, dst, active=None):
+        """Vi (Re V src): project_mirror on real fields (convection_modes.py:134-153)."""
+        N = self.Nz
+        self.zrows(src, self.t["V"], self.s3, N, N, N, N, active=active)
+        self.zrows(self.s3, self.t["Vi"], dst, N, N, N, N, active=active)
+
+    def velocity_unpadded(self, psi, active=None):
+        """ux = grid(D psi), uz = grid(-i k psi) on the unpadded half grid
+        (rbc_galerkin.py:355-372)."""
+        N, n, nru = self.Nz, self.n, self.nru
+        t = self.t
+        self.zrows(psi, t["VD"], self.zv, N, N, N, N, active=active)
+        self.zrows(psi, t["V"], self.zv2, N, N, N, N, rscale=t["kx"], active=active)
+        self.xleft(t["So_u"], nru, self.Nk, self.zv, N, n, self.ux_u, N, nru * N, N,
+                   active=active)
+        self.xleft(t["Ce_u"], nru, self.Nk, self.zv2, N, n, self.uz_u, N, nru * N, N,
+                   active=active)
+
+    def cfl(self, safety, dtmax, dtmin, active=None):
+        """Per-run CFL dt of the current theta (rbc_galerkin.py:465-471,
+        cfl_policy.py:64-74).  Solves Stokes (memoised for the coupled model)."""
+        self.stokes(self.theta, active)
+        self.velocity_unpadded(self.psi, active)
+        L, B = self.L, self.B
+        nh = self.nru * self.Nz
+        L.maxabs(self.prec, B, nh, self.p(self.ux_u), self.p(self.umax), self.p(active), self.stream())
+        L.maxabs(self.prec, B, nh, self.p(self.uz_u), self.p(self.umax2), self.p(active), self.stream())
+        torch.maximum(self.umax, self.umax2, out=self.umax)
+        hc = self.ops.hcfl
+        s = np.broadcast_to(np.asarray(safety, dtype=np.float64), (B,))
+        pr = np.stack([s, np.full(B, hc), np.broadcast_to(dtmin, (B,)),
+                       np.broadcast_to(dtmax, (B,))], 1).astype(np.float64)
+        self.cflp.copy_(torch.from_numpy(pr))
+        L.cfl_dt(B, self.p(self.umax), self.p(self.cflp), self.p(self.dtv), self.p(active),
+                 self.stream())
+        return self.dtv.cpu().numpy().copy()
+
+    def diagnostics(self, active=None):
+        """(B, 6): Nu_top, Nu_bot, Nu_energy, urms, umax, amp of the current
+        theta (rbc_galerkin.py:439-453)."""
+        t, N, n = self.t, self.Nz, self.n
+        self.stokes(self.theta, active)
+        self.velocity_unpadded(self.psi, active)
+        self.zrows(self.theta, t["V"], self.zv, N, N, N, N, active=active)
+        self.xleft(t["Ce_u"], self.nru, self.Nk, self.zv, N, n, self.th_u, N,
+                   self.nru * N, N, active=active)
+        self.zrows(self.theta, t["D"], self.s1, N, N, N, N, active=active)
+        self.L.diag(self.prec, self.B, self.Nx, N, self.p(self.ux_u), self.p(self.uz_u),
+                    self.p(self.th_u), self.p(self.s1), n, self.p(t["V0"]), self.p(t["V1"]),
+                    self.p(t["wq"]), self.p(self.diag_out), self.p(active), self.stream())
+        return self.diag_out.cpu().numpy().copy()
+
+    # ------------------------------------------------- isoviscous (const)
+    def _init_const(self):
+        B, Nk, N, dev = self.B, self.Nk, self.Nz, self.dev
+        z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
+        # lam = 0 factors: one set per shape, shared by every run
+        self.z0 = z(1, Nk, N * N)
+        self.z0_piv = torch.zeros(1, Nk, N, dtype=torch.int32, device=dev)
+        zero = z(1)
+        self.L.tau_assemble(self.prec, 0, 1, Nk, N, self.p(self.t["D2"]), self.p(self.t["kx"]),
+                            self.p(zero), None, None, None, None, None, self.p(self.z0),
+                            None, self.stream())
+        self.L.lu_factor(self.prec, Nk, N, self.p(self.z0), self.p(self.z0_piv), Nk, None,
+                         self.stream())
+        self.zl = z(B, Nk, N * N)
+        self.zl_piv = torch.zeros(B, Nk, N, dtype=torch.int32, device=dev)
+        self.lam = z(B)
+        self._lam_fact = np.full(B, np.nan)
+        self.nc = z(B, self.n)
+        self.eg1 = z(B, self.nrp * N)
+
+    def _stokes_const(self, theta, active):
+        """omega = Z^{-1}(-i k Ra theta), psi = Z^{-1}(-omega) (rbc_galerkin.py:342-353)."""
+        L, B, Nk, N = self.L, self.B, self.Nk, self.Nz
+        s = self.stream()
+        L.lu_solve(self.prec, B * Nk, N, self.p(self.z0), self.p(self.z0_piv), self.p(theta),
+                   self.p(self.omega), 1, -1.0, self.p(self.t["kx"]), self.p(self.Ra), Nk, 1,
+                   self.p(active), s)
+        L.lu_solve(self.prec, B * Nk, N, self.p(self.z0), self.p(self.z0_piv),
+                   self.p(self.omega), self.p(self.psi), 1, -1.0, None, None, Nk, 1,
+                   self.p(active), s)
+
+    def _padded_velocity(self, psi, active):
+        """ux = synth_pad(D psi) (x-odd), uz = synth_pad(-i k psi) (x-even)."""
+        t, N, n, nrp = self.t, self.Nz, self.n, self.nrp
+        self.zrows(psi, t["VD"], self.zv, N, N, N, N, active=active)
+        self.zrows(psi, t["V"], self.zv2, N, N, N, N, rscale=t["kx"], active=active)
+        self.xleft(t["So_p"], nrp, self.Nk, self.zv, N, n, self.ux_p, N, nrp * N, N,
+                   active=active)
+        self.xleft(t["Ce_p"], nrp, self.Nk, self.zv2, N, n, self.uz_p, N, nrp * N, N,
+                   active=active)
+
+    def _phi_grid(self, psi, eta_c, out, active):
+        """(Di/Ra) Phi on the padded half grid (dissipation.py:313-362) for the
+        (extrapolated) psi and eta coefficients ``eta_c`` (None: eta = 1)."""
+        t, N, n, nrp = self.t, self.Nz, self.n, self.nrp
+        # exx = -k D psi (x-even), exz = 0.5 (D D psi + k^2 psi) (x-odd)
+        self.zrows(psi, t["D"], self.s1, N, N, N, N, active=active)
+        self.zrows(self.s1, t["D"], self.s2, N, N, N, N, active=active)
+        self.L.rowscale(self.prec, self.B, self.Nk, N, self.p(self.s1), self.p(t["kx"]), -1.0,
+                        None, 0, self.p(self.s1), self.p(active), self.stream())
+        self.L.rowscale(self.prec, self.B, self.Nk, N, self.p(psi), self.p(self.k2), 1.0,
+                        None, 0, self.p(self.s3), self.p(active), self.stream())
+        self.L.axpby(self.prec, n, self.B, 0.5, None, self.p(self.s2), 0.5, None,
+                     self.p(self.s3), self.p(self.s2), self.p(active), self.stream())
+        self.zrows(self.s1, t["V"], self.zv, N, N, N, N, active=active)
+        self.zrows(self.s2, t["V"], self.zv2, N, N, N, N, active=active)
+        self.xleft(t["Ce_p"], nrp, self.Nk, self.zv, N, n, self.gp1, N, nrp * N, N,
+                   active=active)
+        self.xleft(t["So_p"], nrp, self.Nk, self.zv2, N, n, self.gp2, N, nrp * N, N,
+                   active=active)
+        eta_g = None
+        if eta_c is not None:
+            self.zrows(eta_c, t["V"], self.zv, N, N, N, N, active=active)
+            self.xleft(t["Ce_p"], nrp, self.Nk, self.zv, N, n, self.gp3, N, nrp * N, N,
+                       active=active)
+            eta_g = self.gp3
+        self.L.phi_grid(self.prec, self.B, nrp * N, self.p(self.gp1), self.p(self.gp2),
+                        self.p(eta_g), self.p(self.diss_scale), self.p(out), self.p(active),
+                        self.stream())
+
+    def _step_const(self, dt, active):
+        """Explicit advection + BE diffusion step (rbc_galerkin.py:396-436)."""
+        L, t, B, Nk, N, n, nrp = self.L, self.t, self.B, self.Nk, self.Nz, self.n, self.nrp
+        s = self.stream()
+        self._stokes_const(self.theta, active)
+        self._padded_velocity(self.psi, active)
+        e2 = None
+        if np.any(self.Dih != 0.0):
+            self._phi_grid(self.psi, None, self.eg1, active)
+            e2 = self.eg1
+        # theta derivatives: thx = synth(i k theta) (odd), thz = synth(D theta)
+        self.zrows(self.theta, t["V_VD"], self.zv, N, 2 * N, N, 2 * N, active=active)
+        hh = self.H if np.any(self.Hh != 0.0) else None
+        L.advect_prod(self.prec, B, Nk, nrp, N, self.p(self.zv), self.p(t["kx"]),
+                      self.p(self.ux_p), self.p(self.uz_p), self.p(self.uz_p), self.p(hh),
+                      self.p(e2), self.p(self.s1), self.p(t["So_p"]), self.p(t["Ce_p"]),
+                      self.p(t["Ae_p"]), self.p(active), s)
+        self.zrows(self.s1, t["Vi"], self.nc, N, N, N, N, active=active)
+        # rhs = theta / dt + Nc ; out = Z_lam^{-1}(-rhs), lam = 1/dt
+        inv_dt = 1.0 / np.asarray(dt, dtype=np.float64)
+        self.lam.copy_(torch.from_numpy(inv_dt).to(self.dtype))
+        refac = np.where(self._lam_fact != inv_dt)[0]
+        if active is not None:
+            act = active.cpu().numpy().astype(bool)
+            refac = refac[act[refac]]
+        if len(refac):
+            mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
+            mask[torch.as_tensor(refac, device=self.dev)] = 1
+            L.tau_assemble(self.prec, 0, B, Nk, N, self.p(t["D2"]), self.p(t["kx"]),
+                           self.p(self.lam), None, None, None, None, None, self.p(self.zl),
+                           self.p(mask), s)
+            L.lu_factor(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv), Nk,
+                        self.p(mask), s)
+            self._lam_fact[refac] = inv_dt[refac]
+        L.axpby(self.prec, n, B, 1.0, self.p(self.lam), self.p(self.theta), 1.0, None,
+                self.p(self.nc), self.p(self.s1), self.p(active), s)
+        L.lu_solve(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv), self.p(self.s1),
+                   self.p(self.theta), 1, -1.0, None, None, Nk, 0, self.p(active), s)
+
+    # ---------------------------------------------------- depth-only eta(z)
+    def _reduced_op_factor(self, prof_c, lu, piv, active):
+        """LU of Lm_k M + 2k^2 [D,[D,M]] G_k with tau rows v(0), v(1) for the
+        Chebyshev profile ``prof_c`` [B][N] of every run (vv_stokes.py:266-307,
+        bfbt.py:157-181)."""
+        L, t, B, Nk, N = self.L, self.t, self.B, self.Nk, self.Nz
+        s = self.stream()
+        NN = N * N
+        M0, T1, T2, T3, CM = self.rb_M0, self.rb_T1, self.rb_T2, self.rb_T3, self.rb_C
+        L.cheb_mult(self.prec, B, N, self.p(prof_c), self.p(M0), self.p(active), s)
+        mm = lambda A, sa, Bm, sb, C: self.gemm(N, N, N, A, N, 1, Bm, N, 1, C, N, 1, nz1=B,
+                                                sa=(sa, 0), sb=(sb, 0), sc=(NN, 0),
+                                                active=active, mask_mode=2)
+        mm(t["D2"], 0, M0, NN, T1)           # D2 M
+        mm(t["D"], 0, M0, NN, T2)            # D M
+        mm(T2, NN, t["D"], 0, T3)            # (D M) D
+        mm(M0, NN, t["D2"], 0, CM)           # M D2
+        L.axpby(self.prec, NN, B, 1.0, None, self.p(T1), -2.0, None, self.p(T3), self.p(T3),
+                self.p(active), s)
+        L.axpby(self.prec, NN, B, 1.0, None, self.p(T3), 1.0, None, self.p(CM), self.p(CM),
+                self.p(active), s)           # Comm = (D2 M - 2 D M D) + M D2
+        self.gemm(N, N, N, CM, N, 1, t["G"], N, 1, self.rb_CG, N, 1, nz1=B, nz2=Nk,
+                  sa=(NN, 0), sb=(0, NN), sc=(Nk * NN, NN), active=active, mask_mode=2)
+        L.tau_assemble(self.prec, 1, B, Nk, N, None, self.p(t["kx"]), None, self.p(M0),
+                       self.p(T1), self.p(self.rb_CG), self.p(t["V0"]), self.p(t["V1"]),
+                       self.p(lu), self.p(active), s)
+        L.lu_factor(self.prec, B * Nk, N, self.p(lu), self.p(piv), Nk, self.p(active), s)
+
+    def _alloc_reduced(self):
+        B, Nk, N, dev = self.B, self.Nk, self.Nz, self.dev
+        z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
+        self.rb_M0, self.rb_T1, self.rb_T2 = z(B, N * N), z(B, N * N), z(B, N * N)
+        self.rb_T3, self.rb_C = z(B, N * N), z(B, N * N)
+        self.rb_CG = z(B, Nk, N * N)
+
+    def _init_depth(self):
+        B, Nk, N, dev = self.B, self.Nk, self.Nz, self.dev
+        z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
+        self._alloc_reduced()
+        self.k2 = self.t["kx"] * self.t["kx"]
+        o = self.ops
+        # eta(z) profile, window-clipped, per run (convection_modes.py:86-110)
+        prof = np.zeros((B, N))
+        self.eta_grid_h = np.zeros((B, N))
+        for i in range(B):
+            lo, hi, w = self.lawh[i, 2], self.lawh[i, 3], self.lawh[i, 4]
+            eg = np.exp(-self.lawh[i, 0] * (1.0 + (0.0 - 1.0) * o.z))
+            elo, ehi = lo * (1.0 - 1e-12), hi * (1.0 + 1e-12)
+            if w > 0.0:
+                ylo, yhi = np.log(lo), np.log(hi)
+                y = np.log(eg)
+                sm = lambda q: (lambda c: c * c * c * (c * (6.0 * c - 15.0) + 10.0))(np.clip(q, 0.0, 1.0))
+                Y = ylo + (y - ylo) * sm((y - ylo) / w)
+                eg = np.exp(yhi - (yhi - Y) * sm((yhi - Y) / w))
+            elif np.any((eg < elo) | (eg > ehi)):
+                eg = np.clip(eg, lo, hi)
+            self.eta_grid_h[i] = eg
+            prof[i] = o.Vi @ eg
+        self.eta_prof_c = torch.tensor(prof, dtype=self.dtype, device=dev)
+        ec = np.zeros((B, Nk, N))
+        ec[:, 0, :] = prof
+        self.eta_c_depth = torch.tensor(ec.reshape(B, -1), dtype=self.dtype, device=dev)
+        self.vv_lu = z(B, Nk, N * N)
+        self.vv_piv = torch.zeros(B, Nk, N, dtype=torch.int32, device=dev)
+        self._reduced_op_factor(self.eta_prof_c, self.vv_lu, self.vv_piv, None)
+
+    def _stokes_depth(self, theta, active):
+        """u = A_k^{-1}(i k Ra theta), psi = G_k u, omega = -u (vv_stokes.py:309-321)."""
+        B, Nk, N = self.B, self.Nk, self.Nz
+        self.L.lu_solve(self.prec, B * Nk, N, self.p(self.vv_lu), self.p(self.vv_piv),
+                        self.p(theta), self.p(self.vsol), 1, 1.0, self.p(self.t["kx"]),
+                        self.p(self.Ra), Nk, 0, self.p(active), self.stream())
+        self.green(self.vsol, self.psi, active)
+
+    # ------------------------------------------------------ coupled eta(T)
+    def _init_coupled(self, rtol, restart, maxiter):
+        B, Nk, N, Q, dev = self.B, self.Nk, self.Nz, self.Q, self.dev
+        z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
+        self._alloc_reduced()
+        self.k2 = self.t["kx"] * self.t["kx"]
+        self.eta_p = z(B, self.nrp * N)        # padded law field
+        self.eta_u = z(B, self.nru * N)        # eta on the Nx grid (last_eta)
+        self.eta_c = z(B, self.n)              # its coefficients
+        self.eta_S = z(B, self.nrp * Q)        # eta on the Stokes product grid
+        self.prof = z(B, N)
+        self.prof_c = z(B, N)
+        self.bf_lu = z(B, Nk, N * N)
+        self.bf_piv = torch.zeros(B, Nk, N, dtype=torch.int32, device=dev)
+        self.yz = z(B, Nk * 4 * Q)
+        self.tz = z(B, Nk * 4 * Q)
+        self.tt = z(B, Nk * 4 * N)
+        self.dtt = z(B, Nk * 3 * N)
+        self.wg = z(B, self.n)
+        self.ssol = Solver(self, self.n, restart, maxiter, rtol)
+        fs = torch.ones(4, Nk, dtype=self.dtype, device=dev)
+        fs[1] = 2.0 * self.k2
+        fs[3] = self.t["kx"]
+        self.fscale = fs.contiguous()
+        self.memo_X = z(B, self.n)
+        self.memo_new = z(B, self.n)
+        self.memo_valid = np.zeros(B, dtype=bool)
+        self.memo_hits = np.zeros(B, dtype=np.int64)
+        self.bfbt_iters = np.zeros(B, dtype=np.int64)
+        self.last_bfbt_iters = np.zeros(B, dtype=np.int64)
+        self.last_bfbt_info = np.zeros(B, dtype=np.int64)
+        self.last_bfbt_res = np.zeros(B)
+        self.n_rescue = np.zeros(B, dtype=np.int64)
+
+    def _eta_of(self, theta, active):
+        """eta from theta: padded synthesis, law, window, truncation to Nk
+        modes, Nx-grid synthesis, positivity guard (coupled_model.py:401-496),
+        the BFBT profile, the coefficients and the Stokes-grid values."""
+        L, t, B, Nk, N, n, Q = self.L, self.t, self.B, self.Nk, self.Nz, self.n, self.Q
+        nrp, nru = self.nrp, self.nru
+        s = self.stream()
+        self.zrows(theta, t["V"], self.zv, N, N, N, N, active=active)
+        self.xleft(t["Ce_p"], nrp, Nk, self.zv, N, n, self.gp1, N, nrp * N, N, active=active)
+        L.eta_law(self.prec, B, nrp, N, self.p(self.gp1), self.p(t["z"]), self.p(self.law),
+                  self.p(self.eta_p), self.p(self.counters), self.p(active), s)
+        self.xleft(t["Ae_p"], Nk, nrp, self.eta_p, N, nrp * N, self.s1, N, n, N, active=active)
+        self.xleft(t["Ce_u"], nru, Nk, self.s1, N, n, self.eta_u, N, nru * N, N, active=active)
+        L.eta_guard_profile(self.prec, B, self.Nx, N, self.p(self.eta_u), self.p(self.law),
+                            self.p(self.prof), self.profile, self.p(self.counters),
+                            self.p(active), s)
+        self.xleft(t["Ae_u"], Nk, nru, self.eta_u, N, nru * N, self.s2, N, n, N, active=active)
+        self.zrows(self.s2, t["Vi"], self.eta_c, N, N, N, N, active=active)
+        # eta on the z-padded Stokes product grid
+        self.zrows(self.eta_c, t["Vp"], self.yz, N, Q, N, Q, active=active)
+        self.xleft(t["Ce_p"], nrp, Nk, self.yz, Q, Nk * Q, self.eta_S, Q, nrp * Q, Q,
+                   active=active)
+
+    def schur_matvec(self, v, out, active):
+        """Coupled Schur operator (FreeSchurOperator.matvec, lagged_stokes.py:202-220)
+        with the eta products evaluated pseudo-spectrally."""
+        L, t, B, Nk, N, n, Q = self.L, self.t, self.B, self.Nk, self.Nz, self.n, self.Q
+        s = self.stream()
+        self.green(v, self.wg, active)
+        # z-values on the padded product grid: f0 = v, (f1, f2, f3) = (w, D2 w, D w)
+        self.zrows(v, t["Vp"], self.yz, N, Q, N, 4 * Q, active=active)
+        self.zrows(self.wg, t["VpD2_VpD"], self.yz, N, 3 * Q, N, 4 * Q, doff=Q, active=active)
+        L.xprod(self.prec, B, 4, 3, Nk, self.nrp, Q, self.p(self.yz), self.p(self.fscale),
+                self.p(self.eta_S), self.p(self.tz), self.p(t["So_p"]), self.p(t["Ce_p"]),
+                self.p(t["Ao_p"]), self.p(t["Ae_p"]), self.p(active), s)
+        # truncation to Nz Chebyshev modes, then D2 t0, D2 t1, D t3
+        self.gemm(B * Nk * 4, N, Q, self.tz, Q, 1, t["Tp"], 1, Q, self.tt, N, 1,
+                  active=active, mask_mode=1, rpr=Nk * 4)
+        for f_in, f_out, Mat in ((0, 0, "D2"), (1, 1, "D2"), (3, 2, "D")):
+            self.gemm(Nk, N, N, self.tt, 4 * N, 1, t[Mat], 1, N, self.dtt, 3 * N, 1,
+                      nz1=B, sa=(Nk * 4 * N, 0), sc=(Nk * 3 * N, 0), aoff=f_in * N,
+                      coff=f_out * N, active=active, mask_mode=2)
+        L.stokes_combine(self.prec, B, Nk, N, self.p(self.tt), self.p(self.dtt), self.p(t["kx"]),
+                         self.p(v), self.p(t["V0"]), self.p(t["V1"]), self.p(out),
+                         self.p(active), s)
+
+    def _stokes_coupled(self, theta, active):
+        """eta_of + BFBT preconditioner + FGMRES on the Schur system + unpack
+        (coupled_model.py:588-723, precond='bfbt')."""
+        L, t, B, Nk, N, n = self.L, self.t, self.B, self.Nk, self.Nz, self.n
+        s = self.stream()
+        self._eta_of(theta, active)
+        self.gemm(B, N, N, self.prof, N, 1, t["Qprof"], 1, N, self.prof_c, N, 1,
+                  active=active, mask_mode=1, rpr=1)
+        self._reduced_op_factor(self.prof_c, self.bf_lu, self.bf_piv, active)
+        sv = self.ssol
+        L.rowscale(self.prec, B, Nk, N, self.p(theta), self.p(t["kx"]), 1.0, self.p(self.Ra), 1,
+                   self.p(sv.b), self.p(active), s)
+
+        def precond(vin, zout, mask):
+            L.lu_solve(self.prec, B * Nk, N, self.p(self.bf_lu), self.p(self.bf_piv),
+                       self.p(vin), self.p(zout), 1, 1.0, None, None, Nk, 0, self.p(mask), s)
+
+        def op(zin, wout, mask):
+            self.schur_matvec(zin, wout, mask)
+
+        sv.solve(precond, op, active)
+        it, info, res = sv.results()
+        act = np.ones(B, bool) if active is None else active.cpu().numpy().astype(bool)
+        self.bfbt_iters[act] += it[act]
+        self.last_bfbt_iters[act] = it[act]
+        self.last_bfbt_info[act] = info[act]
+        self.last_bfbt_res[act] = res[act]
+        bad = act & (info != 0) & ~(res <= sv.rtol)
+        if np.any(bad):
+            self.n_rescue[bad] += 1
+            self._rescue(theta, np.where(bad)[0])
+        if active is None:
+            self.vsol.copy_(sv.x)
+        else:
+            m = active.bool()
+            self.vsol[m] = sv.x[m]
+        self.green(self.vsol, self.psi, active)
+
+    def _rescue(self, theta, idx):
+        raise RuntimeError(
+            "BFBT FGMRES did not converge for run(s) %s (iters %s, true residual %s); "
+            "the dense rescue route is not available on the device"
+            % (list(idx), list(self.last_bfbt_iters[idx]), list(self.last_bfbt_res[idx])))
+
+    def stokes(self, theta, active=None):
+        """(psi, v = -omega) of ``theta`` for the active runs into self.psi /
+        self.vsol (and self.omega for const)."""
+        if self.kind == KIND_CONST:
+            self._stokes_const(theta, active)
+        elif self.kind == KIND_DEPTH:
+            self._stokes_depth(theta, active)
+        else:
+            self._stokes_coupled_memo(theta, active)
+        act = np.ones(self.B, bool) if active is None else active.cpu().numpy().astype(bool)
+        self.n_stokes[act] += 1
+
+    def _stokes_coupled_memo(self, theta, active):
+        """Reuse the last solve when the rfft signature V theta matches to 1e-12
+        relative (coupled_model.py:725-765)."""
+        B, N = self.B, self.Nz
+        act = np.ones(B, bool) if active is None else active.cpu().numpy().astype(bool)
+        self.zrows(theta, self.t["V"], self.memo_new, N, N, N, N)
+        need = act.copy()
+        if np.any(self.memo_valid & act):
+            self.L.axpby(self.prec, self.n, B, 1.0, None, self.p(self.memo_new), -1.0, None,
+                         self.p(self.memo_X), self.p(self.s2), None, self.stream())
+            self.L.maxabs(self.prec, B, self.n, self.p(self.s2), self.p(self.umax), None,
+                          self.stream())
+            self.L.maxabs(self.prec, B, self.n, self.p(self.memo_X), self.p(self.umax2), None,
+                          self.stream())
+            d = self.umax.cpu().numpy()
+            c = np.maximum(self.umax2.cpu().numpy(), 1e-300)
+            hit = act & self.memo_valid & (d <= MEMO_RTOL * c)
+            self.memo_hits[hit] += 1
+            need = act & ~hit
+        if np.any(need):
+            mask = None if np.all(need) else torch.as_tensor(need.astype(np.int32), device=self.dev)
+            self._stokes_coupled(theta, mask)
+            m = torch.as_tensor(need, device=self.dev)
+            self.memo_X[m] = self.memo_new[m]
+            self.memo_valid |= need
+
+    # ------------------------------------------------- BDF2 implicit theta
+    def theta_matvec(self, v, out, active):
+        """a0 v - (D2 v - k^2 v) - advect(v) with tau rows (implicit_theta.py:296-383)."""
+        L, t, B, Nk, N = self.L, self.t, self.B, self.Nk, self.Nz
+        s = self.stream()
+        self.zrows(v, t["V_VD"], self.zv, N, 2 * N, N, 2 * N, active=active)
+        L.advect_prod(self.prec, B, Nk, self.nrp, N, self.p(self.zv), self.p(t["kx"]),
+                      self.p(self.ux_p), self.p(self.uz_p), None, None, None, self.p(self.s1),
+                      self.p(t["So_p"]), self.p(t["Ce_p"]), self.p(t["Ae_p"]), self.p(active), s)
+        self.zrows(self.s1, t["Vi"], self.adv, N, N, N, N, active=active)
+        self.zrows(v, t["D2"], self.d2v, N, N, N, N, active=active)
+        L.theta_combine(self.prec, B, Nk, N, self.p(v), self.p(self.d2v), self.p(self.adv),
+                        self.p(t["kx"]), self.p(self.a0), self.p(t["V0"]), self.p(t["V1"]),
+                        self.p(out), self.p(active), s)
+
+    def _set_step(self, dt):
+        """BDF2 / BE coefficients per run (implicit_theta.py:215-238)."""
+        dt = np.asarray(dt, dtype=np.float64)
+        a0 = np.empty(self.B)
+        b0 = np.empty(self.B)
+        b1 = np.zeros(self.B)
+        for i in range(self.B):
+            if not self.has_hist[i]:
+                a0[i] = b0[i] = 1.0 / dt[i]
+            else:
+                r = dt[i] / self.dt_prev[i]
+                a0[i] = (1.0 + 2.0 * r) / ((1.0 + r) * dt[i])
+                b0[i] = (1.0 + r) / dt[i]
+                b1[i] = r * r / ((1.0 + r) * dt[i])
+        return a0, b0, b1
+
+    def _step_implicit(self, dt, active):
+        """RBCImplicitTheta.step (implicit_theta.py:523-535, 608-620)."""
+        L, t, B, Nk, N, n = self.L, self.t, self.B, self.Nk, self.Nz, self.n
+        s = self.stream()
+        act = np.ones(B, bool) if active is None else active.cpu().numpy().astype(bool)
+        dt = np.broadcast_to(np.asarray(dt, dtype=np.float64), (B,)).copy()
+        th = torch.empty_like(self.theta)
+        self.project(self.theta, th, active)             # th = project(that)
+        t2 = torch.empty_like(self.theta)
+        self.project(th, t2, active)                     # projected again
+        self.stokes(t2, active)                          # psi_n (memo hit)
+        psi_n = self.psi.clone()
+        # extrapolated velocity psi* = 2 psi^n - psi^{n-1}
+        ca = torch.tensor(np.where(self.has_hist, 2.0, 1.0), dtype=self.dtype, device=self.dev)
+        cb = torch.tensor(np.where(self.has_hist, -1.0, 0.0), dtype=self.dtype, device=self.dev)
+        psx = self.s2
+        L.axpby(self.prec, n, B, 1.0, self.p(ca), self.p(psi_n), 1.0, self.p(cb),
+                self.p(self.psi_prev), self.p(psx), self.p(active), s)
+        psx = psx.clone()
+        self._padded_velocity(psx, active)
+        src = None
+        if np.any(self.Dih != 0.0):
+            eta_c = self.eta_c if self.kind == KIND_COUPLED else self.eta_c_depth
+            self._phi_grid(psx, eta_c, self.gp4, active)
+            self.xleft(t["Ae_p"], Nk, self.nrp, self.gp4, N, self.nrp * N, self.s3, N, n, N,
+                       active=active)
+            self.zrows(self.s3, t["Vi"], self.src, N, N, N, N, active=active)
+            src = self.src
+        a0, b0, b1 = self._set_step(dt)
+        self.a0.copy_(torch.from_numpy(a0).to(self.dtype))
+        self.b0.copy_(torch.from_numpy(b0).to(self.dtype))
+        self.b1.copy_(torch.from_numpy(b1).to(self.dtype))
+        sv = self.tsol
+        hh = self.H if np.any(self.Hh != 0.0) else None
+        L.theta_rhs(self.prec, B, Nk, N, self.p(t2), self.p(self.theta_prev), self.p(psx),
+                    self.p(t["kx"]), self.p(self.b0), self.p(self.b1), self.p(hh), self.p(src),
+                    self.p(sv.b), self.p(active), s)
+        # Helmholtz factors (a0 + k^2 - D2) per run, refreshed when a0 changes
+        refac = np.where(act & (self._a0_fact != a0))[0]
+        if len(refac):
+            mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
+            mask[torch.as_tensor(refac, device=self.dev)] = 1
+            self.hlam.copy_(self.a0)
+            L.tau_assemble(self.prec, 0, B, Nk, N, self.p(t["D2"]), self.p(t["kx"]),
+                           self.p(self.hlam), None, None, None, None, None, self.p(self.helm),
+                           self.p(mask), s)
+            L.lu_factor(self.prec, B * Nk, N, self.p(self.helm), self.p(self.helm_piv), Nk,
+                        self.p(mask), s)
+            self._a0_fact[refac] = a0[refac]
+
+        def precond(vin, zout, mask):
+            L.lu_solve(self.prec, B * Nk, N, self.p(self.helm), self.p(self.helm_piv),
+                       self.p(vin), self.p(zout), 1, -1.0, None, None, Nk, 0, self.p(mask), s)
+
+        def op(zin, wout, mask):
+            self.theta_matvec(zin, wout, mask)
+
+        sv.solve(precond, op, active)
+        self.last_theta = sv.results()
+        self.project(sv.x, self.theta, active)           # output projected
+        # history (theta^{n-1} = th, dt_prev, psi^{n-1} = psi_n)
+        if active is None:
+            self.theta_prev.copy_(th)
+            self.psi_prev.copy_(psi_n)
+        else:
+            m = active.bool()
+            self.theta_prev[m] = th[m]
+            self.psi_prev[m] = psi_n[m]
+        self.dt_prev[act] = dt[act]
+        self.has_hist[act] = True
+
+    def step(self, dt, active=None):
+        """Advance every active run by its own dt (numpy array or scalar)."""
+        dt = np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.B,)).copy()
+        if self.kind == KIND_CONST:
+            self._step_const(dt, active)
+        else:
+            self._step_implicit(dt, active)
+        act = np.ones(self.B, bool) if active is None else active.cpu().numpy().astype(bool)
+        self.n_steps[act] += 1

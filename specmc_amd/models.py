@@ -1,0 +1,258 @@
+"""Device-backed model classes with the reference's model protocol.
+
+``api.build_model`` returns one of
+
+* ``RBC``              -- ``viscosity_model='const'``  (rbc_galerkin.py:258-497)
+* ``RBCImplicitTheta`` -- ``viscosity_model='eta_z'``  (implicit_theta.py:538-638)
+* ``RBCCoupled``       -- ``'eta_T'`` / ``'arrhenius'`` (coupled_model.py:123-821)
+
+and every one of them offers what ``api`` / ``cfl_policy`` / ``bench_ic`` and
+user scripts call (§ "drop-in boundary" of the design notes): ``step``,
+``cfl_dt``, ``stokes``, ``velocity``, ``diagnostics``, ``seed``, ``coeffs``,
+``grid``, ``describe``, ``reset_history`` and the attributes ``x, z, kx, Nk,
+Nx, Nz, Lx, V, Vi, D, wq, n_steps``.
+
+The state crosses the boundary as a host numpy array (the reference's
+convention); the device copy is reused as long as the caller hands back the
+array the model returned last, so a march makes no host-to-device copy per
+step.  The stepper runs on the HIP device only: the states must be
+mirror-symmetric (real x-mode coefficients up to roundoff), which every
+scenario of the reference starts from and the box projection maintains.
+``seed`` / ``coeffs`` / ``grid`` are host-side input/output helpers (the
+rfft conventions of rbc_galerkin.py:322-330), not part of the step.
+"""
+import numpy as np
+import torch
+
+from . import _numerics as NUM
+from .engine import Engine, KIND_CONST, KIND_DEPTH, KIND_COUPLED
+
+SYM_TOL = 1e-10
+
+
+class _ThetaInfo:
+    """Last implicit theta solve (the reference's ``model.theta`` fields)."""
+
+    def __init__(self, rtol, restart):
+        self.rtol, self.restart = rtol, restart
+        self.last_iters = 0
+        self.last_info = 0
+        self.last_res = 0.0
+        self.n_nonconv = 0
+        self.n_solves = 0
+        self.scheme = "bdf2"
+
+
+class _DeviceModel:
+    kind = None
+    prec = 0
+
+    def __init__(self, Ra=1e4, Nx=64, Nz=48, Lx=2.0, deta_T=1.0,
+                 internal_heating=0.0, dissipation_number=0.0,
+                 eta_min=1e-3, eta_max=1e3, eta_smooth=0.0, arrhenius_Ts=0.0,
+                 precond="bfbt", symmetrize=True, stokes_maxiter=None,
+                 **extra):
+        self.Ra, self.Nx, self.Nz, self.Lx = float(Ra), int(Nx), int(Nz), float(Lx)
+        self.internal_heating = float(internal_heating)
+        self.dissipation_number = float(dissipation_number)
+        self.deta_T = float(deta_T)
+        self.symmetrize = bool(symmetrize)
+        self.extra = dict(extra)
+        params = dict(Ra=self.Ra, deta_T=self.deta_T, Ts=float(arrhenius_Ts),
+                      eta_min=float(eta_min), eta_max=float(eta_max),
+                      eta_smooth=float(eta_smooth),
+                      internal_heating=self.internal_heating,
+                      dissipation_number=self.dissipation_number)
+        kw = {}
+        if stokes_maxiter is not None:
+            kw["stokes_maxiter"] = int(stokes_maxiter)
+        self.eng = Engine(self.kind, self.Nx, self.Nz, self.Lx, [params],
+                          prec=self.prec, **kw)
+        o = self.eng.ops
+        self.kx, self.Nk, self.x, self.z = o.kx, o.Nk, o.x, o.z
+        self.V, self.Vi, self.D, self.wq, self.Np = o.V, o.Vi, o.D, o.wq, 2 * self.Nx
+        self._dev_key = None
+        self.theta = _ThetaInfo(NUM.THETA_RTOL, NUM.THETA_RESTART)
+
+    # ---- counters
+    @property
+    def n_steps(self):
+        return int(self.eng.n_steps[0])
+
+    @property
+    def n_stokes(self):
+        return int(self.eng.n_stokes[0])
+
+    # ---- host transforms (rbc_galerkin.py:317-330)
+    def grid(self, cf):
+        return np.fft.irfft((self.V @ np.asarray(cf).T).T * self.Nx, n=self.Nx, axis=0)
+
+    def coeffs(self, g):
+        A = np.fft.rfft(np.asarray(g), axis=0) / self.Nx
+        return (self.Vi @ A.T).T
+
+    def vol_mean(self, g):
+        return float(np.sum(self.wq * np.asarray(g).real.mean(axis=0)))
+
+    def seed(self, amp=1e-3, m=1, n=1):
+        X, Z = np.meshgrid(self.x, self.z, indexing="ij")
+        return self.coeffs(amp * np.cos(m * np.pi * X) * np.sin(n * np.pi * Z))
+
+    # ---- state transfer
+    def _upload(self, that):
+        a = np.asarray(that)
+        if a.shape != (self.Nk, self.Nz):
+            raise ValueError("state must have shape (Nk, Nz) = (%d, %d), got %r"
+                             % (self.Nk, self.Nz, a.shape))
+        key = a.tobytes()
+        if key == self._dev_key:
+            return
+        if np.iscomplexobj(a):
+            scale = max(float(np.abs(a).max()), 1e-300)
+            if float(np.abs(a.imag).max()) > SYM_TOL * scale:
+                raise ValueError(
+                    "the device stepper carries mirror-symmetric states only "
+                    "(real x-mode coefficients); this state has a relative "
+                    "antisymmetric part of %.2e"
+                    % (float(np.abs(a.imag).max()) / scale))
+        self.eng.set_theta(np.real(a)[None])
+        self._dev_key = key
+
+    def _download(self):
+        out = self.eng.get(self.eng.theta)[0]
+        if self.kind == KIND_CONST:
+            out = out.astype(complex)
+        self._dev_key = out.tobytes()
+        return out
+
+    # ---- protocol
+    def cfl_dt(self, that, safety=0.5, dtmax=5e-3, dtmin=1e-10):
+        """dt = clip(safety min(Lx/Nx, dz_min) / max|u|, dtmin, dtmax)
+        (rbc_galerkin.py:465-471)."""
+        self._upload(that)
+        return float(self.eng.cfl(safety, dtmax, dtmin)[0])
+
+    def step(self, that, dt):
+        self._upload(that)
+        self.eng.step(float(dt))
+        self._after_step()
+        return self._download()
+
+    def _after_step(self):
+        if self.kind != KIND_CONST:
+            it, info, res = self.eng.last_theta
+            th = self.theta
+            th.last_iters, th.last_info, th.last_res = int(it[0]), int(info[0]), float(res[0])
+            th.n_solves += 1
+            if info[0] != 0:
+                th.n_nonconv += 1
+
+    def stokes(self, that):
+        """(psi, omega) coefficients of ``that`` (complex, (Nk, Nz))."""
+        self._upload(that)
+        e = self.eng
+        e.stokes(e.theta)
+        psi = 1j * e.get(e.psi)[0]
+        if self.kind == KIND_CONST:
+            om = 1j * e.get(e.omega)[0]
+        else:
+            om = -1j * e.get(e.vsol)[0]
+        return psi, om
+
+    def velocity(self, psi):
+        """(ux, uz) on the (Nx, Nz) grid for stream-function coefficients."""
+        e = self.eng
+        p = np.asarray(psi)
+        a = torch.from_numpy(np.ascontiguousarray(np.imag(p), dtype=np.float64))
+        e.psi.copy_(a.reshape(1, -1).to(device=e.dev, dtype=e.dtype))
+        e.velocity_unpadded(e.psi)
+        return self._mirror(e.ux_u, odd=True), self._mirror(e.uz_u, odd=False)
+
+    def _mirror(self, half, odd):
+        h = half.detach().to("cpu", torch.float64).numpy().reshape(self.Nx // 2 + 1, self.Nz)
+        full = np.empty((self.Nx, self.Nz))
+        full[: self.Nx // 2 + 1] = h
+        tail = h[1: self.Nx // 2][::-1]
+        full[self.Nx // 2 + 1:] = -tail if odd else tail
+        return full
+
+    def diagnostics(self, that):
+        """Nu_top, Nu_bot, Nu_energy, urms, umax, amp (rbc_galerkin.py:439-453)."""
+        self._upload(that)
+        d = self.eng.diagnostics()[0]
+        return dict(Nu_top=float(d[0]), Nu_bot=float(d[1]), Nu_energy=float(d[2]),
+                    urms=float(d[3]), umax=float(d[4]), amp=float(d[5]))
+
+    def reset_history(self):
+        self.eng.reset_history()
+
+    def project_mirror(self, that):
+        X = (self.V @ np.asarray(that).T).T
+        return (self.Vi @ X.real.T).T
+
+    def describe(self):
+        e = self.eng
+        c = e.counters.cpu().numpy()[0]
+        d = dict(device="hip", kind=self.kind, Nx=self.Nx, Nz=self.Nz, Lx=self.Lx,
+                 Ra=self.Ra, deta_T=self.deta_T, internal_heating=self.internal_heating,
+                 dissipation_number=self.dissipation_number, n_steps=self.n_steps,
+                 n_stokes=self.n_stokes, precision="fp64" if self.prec == 0 else "fp32")
+        if self.kind != KIND_CONST:
+            th = self.theta
+            d.update(theta_scheme=NUM.THETA_SCHEME, theta_rtol=NUM.THETA_RTOL,
+                     n_theta_nonconv=th.n_nonconv, last_theta_iters=th.last_iters)
+        if self.kind == KIND_COUPLED:
+            d.update(precond="bfbt", n_T_out_of_range=int(c[0]), n_clip_T=int(c[1]),
+                     n_clip_eta=int(c[2]), max_T_excursion=float(c[3]),
+                     n_clip_eta_proj=int(c[4]), n_memo_hits=int(e.memo_hits[0]),
+                     n_bfbt_iters=int(e.bfbt_iters[0]),
+                     last_bfbt_iters=int(e.last_bfbt_iters[0]),
+                     last_bfbt_info=int(e.last_bfbt_info[0]),
+                     last_bfbt_true_res=float(e.last_bfbt_res[0]),
+                     n_bfbt_rescue=int(e.n_rescue[0]))
+        return d
+
+
+class RBC(_DeviceModel):
+    """Isoviscous model: explicit dealiased advection + BE diffusion
+    (rbc_galerkin.py:258-497)."""
+    kind = KIND_CONST
+
+
+class RBCImplicitTheta(_DeviceModel):
+    """Depth-only eta(z) Stokes + BDF2 implicit theta (implicit_theta.py:538-638)."""
+    kind = KIND_DEPTH
+
+    @property
+    def eta_grid(self):
+        return self.eng.eta_grid_h[0].copy()
+
+
+class RBCCoupled(_DeviceModel):
+    """eta(T(x,z,t)) coupled Stokes through the BFBT-preconditioned FGMRES
+    route + BDF2 implicit theta (coupled_model.py:123-821)."""
+    kind = KIND_COUPLED
+
+    @property
+    def last_bfbt_iters(self):
+        return int(self.eng.last_bfbt_iters[0])
+
+    @property
+    def n_bfbt_iters(self):
+        return int(self.eng.bfbt_iters[0])
+
+    @property
+    def n_memo_hits(self):
+        return int(self.eng.memo_hits[0])
+
+    @property
+    def n_bfbt_rescue(self):
+        return int(self.eng.n_rescue[0])
+
+    def eta_of(self, that):
+        """eta on the (Nx, Nz) grid as the Stokes operator consumes it
+        (coupled_model.py:401-496)."""
+        self._upload(that)
+        e = self.eng
+        e._eta_of(e.theta, None)
+        return self._mirror(e.eta_u, odd=False)

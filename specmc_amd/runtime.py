@@ -1,0 +1,103 @@
+"""ctypes binding of ``libspecmc_hip.so`` (the C ABI of include/specmc_hip.h).
+
+There is no CPU fallback: importing the engine without the built library, or
+stepping a model without a HIP device, raises ``RuntimeError``.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libspecmc_hip.so")
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+L = ctypes.c_long
+D = ctypes.c_double
+
+# name -> argtypes, in the order of include/specmc_hip.h
+SIGNATURES = {
+    "smx_version": [],
+    "smx_last_error": [],
+    "smx_gemm": [I, I, I, I, D, D, P, L, L, L, L, P, L, L, L, L, P, L, L, L, L,
+                 I, I, P, I, P, I, I, P],
+    "smx_lu_factor": [I, I, I, P, P, I, P, P],
+    "smx_lu_solve": [I, I, I, P, P, P, P, I, D, P, P, I, I, P, P],
+    "smx_tau_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P],
+    "smx_cheb_mult": [I, I, I, P, P, P, P],
+    "smx_eta_law": [I, I, I, I, P, P, P, P, P, P, P],
+    "smx_eta_guard_profile": [I, I, I, I, P, P, P, I, P, P, P],
+    "smx_xprod": [I, I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P],
+    "smx_advect_prod": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P],
+    "smx_axpby": [I, L, I, D, P, P, D, P, P, P, P, P],
+    "smx_rowscale": [I, I, I, I, P, P, D, P, I, P, P, P],
+    "smx_tau_rows": [I, I, I, I, P, P, P, P, P, P],
+    "smx_stokes_combine": [I, I, I, I, P, P, P, P, P, P, P, P, P],
+    "smx_theta_combine": [I, I, I, I, P, P, P, P, P, P, P, P, P, P],
+    "smx_theta_rhs": [I, I, I, I, P, P, P, P, P, P, P, P, P, P, P],
+    "smx_phi_grid": [I, I, L, P, P, P, P, P, P, P],
+    "smx_diag": [I, I, I, I, P, P, P, P, L, P, P, P, P, P, P],
+    "smx_maxabs": [I, I, L, P, P, P, P],
+    "smx_fg_init": [I, I, L, I, P, P, P, P, P, P, D, D, I, P, P],
+    "smx_fg_cycle_end": [I, I, L, I, P, P, P, P, P, P, P, P, P, P],
+    "smx_fg_post": [I, I, L, I, P, P, P, P, P, P, P, P, P, P, P, P],
+    "smx_fg_count": [I, P, P, P],
+    "smx_cfl_dt": [I, P, P, P, P, P],
+}
+
+_lib = None
+
+
+class _Checked:
+    """Calls an entry point and raises on a nonzero status."""
+
+    def __init__(self, lib, name):
+        self.fn = getattr(lib, name)
+        self.name = name
+        self.err = lib.smx_last_error
+
+    def __call__(self, *args):
+        rc = self.fn(*args)
+        if rc != 0:
+            raise RuntimeError("%s failed (%d): %s"
+                               % (self.name, rc, self.err().decode()))
+        return rc
+
+
+class _Lib:
+    def __init__(self, cdll):
+        self.cdll = cdll
+        for name in SIGNATURES:
+            if name in ("smx_version", "smx_last_error"):
+                continue
+            setattr(self, name[4:], _Checked(cdll, name))
+
+    def version(self):
+        return self.cdll.smx_version()
+
+
+def load():
+    """-> the bound library (built in-tree by ``specmc_amd.build_ext``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "the HIP library %s is not built; run "
+                "`python -m specmc_amd.build_ext` (or __graft_entry__.build())"
+                % LIB_PATH)
+        cdll = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(cdll, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        cdll.smx_last_error.restype = ctypes.c_char_p
+        _lib = _Lib(cdll)
+    return _lib
+
+
+def require_device():
+    """Raise unless a HIP device is usable through torch."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("specmc_amd runs on a HIP device (MI355X); none is "
+                           "visible -- there is no CPU fallback")
+    return torch.device("cuda", torch.cuda.current_device())

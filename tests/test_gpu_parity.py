@@ -1,0 +1,120 @@
+"""Device path vs the oracle and the reference's trajectory fixtures (MI355X).
+
+Tolerances: the FP64 device path must agree with the oracle / reference to
+1e-10 relative on the theta field (L2), Nu and Vrms; dt sequences to 1e-10
+relative and FGMRES iteration counts exactly.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from golden_io import load, rel, seeded_theta
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a HIP device")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    _need_gpu()
+    from specmc_amd import runtime
+    return runtime.load()
+
+
+def test_gemm_and_lu_kernels(lib):
+    from specmc_amd.engine import Engine
+    e = Engine("const", 16, 24, 2.0, [dict(Ra=1e4)])
+    rng = np.random.default_rng(0)
+    A = torch.tensor(rng.normal(size=(37, 29)), device=e.dev, dtype=torch.float64)
+    Bm = torch.tensor(rng.normal(size=(29, 71)), device=e.dev, dtype=torch.float64)
+    C = torch.zeros(37, 71, device=e.dev, dtype=torch.float64)
+    e.gemm(37, 71, 29, A, 29, 1, Bm, 71, 1, C, 71, 1)
+    torch.cuda.synchronize()
+    assert rel(C.cpu().numpy(), (A @ Bm).cpu().numpy()) < 1e-14
+    # batched LU + solve against numpy
+    N, nm = 24, 5
+    M = rng.normal(size=(nm, N, N)) + N * np.eye(N)
+    LU = torch.tensor(np.ascontiguousarray(M.transpose(0, 2, 1)), device=e.dev)
+    piv = torch.zeros(nm, N, dtype=torch.int32, device=e.dev)
+    lib.lu_factor(0, nm, N, LU.data_ptr(), piv.data_ptr(), 1, None, e.stream())
+    b = rng.normal(size=(nm, N))
+    bt = torch.tensor(b, device=e.dev)
+    x = torch.zeros_like(bt)
+    lib.lu_solve(0, nm, N, LU.data_ptr(), piv.data_ptr(), bt.data_ptr(), x.data_ptr(),
+                 0, 1.0, None, None, 1, 0, None, e.stream())
+    torch.cuda.synchronize()
+    want = np.stack([np.linalg.solve(M[i], b[i]) for i in range(nm)])
+    assert rel(x.cpu().numpy(), want) < 1e-12
+
+
+def _ic(model, meta):
+    ic = meta["ic"]
+    if ic["kind"] == "conductive":
+        return model.seed(amp=ic["amp"], m=ic.get("m", 1), n=ic.get("n", 1))
+    return seeded_theta(model, float(meta["fields"].get("internal_heating", 0.0)),
+                        ic["seed"])
+
+
+GOLDEN = ["c0_const_32x24", "const_h_di_16x24", "etaz_16x24", "etaT_bfbt_16x24",
+          "arrh_bfbt_16x24", "etaT_bfbt_32x48_h", "c1_short_64x96"]
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_device_matches_reference_trajectory(lib, name):
+    from specmc_amd import ConvectionConfig, api
+    g = load(name)
+    meta = g["meta"]
+    print("case", name, meta["fields"], flush=True)
+    cfg = ConvectionConfig(**meta["fields"])
+    m = api.build_model(cfg)
+    th0 = _ic(m, meta)
+    assert rel(th0, g["theta0"]) < 1e-14
+    dts, sits, tits = [], [], []
+    last = {"t": 0.0}
+
+    def on_step(n, t, s):
+        dts.append(t - last["t"])
+        last["t"] = t
+        sits.append(int(getattr(m, "last_bfbt_iters", 0)))
+        tits.append(int(m.theta.last_iters))
+
+    if meta["tstop"] is not None:
+        s, info = api.march_adaptive(m, th0, meta["tstop"], on_step=on_step)
+    else:
+        s, info = api.march(m, th0, meta["dt"], meta["nsteps"], on_step=on_step)
+    assert info["steps"] == meta["info"]["steps"]
+    np.testing.assert_allclose(dts, g["dts"], rtol=TOL, atol=0)
+    if meta["fields"]["viscosity_model"] != "const":
+        assert sits == [int(v) for v in g["stokes_iters"]]
+        assert tits == [int(v) for v in g["theta_iters"]]
+    err = rel(s, g["final"].real)
+    print("theta rel err %.3e" % err, flush=True)
+    assert err < TOL
+    d = m.diagnostics(s)
+    for k in ("Nu_top", "Nu_bot", "Nu_energy", "urms", "umax", "amp"):
+        ref = meta["diagnostics"][k]
+        assert abs(d[k] - ref) <= TOL * max(abs(ref), 1.0), (k, d[k], ref)
+
+
+def test_device_matches_oracle_single_operators(lib):
+    """Stokes solve and diagnostics of one state against the oracle."""
+    from specmc_amd import ConvectionConfig, api
+    cfg = ConvectionConfig(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=32, Nz=48,
+                           Lx=2.0, precond="bfbt")
+    m = api.build_model(cfg)
+    ref = oracle.build_oracle(cfg)
+    th = seeded_theta(m, 0.0, 5, sigma=3e-2).real
+    psi, om = m.stokes(th)
+    psi_r, om_r = ref.stokes(th)
+    assert rel(psi, psi_r) < 1e-10
+    assert rel(om, om_r) < 1e-10
+    assert m.last_bfbt_iters == ref.last_bfbt_iters
+    eta = m.eta_of(th)
+    assert rel(eta, ref.eta_of(th)) < 1e-12
