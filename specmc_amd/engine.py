@@ -163,6 +163,7 @@ class Engine:
         self.hlam = z(B)
         self._a0_fact = np.full(B, np.nan)
         self.all_on = torch.ones(B, dtype=torch.int32, device=dev)
+        self.k2 = (self.t["kx"] * self.t["kx"]).contiguous()
         self.n_steps = np.zeros(B, dtype=np.int64)
         self.n_stokes = np.zeros(B, dtype=np.int64)
         # theta machinery (implicit BDF2) for eta_z / coupled; the explicit
@@ -444,7 +445,6 @@ class Engine:
         B, Nk, N, dev = self.B, self.Nk, self.Nz, self.dev
         z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
         self._alloc_reduced()
-        self.k2 = self.t["kx"] * self.t["kx"]
         o = self.ops
         # eta(z) profile, window-clipped, per run (convection_modes.py:86-110)
         prof = np.zeros((B, N))
@@ -484,7 +484,6 @@ class Engine:
         B, Nk, N, Q, dev = self.B, self.Nk, self.Nz, self.Q, self.dev
         z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
         self._alloc_reduced()
-        self.k2 = self.t["kx"] * self.t["kx"]
         self.eta_p = z(B, self.nrp * N)        # padded law field
         self.eta_u = z(B, self.nru * N)        # eta on the Nx grid (last_eta)
         self.eta_c = z(B, self.n)              # its coefficients
