@@ -122,9 +122,12 @@ int smx_eta_guard_profile(int prec, int B, int Nx, int N, void *eta,
  * y[b][f][:, q] on the mirror half of the 2Nx grid (odd parity for f < nodd,
  * even otherwise), multiply by eta[b][:, q] and analyse back to Nk modes.
  * y/out layout [B][Nk][nf][Q]; fscale [nf][Nk] (may be NULL) scales the
- * input modes (the 2 k^2 and k factors inside the convolution). */
+ * input modes (the 2 k^2 and k factors inside the convolution); eta of run
+ * b starts at eta + b*eta_bs (eta_bs = 0: one eta field for every column of
+ * a dense assembly). */
 int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
-              const void *y, const void *fscale, const void *eta, void *out,
+              const void *y, const void *fscale, const void *eta, long eta_bs,
+              void *out,
               const void *So, const void *Ce, const void *Ao, const void *Ae,
               const int *active, void *stream);
 

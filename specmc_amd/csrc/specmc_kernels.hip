@@ -99,17 +99,18 @@ struct GemmArgs {
   const int *active; int mask_mode, rows_per_run;
 };
 
-template <typename T>
+template <typename T, int TM, int TN>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
-  __shared__ T As[16][65];
-  __shared__ T Bs[16][65];
+  constexpr int PM = TM / 16, PN = TN / 16;
+  __shared__ T As[16][TM + 1];
+  __shared__ T Bs[16][TN + 1];
   const int z = blockIdx.z, z1 = z / g.nz2, z2 = z % g.nz2;
-  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
   if (g.active) {
     if (g.mask_mode == 2 && !g.active[z1]) return;
     if (g.mask_mode == 3 && !g.active[z2]) return;
     if (g.mask_mode == 1) {
-      int ilast = min(i0 + 63, g.M - 1);
+      int ilast = min(i0 + TM - 1, g.M - 1);
       int r0 = i0 / g.rows_per_run, r1 = ilast / g.rows_per_run;
       bool any = false;
       for (int r = r0; r <= r1; ++r) any |= (g.active[r] != 0);
@@ -120,47 +121,52 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   const T *Bm = (const T *)g.B + z1 * g.sb1 + z2 * g.sb2;
   T *C = (T *)g.C + z1 * g.sc1 + z2 * g.sc2;
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  T acc[4][4];
+  T acc[PM][PN];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < PM; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0;
+    for (int b = 0; b < PN; ++b) acc[a][b] = 0;
   const bool a_kfast = (g.acs == 1), b_jfast = (g.bcs == 1);
   for (int k0 = 0; k0 < g.K; k0 += 16) {
 #pragma unroll
-    for (int e0 = 0; e0 < 1024; e0 += 256) {
+    for (int e0 = 0; e0 < 16 * TM; e0 += 256) {
       int e = e0 + tid, ii, kk;
-      if (a_kfast) { ii = e >> 4; kk = e & 15; } else { kk = e >> 6; ii = e & 63; }
+      if (a_kfast) { ii = e >> 4; kk = e & 15; } else { kk = e / TM; ii = e % TM; }
       int i = i0 + ii, k = k0 + kk;
       As[kk][ii] = (i < g.M && k < g.K) ? A[(long)i * g.ars + (long)k * g.acs] : T(0);
-      int jj, kb;
-      if (b_jfast) { kb = e >> 6; jj = e & 63; } else { jj = e >> 4; kb = e & 15; }
+    }
+#pragma unroll
+    for (int e0 = 0; e0 < 16 * TN; e0 += 256) {
+      int e = e0 + tid, jj, kb;
+      if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e >> 4; kb = e & 15; }
       int j = j0 + jj, kq = k0 + kb;
       Bs[kb][jj] = (j < g.N && kq < g.K) ? Bm[(long)kq * g.brs + (long)j * g.bcs] : T(0);
     }
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
-      T a[4], b[4];
+      T a[PM], b[PN];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) { a[q] = As[kk][ty + 16 * q]; b[q] = Bs[kk][tx + 16 * q]; }
+      for (int q = 0; q < PM; ++q) a[q] = As[kk][ty + 16 * q];
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
+      for (int q = 0; q < PN; ++q) b[q] = Bs[kk][tx + 16 * q];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[p][q] = fma(a[p], b[q], acc[p][q]);
+      for (int p = 0; p < PM; ++p)
+#pragma unroll
+        for (int q = 0; q < PN; ++q) acc[p][q] = fma(a[p], b[q], acc[p][q]);
     }
     __syncthreads();
   }
   const T alpha = (T)g.alpha, beta = (T)g.beta;
   const T *rs = (const T *)g.rscale;
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
+  for (int p = 0; p < PM; ++p) {
     int i = i0 + ty + 16 * p;
     if (i >= g.M) continue;
     if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
     T al = rs ? alpha * rs[i % g.rperiod] : alpha;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < PN; ++q) {
       int j = j0 + tx + 16 * q;
       if (j >= g.N) continue;
       long o = (long)i * g.crs + (long)j * g.ccs;
@@ -169,6 +175,23 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       C[o] = v;
     }
   }
+}
+
+template <typename T>
+static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
+  // tile choice: cover N without half-empty tiles; tall problems use TM = 128
+  const int N = g.N;
+  const bool tall = (long)g.M * nz >= 4096;
+  int tn = (N <= 32) ? 32 : (N % 64 == 0 || N > 96 ? 64 : (N <= 48 ? 48 : 96));
+  if (N > 96 && N % 96 == 0 && N % 64 != 0) tn = 96;
+#define GL(TM_, TN_)                                                                      \
+  hipLaunchKernelGGL((k_gemm<T, TM_, TN_>), dim3(cdiv(N, TN_), cdiv(g.M, TM_), nz),        \
+                     dim3(256), 0, st, g)
+  if (tn == 32) { if (tall) GL(128, 32); else GL(64, 32); }
+  else if (tn == 48) GL(64, 48);
+  else if (tn == 96) { if (tall) GL(128, 96); else GL(64, 96); }
+  else { if (tall) GL(128, 64); else GL(64, 64); }
+#undef GL
 }
 
 extern "C" int smx_version(void) { return SMX_VERSION; }
@@ -189,10 +212,9 @@ extern "C" int smx_gemm(int prec, int M, int N, int K, double alpha, double beta
   GemmArgs g{M, N, K, alpha, beta, A, ars, acs, sa1, sa2, B, brs, bcs, sb1, sb2,
              C, crs, ccs, sc1, sc2, nz2, rscale, rperiod, active, mask_mode,
              rows_per_run};
-  dim3 grid(cdiv(N, 64), cdiv(M, 64), nz1 * nz2);
-  if (grid.z > 65535) return fail_msg("smx_gemm: batch > 65535");
-  if (prec == 0) hipLaunchKernelGGL(k_gemm<double>, grid, dim3(256), 0, STREAM(stream), g);
-  else hipLaunchKernelGGL(k_gemm<float>, grid, dim3(256), 0, STREAM(stream), g);
+  if ((long)nz1 * nz2 > 65535) return fail_msg("smx_gemm: batch > 65535");
+  if (prec == 0) launch_gemm<double>(g, nz1 * nz2, STREAM(stream));
+  else launch_gemm<float>(g, nz1 * nz2, STREAM(stream));
   CHECK_LAUNCH("smx_gemm");
   return 0;
 }
@@ -259,16 +281,163 @@ __global__ void __launch_bounds__(256) k_lu_factor(T *LU, int *piv, int N, int p
   }
 }
 
+// LDS-resident variant: the whole matrix is staged in LDS, factorised with the
+// same partial-pivoting sweep as k_lu_factor, and written back.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_factor_lds(T *LU, int *piv, int N, int per_run,
+                                                       const int *active) {
+  extern __shared__ unsigned char smem_raw[];
+  T *A = (T *)smem_raw;
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  T *G = LU + (long)m * N * N;
+  int *p = piv + (long)m * N;
+  __shared__ T rv[8];
+  __shared__ int ri[8];
+  __shared__ int s_piv;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int NN = N * N;
+  for (int e = tid; e < NN; e += nt) A[e] = G[e];
+  __syncthreads();
+  for (int c = 0; c < N; ++c) {
+    T best = T(-1);
+    int bi = N;
+    for (int r = c + tid; r < N; r += nt) {
+      T v = fabs(A[c * N + r]);
+      if (v > best) { best = v; bi = r; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      T ov = __shfl_xor(best, o, 64);
+      int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      T b = rv[0];
+      int i = ri[0];
+      for (int w = 1; w < (nt >> 6); ++w)
+        if (rv[w] > b || (rv[w] == b && ri[w] < i)) { b = rv[w]; i = ri[w]; }
+      if (i >= N) i = c;
+      s_piv = i;
+      p[c] = i;
+    }
+    __syncthreads();
+    const int pr = s_piv;
+    if (pr != c)
+      for (int col = tid; col < N; col += nt) {
+        T t = A[col * N + c];
+        A[col * N + c] = A[col * N + pr];
+        A[col * N + pr] = t;
+      }
+    __syncthreads();
+    const T d = A[c * N + c];
+    if (d != T(0)) {
+      const T inv = T(1) / d;
+      for (int r = c + 1 + tid; r < N; r += nt) A[c * N + r] *= inv;
+    }
+    __syncthreads();
+    const int w = N - c - 1;
+    for (int e = tid; e < w * w; e += nt) {
+      int q = c + 1 + e / w, r = c + 1 + e % w;
+      A[q * N + r] -= A[c * N + r] * A[q * N + c];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < NN; e += nt) G[e] = A[e];
+}
+
+// LDS-resident solve: one block per system; same row interchanges and
+// column-oriented sweeps as k_lu_solve.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_solve_lds(const T *LU, const int *piv, const T *rhs,
+                                                      T *x, int N, int tau_zero, T scale,
+                                                      const T *kscale, const T *bscale,
+                                                      int per_run, int shared,
+                                                      const int *active) {
+  extern __shared__ unsigned char smem_raw[];
+  T *A = (T *)smem_raw;
+  T *y = A + N * N;
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  const long mi = shared ? (m % per_run) : m;
+  const T *G = LU + mi * N * N;
+  const int tid = threadIdx.x, nt = blockDim.x, NN = N * N;
+  for (int e = tid; e < NN; e += nt) A[e] = G[e];
+  T s = kscale ? scale * kscale[m % per_run] : scale;
+  if (bscale) s = s * bscale[m / per_run];
+  for (int r = tid; r < N; r += nt) {
+    T v = rhs[(long)m * N + r] * s;
+    if (tau_zero && r >= N - 2) v = T(0);
+    y[r] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int *p = piv + mi * N;
+    for (int c = 0; c < N; ++c) {
+      int pc = p[c];
+      if (pc != c) { T t = y[c]; y[c] = y[pc]; y[pc] = t; }
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < N - 1; ++c) {
+    const T yc = y[c];
+    for (int r = c + 1 + tid; r < N; r += nt) y[r] -= A[c * N + r] * yc;
+    __syncthreads();
+  }
+  for (int c = N - 1; c >= 0; --c) {
+    if (tid == 0) y[c] = y[c] / A[c * N + c];
+    __syncthreads();
+    const T xc = y[c];
+    for (int r = tid; r < c; r += nt) y[r] -= A[c * N + r] * xc;
+    __syncthreads();
+  }
+  for (int r = tid; r < N; r += nt) x[(long)m * N + r] = y[r];
+}
+
+// largest dynamic LDS a block may use (gfx950: 160 KiB per CU)
+static size_t lds_limit() {
+  static size_t lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+      lim = (size_t)v;
+    else
+      lim = 64 * 1024;
+  }
+  return lim;
+}
+
+template <typename K>
+static bool lds_optin(K kern, size_t bytes) {
+  if (bytes <= 64 * 1024) return true;
+  if (bytes > lds_limit()) return false;
+  return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes) == hipSuccess;
+}
+
 extern "C" int smx_lu_factor(int prec, int nmat, int N, void *LU, int *piv,
                              int per_run, const int *active, void *stream) {
   if (nmat <= 0) return 0;
   if (N <= 0 || per_run <= 0) return fail_msg("smx_lu_factor: bad sizes");
-  if (prec == 0)
-    hipLaunchKernelGGL(k_lu_factor<double>, dim3(nmat), dim3(256), 0, STREAM(stream),
-                       (double *)LU, piv, N, per_run, active);
-  else
-    hipLaunchKernelGGL(k_lu_factor<float>, dim3(nmat), dim3(256), 0, STREAM(stream),
-                       (float *)LU, piv, N, per_run, active);
+  const size_t sh = (size_t)N * N * (prec == 0 ? 8 : 4);
+  if (prec == 0) {
+    if (lds_optin(k_lu_factor_lds<double>, sh))
+      hipLaunchKernelGGL(k_lu_factor_lds<double>, dim3(nmat), dim3(256), sh, STREAM(stream),
+                         (double *)LU, piv, N, per_run, active);
+    else
+      hipLaunchKernelGGL(k_lu_factor<double>, dim3(nmat), dim3(256), 0, STREAM(stream),
+                         (double *)LU, piv, N, per_run, active);
+  } else {
+    if (lds_optin(k_lu_factor_lds<float>, sh))
+      hipLaunchKernelGGL(k_lu_factor_lds<float>, dim3(nmat), dim3(256), sh, STREAM(stream),
+                         (float *)LU, piv, N, per_run, active);
+    else
+      hipLaunchKernelGGL(k_lu_factor<float>, dim3(nmat), dim3(256), 0, STREAM(stream),
+                         (float *)LU, piv, N, per_run, active);
+  }
   CHECK_LAUNCH("smx_lu_factor");
   return 0;
 }
@@ -332,6 +501,23 @@ extern "C" int smx_lu_solve(int prec, int nmat, int N, const void *LU, const int
                             const int *active, void *stream) {
   if (nmat <= 0) return 0;
   if (N <= 0 || N > LUS_MAXN || per_run <= 0) return fail_msg("smx_lu_solve: bad sizes");
+  const size_t sh = (size_t)(N * N + N) * (prec == 0 ? 8 : 4);
+  if (prec == 0 && lds_optin(k_lu_solve_lds<double>, sh)) {
+    hipLaunchKernelGGL(k_lu_solve_lds<double>, dim3(nmat), dim3(256), sh, STREAM(stream),
+                       (const double *)LU, piv, (const double *)rhs, (double *)x, N, tau_zero,
+                       rhs_scale, (const double *)kscale, (const double *)bscale, per_run,
+                       shared, active);
+    CHECK_LAUNCH("smx_lu_solve");
+    return 0;
+  }
+  if (prec == 1 && lds_optin(k_lu_solve_lds<float>, sh)) {
+    hipLaunchKernelGGL(k_lu_solve_lds<float>, dim3(nmat), dim3(256), sh, STREAM(stream),
+                       (const float *)LU, piv, (const float *)rhs, (float *)x, N, tau_zero,
+                       (float)rhs_scale, (const float *)kscale, (const float *)bscale, per_run,
+                       shared, active);
+    CHECK_LAUNCH("smx_lu_solve");
+    return 0;
+  }
   dim3 grid(cdiv(nmat, LUS_WAVES));
   if (prec == 0)
     hipLaunchKernelGGL(k_lu_solve<double>, grid, dim3(64 * LUS_WAVES), 0, STREAM(stream),
@@ -590,7 +776,8 @@ extern "C" int smx_eta_guard_profile(int prec, int B, int Nx, int N, void *eta,
 #define XC 16
 template <typename T>
 __global__ void __launch_bounds__(256) k_xprod(int nf, int nodd, int Nk, int nrow, int Q,
-                                               const T *y, const T *fscale, const T *eta, T *out,
+                                               const T *y, const T *fscale, const T *eta,
+                                               long eta_bs, T *out,
                                                const T *So, const T *Ce, const T *Ao,
                                                const T *Ae, const int *active) {
   extern __shared__ unsigned char smem_raw[];
@@ -609,7 +796,7 @@ __global__ void __launch_bounds__(256) k_xprod(int nf, int nodd, int Nk, int nro
     Ys[e] = fscale ? fscale[(long)f * Nk + m] * v : v;
   }
   __syncthreads();
-  const T *eb = eta + (long)b * nrow * Q;
+  const T *eb = eta + (long)b * eta_bs;
   for (int e = threadIdx.x; e < nrow * XC; e += blockDim.x) {
     int j = e / XC, c = e % XC, q = q0 + c;
     const T *Sr = S + (long)j * Nk;
@@ -630,7 +817,8 @@ __global__ void __launch_bounds__(256) k_xprod(int nf, int nodd, int Nk, int nro
 }
 
 extern "C" int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
-                         const void *y, const void *fscale, const void *eta, void *out,
+                         const void *y, const void *fscale, const void *eta, long eta_bs,
+                         void *out,
                          const void *So,
                          const void *Ce, const void *Ao, const void *Ae, const int *active,
                          void *stream) {
@@ -641,13 +829,13 @@ extern "C" int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, in
   if (prec == 0)
     hipLaunchKernelGGL(k_xprod<double>, grid, dim3(256), sh, STREAM(stream), nf, nodd, Nk,
                        nrow, Q, (const double *)y, (const double *)fscale,
-                       (const double *)eta, (double *)out,
+                       (const double *)eta, eta_bs, (double *)out,
                        (const double *)So, (const double *)Ce, (const double *)Ao,
                        (const double *)Ae, active);
   else
     hipLaunchKernelGGL(k_xprod<float>, grid, dim3(256), sh, STREAM(stream), nf, nodd, Nk, nrow,
                        Q, (const float *)y, (const float *)fscale, (const float *)eta,
-                       (float *)out,
+                       eta_bs, (float *)out,
                        (const float *)So, (const float *)Ce, (const float *)Ao,
                        (const float *)Ae, active);
   CHECK_LAUNCH("smx_xprod");

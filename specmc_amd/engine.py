@@ -94,6 +94,18 @@ class Solver:
         return c[3].copy(), c[4].copy(), r[3].copy()
 
 
+class _Workspace:
+    """Scratch of one Schur operator apply for ``B`` columns."""
+
+    def __init__(self, eng, B):
+        z = lambda *s: torch.zeros(*s, dtype=eng.dtype, device=eng.dev)
+        Nk, N, Q, n = eng.Nk, eng.Nz, eng.Q, eng.n
+        self.B = B
+        self.yz, self.tz = z(B, Nk * 4 * Q), z(B, Nk * 4 * Q)
+        self.tt, self.dtt = z(B, Nk * 4 * N), z(B, Nk * 3 * N)
+        self.wg, self.vin, self.out = z(B, n), z(B, n), z(B, n)
+
+
 class Engine:
     """B runs of one kind and one shape.  ``params`` is a list of per-run
     dicts with keys Ra, deta_T, Ts, eta_min, eta_max, eta_smooth,
@@ -102,7 +114,7 @@ class Engine:
     def __init__(self, kind, Nx, Nz, Lx, params, prec=0, device=None,
                  profile="harm", stokes_rtol=1e-8, stokes_restart=80,
                  stokes_maxiter=800, theta_rtol=1e-10, theta_restart=40,
-                 theta_maxiter=200):
+                 theta_maxiter=200, defect_tol=1e-8, max_K=8, K=3):
         self.dev = runtime.require_device() if device is None else device
         self.L = runtime.load()
         self.kind = kind
@@ -179,7 +191,8 @@ class Engine:
             if kind == KIND_DEPTH:
                 self._init_depth()
             elif kind == KIND_COUPLED:
-                self._init_coupled(stokes_rtol, stokes_restart, stokes_maxiter)
+                self._init_coupled(stokes_rtol, stokes_restart, stokes_maxiter,
+                                   defect_tol, max_K, K)
             else:
                 raise ValueError("unknown engine kind %r" % (kind,))
 
@@ -204,10 +217,11 @@ class Engine:
                     mask_mode if active is not None else 0, rpr, self.stream())
 
     def zrows(self, src, Mat, dst, K, O, src_ld, dst_ld, alpha=1.0, beta=0.0,
-              rscale=None, active=None, soff=0, doff=0):
+              rscale=None, active=None, soff=0, doff=0, nb=None):
         """dst[(b,k)][o] = alpha rs(k) sum_i src[(b,k)][i] Mat[o][i] over all
         B*Nk mode rows (a z-transform of every x-mode row)."""
-        self.gemm(self.B * self.Nk, O, K, src, src_ld, 1, Mat, 1, K, dst, dst_ld, 1,
+        nb = self.B if nb is None else nb
+        self.gemm(nb * self.Nk, O, K, src, src_ld, 1, Mat, 1, K, dst, dst_ld, 1,
                   alpha=alpha, beta=beta, aoff=soff, coff=doff, rscale=rscale,
                   rperiod=self.Nk, active=active, mask_mode=1, rpr=self.Nk)
 
@@ -218,10 +232,11 @@ class Engine:
                   nz1=self.B, sb=(src_bs, 0), sc=(dst_bs, 0), boff=soff, coff=doff,
                   active=active, mask_mode=2)
 
-    def green(self, src, dst, active=None):
+    def green(self, src, dst, active=None, nb=None):
         """dst[b][k] = G_k src[b][k] (stokes_coupled.py:318-325)."""
         N, n = self.Nz, self.n
-        self.gemm(self.B, N, N, src, n, 1, self.t["G"], 1, N, dst, n, 1,
+        nb = self.B if nb is None else nb
+        self.gemm(nb, N, N, src, n, 1, self.t["G"], 1, N, dst, n, 1,
                   nz1=1, nz2=self.Nk, sa=(0, N), sb=(0, N * N), sc=(0, N),
                   active=active, mask_mode=1, rpr=1)
 
@@ -480,7 +495,7 @@ class Engine:
         self.green(self.vsol, self.psi, active)
 
     # ------------------------------------------------------ coupled eta(T)
-    def _init_coupled(self, rtol, restart, maxiter):
+    def _init_coupled(self, rtol, restart, maxiter, defect_tol=1e-8, max_K=8, K=3):
         B, Nk, N, Q, dev = self.B, self.Nk, self.Nz, self.Q, self.dev
         z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
         self._alloc_reduced()
@@ -511,6 +526,14 @@ class Engine:
         self.last_bfbt_info = np.zeros(B, dtype=np.int64)
         self.last_bfbt_res = np.zeros(B)
         self.n_rescue = np.zeros(B, dtype=np.int64)
+        self.n_rescue_direct = np.zeros(B, dtype=np.int64)
+        self.n_rescue_weak = np.zeros(B, dtype=np.int64)
+        self.n_defect_failed = np.zeros(B, dtype=np.int64)
+        self.n_defect_double_fail = np.zeros(B, dtype=np.int64)
+        self.last_defect_res = np.zeros(B)
+        self.last_route = ["bfbt"] * B
+        self.defect_tol = defect_tol
+        self.max_K, self.K = int(max_K), int(K)
 
     def _eta_of(self, theta, active):
         """eta from theta: padded synthesis, law, window, truncation to Nk
@@ -535,28 +558,58 @@ class Engine:
         self.xleft(t["Ce_p"], nrp, Nk, self.yz, Q, Nk * Q, self.eta_S, Q, nrp * Q, Q,
                    active=active)
 
-    def schur_matvec(self, v, out, active):
+    def schur_matvec(self, v, out, active, W=None, eta=None, eta_bs=None):
         """Coupled Schur operator (FreeSchurOperator.matvec, lagged_stokes.py:202-220)
-        with the eta products evaluated pseudo-spectrally."""
-        L, t, B, Nk, N, n, Q = self.L, self.t, self.B, self.Nk, self.Nz, self.n, self.Q
+        with the eta products evaluated pseudo-spectrally.  ``W`` is a
+        workspace (default: the engine's own, one column per run); ``eta`` /
+        ``eta_bs`` the eta field on the product grid and its batch stride."""
+        L, t, Nk, N, Q = self.L, self.t, self.Nk, self.Nz, self.Q
+        W = self if W is None else W
+        B = W.B
+        eta = self.eta_S if eta is None else eta
+        eta_bs = self.nrp * Q if eta_bs is None else eta_bs
         s = self.stream()
-        self.green(v, self.wg, active)
+        self.green(v, W.wg, active, nb=B)
         # z-values on the padded product grid: f0 = v, (f1, f2, f3) = (w, D2 w, D w)
-        self.zrows(v, t["Vp"], self.yz, N, Q, N, 4 * Q, active=active)
-        self.zrows(self.wg, t["VpD2_VpD"], self.yz, N, 3 * Q, N, 4 * Q, doff=Q, active=active)
-        L.xprod(self.prec, B, 4, 3, Nk, self.nrp, Q, self.p(self.yz), self.p(self.fscale),
-                self.p(self.eta_S), self.p(self.tz), self.p(t["So_p"]), self.p(t["Ce_p"]),
-                self.p(t["Ao_p"]), self.p(t["Ae_p"]), self.p(active), s)
+        self.zrows(v, t["Vp"], W.yz, N, Q, N, 4 * Q, active=active, nb=B)
+        self.zrows(W.wg, t["VpD2_VpD"], W.yz, N, 3 * Q, N, 4 * Q, doff=Q, active=active, nb=B)
+        L.xprod(self.prec, B, 4, 3, Nk, self.nrp, Q, self.p(W.yz), self.p(self.fscale),
+                self.p(eta), eta_bs, self.p(W.tz), self.p(t["So_p"]),
+                self.p(t["Ce_p"]), self.p(t["Ao_p"]), self.p(t["Ae_p"]), self.p(active), s)
         # truncation to Nz Chebyshev modes, then D2 t0, D2 t1, D t3
-        self.gemm(B * Nk * 4, N, Q, self.tz, Q, 1, t["Tp"], 1, Q, self.tt, N, 1,
+        self.gemm(B * Nk * 4, N, Q, W.tz, Q, 1, t["Tp"], 1, Q, W.tt, N, 1,
                   active=active, mask_mode=1, rpr=Nk * 4)
         for f_in, f_out, Mat in ((0, 0, "D2"), (1, 1, "D2"), (3, 2, "D")):
-            self.gemm(Nk, N, N, self.tt, 4 * N, 1, t[Mat], 1, N, self.dtt, 3 * N, 1,
+            self.gemm(Nk, N, N, W.tt, 4 * N, 1, t[Mat], 1, N, W.dtt, 3 * N, 1,
                       nz1=B, sa=(Nk * 4 * N, 0), sc=(Nk * 3 * N, 0), aoff=f_in * N,
                       coff=f_out * N, active=active, mask_mode=2)
-        L.stokes_combine(self.prec, B, Nk, N, self.p(self.tt), self.p(self.dtt), self.p(t["kx"]),
+        L.stokes_combine(self.prec, B, Nk, N, self.p(W.tt), self.p(W.dtt), self.p(t["kx"]),
                          self.p(v), self.p(t["V0"]), self.p(t["V1"]), self.p(out),
                          self.p(active), s)
+
+    def assemble_schur(self, r, chunk=256):
+        """Dense real Schur matrix of run ``r`` (the imaginary-to-imaginary block
+        of CoupledStokes.assemble_schur, stokes_coupled.py:242-289), built by
+        applying the operator to identity columns in chunks."""
+        n = self.n
+        W = _Workspace(self, min(chunk, n))
+        At = torch.empty(n, n, dtype=self.dtype, device=self.dev)
+        eta = self.eta_S[r]
+        for j0 in range(0, n, W.B):
+            c = min(W.B, n - j0)
+            W.vin.zero_()
+            idx = torch.arange(c, device=self.dev)
+            W.vin[idx, j0 + idx] = 1.0
+            self.schur_matvec(W.vin, W.out, None, W=W, eta=eta, eta_bs=0)
+            At[j0:j0 + c] = W.out[:c]
+        A = At.t().contiguous()
+        # x-mean block: the stream-function class has v_0 = 0 and b_0 = 0, and
+        # the reference's (0, 0) block is decoupled from every other mode
+        # (the two halves of the convolution cancel for a real eta), so it only
+        # has to be invertible; its equation rows get the identity.
+        j = torch.arange(self.Nz - 2, device=self.dev)
+        A[j, j] = 1.0
+        return A
 
     def _stokes_coupled(self, theta, active):
         """eta_of + BFBT preconditioner + FGMRES on the Schur system + unpack
@@ -585,6 +638,8 @@ class Engine:
         self.last_bfbt_iters[act] = it[act]
         self.last_bfbt_info[act] = info[act]
         self.last_bfbt_res[act] = res[act]
+        for r in np.where(act)[0]:
+            self.last_route[r] = "bfbt"
         bad = act & (info != 0) & ~(res <= sv.rtol)
         if np.any(bad):
             self.n_rescue[bad] += 1
@@ -597,10 +652,59 @@ class Engine:
         self.green(self.vsol, self.psi, active)
 
     def _rescue(self, theta, idx):
-        raise RuntimeError(
-            "BFBT FGMRES did not converge for run(s) %s (iters %s, true residual %s); "
-            "the dense rescue route is not available on the device"
-            % (list(idx), list(self.last_bfbt_iters[idx]), list(self.last_bfbt_res[idx])))
+        """Recompute rejected BFBT solves with the robust route
+        (coupled_model.py:613-654): dense Schur LU at this eta + Picard defect
+        correction against the matrix-free operator, accepted when the true
+        relative residual meets the tolerance, else the direct dense solve
+        (Nk*Nz <= 4096) or a counted weak acceptance."""
+        sv, n, B = self.ssol, self.n, self.B
+        tol = self.defect_tol
+        for r in idx:
+            A = self.assemble_schur(int(r))
+            LU, piv = torch.linalg.lu_factor(A)
+            b = sv.b[r].clone()
+
+            def solve(rhs):
+                return torch.linalg.lu_solve(LU, piv, rhs[:, None])[:, 0]
+
+            def afree(x):
+                self.s1.zero_()
+                self.s1[r] = x
+                mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
+                mask[r] = 1
+                self.schur_matvec(self.s1, self.s2, mask)
+                return self.s2[r].clone()
+
+            x0 = solve(b)
+            x = x0.clone()
+            bn = max(float(torch.linalg.norm(b)), 1e-300)
+            xn = max(float(torch.linalg.norm(x)), 1e-300)
+            kmax = self.max_K if tol is not None else self.K
+            corr = []
+            for _ in range(kmax):
+                dx = solve(b - afree(x))
+                c = float(torch.linalg.norm(dx)) / xn
+                corr.append(c)
+                x = x + dx
+                if tol is not None and c <= tol:
+                    break
+            last = corr[-1] if corr else 0.0
+            if tol is not None and (not np.isfinite(last) or last > tol):
+                self.n_defect_failed[r] += 1
+                self.n_defect_double_fail[r] += 1
+            res = float(torch.linalg.norm(b - afree(x))) / bn
+            self.last_defect_res[r] = res
+            if res <= sv.rtol or not np.isfinite(res):
+                self.last_route[r] = "bfbt->legacy"
+            elif self.Nk * self.Nz > 4096:
+                self.n_rescue_weak[r] += 1
+                self.last_route[r] = "bfbt->legacy(weak)"
+            else:
+                x = x0
+                self.n_rescue_direct[r] += 1
+                self.last_defect_res[r] = float(torch.linalg.norm(b - afree(x))) / bn
+                self.last_route[r] = "bfbt->direct"
+            sv.x[r] = x
 
     def stokes(self, theta, active=None):
         """(psi, v = -omega) of ``theta`` for the active runs into self.psi /

@@ -51,7 +51,7 @@ class _DeviceModel:
                  internal_heating=0.0, dissipation_number=0.0,
                  eta_min=1e-3, eta_max=1e3, eta_smooth=0.0, arrhenius_Ts=0.0,
                  precond="bfbt", symmetrize=True, stokes_maxiter=None,
-                 **extra):
+                 defect_tol=1e-8, max_K=8, K=3, **extra):
         self.Ra, self.Nx, self.Nz, self.Lx = float(Ra), int(Nx), int(Nz), float(Lx)
         self.internal_heating = float(internal_heating)
         self.dissipation_number = float(dissipation_number)
@@ -63,7 +63,7 @@ class _DeviceModel:
                       eta_smooth=float(eta_smooth),
                       internal_heating=self.internal_heating,
                       dissipation_number=self.dissipation_number)
-        kw = {}
+        kw = dict(defect_tol=defect_tol, max_K=int(max_K), K=int(K))
         if stokes_maxiter is not None:
             kw["stokes_maxiter"] = int(stokes_maxiter)
         self.eng = Engine(self.kind, self.Nx, self.Nz, self.Lx, [params],
@@ -209,7 +209,12 @@ class _DeviceModel:
                      last_bfbt_iters=int(e.last_bfbt_iters[0]),
                      last_bfbt_info=int(e.last_bfbt_info[0]),
                      last_bfbt_true_res=float(e.last_bfbt_res[0]),
-                     n_bfbt_rescue=int(e.n_rescue[0]))
+                     n_bfbt_rescue=int(e.n_rescue[0]),
+                     n_bfbt_rescue_direct=int(e.n_rescue_direct[0]),
+                     n_bfbt_rescue_weak=int(e.n_rescue_weak[0]),
+                     n_defect_failed=int(e.n_defect_failed[0]),
+                     last_defect_res=float(e.last_defect_res[0]),
+                     last_stokes_route=e.last_route[0])
         return d
 
 
@@ -248,6 +253,10 @@ class RBCCoupled(_DeviceModel):
     @property
     def n_bfbt_rescue(self):
         return int(self.eng.n_rescue[0])
+
+    @property
+    def last_stokes_route(self):
+        return self.eng.last_route[0]
 
     def eta_of(self, that):
         """eta on the (Nx, Nz) grid as the Stokes operator consumes it

@@ -26,7 +26,7 @@ SIGNATURES = {
     "smx_cheb_mult": [I, I, I, P, P, P, P],
     "smx_eta_law": [I, I, I, I, P, P, P, P, P, P, P],
     "smx_eta_guard_profile": [I, I, I, I, P, P, P, I, P, P, P],
-    "smx_xprod": [I, I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P],
+    "smx_xprod": [I, I, I, I, I, I, I, P, P, P, L, P, P, P, P, P, P],
     "smx_advect_prod": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P],
     "smx_axpby": [I, L, I, D, P, P, D, P, P, P, P, P],
     "smx_rowscale": [I, I, I, I, P, P, D, P, I, P, P, P],

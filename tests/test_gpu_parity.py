@@ -118,3 +118,41 @@ def test_device_matches_oracle_single_operators(lib):
     assert m.last_bfbt_iters == ref.last_bfbt_iters
     eta = m.eta_of(th)
     assert rel(eta, ref.eta_of(th)) < 1e-12
+
+
+def test_dense_schur_assembly_matches_reference_form(lib):
+    """The device's dense Schur matrix equals the assembled form of
+    stokes_coupled.py:242-289 (imaginary-to-imaginary block)."""
+    from specmc_amd.models import RBCCoupled
+    from oracle.models import _SchurOps
+    m = RBCCoupled(Ra=1e4, Nx=16, Nz=24, Lx=2.0, deta_T=1e3)
+    ref = oracle.build_oracle(dict(viscosity_model="eta_T", Nx=16, Nz=24, Lx=2.0,
+                                   deta_T=1e3, precond="bfbt"))
+    th = seeded_theta(m, 0.0, 7, sigma=5e-2).real
+    m._upload(th)
+    e = m.eng
+    e._eta_of(e.theta, None)
+    A = e.assemble_schur(0).cpu().numpy()
+    Aref = _SchurOps(ref, ref.eta_of(th)).assemble_schur()
+    N = m.Nz
+    # every block coupling the x-modes k >= 1 (the stream-function class)
+    assert rel(A[N:, N:], Aref.real[N:, N:]) < 1e-12
+    # the x-mean block is decoupled in the reference matrix
+    assert np.abs(Aref[:N - 2, N:]).max() <= 1e-12 * np.abs(Aref).max()
+    assert np.allclose(A[N - 2:N], Aref.real[N - 2:N])
+
+
+def test_forced_bfbt_rescue_matches_oracle(lib):
+    """stokes_maxiter=1 cannot converge, so the rescue route must produce the
+    accepted solve (coupled_model.py:955-972 is the reference's own check)."""
+    from specmc_amd.models import RBCCoupled
+    from oracle.models import OracleCoupled
+    m = RBCCoupled(Ra=1e4, Nx=16, Nz=24, Lx=2.0, deta_T=1e3, stokes_maxiter=1)
+    ref = OracleCoupled(Ra=1e4, Nx=16, Nz=24, Lx=2.0, deta_T=1e3, precond="bfbt",
+                        stokes_maxiter=1)
+    th = m.seed(amp=1e-2).real
+    psi, _ = m.stokes(th)
+    psi_r, _ = ref.stokes(th)
+    assert m.n_bfbt_rescue == 1 and ref.n_bfbt_rescue == 1
+    assert m.last_stokes_route == ref.last_stokes_route
+    assert rel(psi, psi_r) < 1e-10
