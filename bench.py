@@ -1,18 +1,27 @@
 """Benchmark: batched steps/s at 64x96 eta_T (deta_T = 1e3) vs the plain FP64
-single-run device stepper, with p50/p99 run latency.
+single-run device stepper.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+    python bench.py --make-states        # record the config-1 trajectory once
 
-Workload (config 1 of the scenario set): eta_T viscosity, deta_T = 1e3,
-Ra = 1e4, Nx = 64, Nz = 96, Lx = 2 (the unit box on its mirror-doubled carrier),
-BFBT Stokes route, theta0 = conductive + amp cos(pi x) sin(pi z).  A "step"
-is one run-step (CFL dt + Stokes solve + BDF2 theta solve) of one run.
+Workload (config 1): eta_T viscosity, deta_T = 1e3, Ra = 1e4, Nx = 64, Nz = 96,
+Lx = 2 (the unit box on its mirror-doubled carrier), BFBT Stokes route, theta0
+= conductive + 3e-2 cos(pi x) sin(pi z), marched to tstop = 5e-3.  A "step" is
+one run-step (CFL dt + coupled Stokes solve + BDF2 theta solve).
 
-* plain   -- one run (amp = 3e-2) on the FP64 device stepper, unbatched;
-* batched -- B concurrent runs (amp ~ U[2.5e-2, 3.5e-2], seeded) of the same
-  shape advancing together, each with its own dt.
+The cost of a step changes by ~50x along that run (the Stokes FGMRES needs ~10
+iterations at first, then hits its 800-iteration budget and goes through the
+dense rescue route, then settles near 400).  ``--make-states`` marches the
+plain stepper through the whole run and records phase checkpoints and the
+per-step log in ``bench_states_c1_64x96.npz``.  The default run times, from
+each phase checkpoint, ``--steps`` steps of
 
-Prints one JSON line.  ``value`` is the batched aggregate run-steps/s.
+* plain   -- one run on the FP64 device stepper, unbatched;
+* batched -- B runs (the checkpoint with seeded 1e-3 relative perturbations)
+  advancing together, each with its own dt,
+
+and weights the phases by their share of the recorded run, giving the
+whole-run aggregate rates.  ``value`` is the batched aggregate run-steps/s.
 """
 import argparse
 import json
@@ -24,38 +33,47 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+STATES = os.path.join(ROOT, "bench_states_c1_64x96.npz")
 
 WORKLOAD = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64, Nz=96, Lx=2.0,
                 precond="bfbt")
+TSTOP = 5e-3
+CHECKPOINTS = (100, 300, 525)
 
 
-def _configs(B, seed=1):
-    from specmc_amd import ConvectionConfig
-    rng = np.random.default_rng(seed)
-    amps = rng.uniform(2.5e-2, 3.5e-2, size=B)
-    return [ConvectionConfig(**WORKLOAD) for _ in range(B)], amps
-
-
-def _ensemble(B, amps, prec=0):
-    from specmc_amd.batch import Ensemble
+def make_states(path=STATES):
+    """March the plain FP64 stepper through config 1 and record checkpoints."""
+    from specmc_amd import ConvectionConfig, api
     from specmc_amd.core import bench_ic
+    m = api.build_model(ConvectionConfig(**WORKLOAD))
+    th0 = bench_ic.conductive_theta(m, amp=3e-2)
+    log, saved = [], {}
+    t0 = time.perf_counter()
+
+    def cb(n, t, s):
+        log.append((n, t, m.last_bfbt_iters, m.theta.last_iters,
+                    m.last_stokes_route != "bfbt", time.perf_counter() - t0))
+        if n in CHECKPOINTS:
+            saved["s%d" % n] = np.array(s)
+        if n % 50 == 0:
+            print("step %d t=%.4e bfbt=%d theta=%d route=%s" % (
+                n, t, m.last_bfbt_iters, m.theta.last_iters, m.last_stokes_route),
+                flush=True)
+
+    s, info = api.march_adaptive(m, th0, TSTOP, on_step=cb)
+    wall = time.perf_counter() - t0
+    d = m.diagnostics(s)
+    np.savez_compressed(path, log=np.array(log), final=s, wall=wall,
+                        diag=json.dumps(d), **saved)
+    print(json.dumps(dict(steps=info["steps"], t=info["t"], wall=wall, diag=d)), flush=True)
+
+
+def _ensemble(states, prec=0):
     from specmc_amd import ConvectionConfig
-    from specmc_amd.basis import shape_ops
-    cfgs = [ConvectionConfig(**WORKLOAD) for _ in range(B)]
-    o = shape_ops(64, 96, 2.0)
-
-    class _M:
-        x, z, kx, Nx, V, Vi = o.x, o.z, o.kx, o.Nx, o.V, o.Vi
-
-        def coeffs(self, g):
-            A = np.fft.rfft(np.asarray(g), axis=0) / self.Nx
-            return (self.Vi @ A.T).T
-
-        def seed(self, amp=1e-3, m=1, n=1):
-            X, Z = np.meshgrid(self.x, self.z, indexing="ij")
-            return self.coeffs(amp * np.cos(m * np.pi * X) * np.sin(n * np.pi * Z))
-    th = [bench_ic.conductive_theta(_M(), amp=float(a)) for a in amps]
-    return Ensemble(cfgs, th, np.full(B, np.inf), prec=prec)
+    from specmc_amd.batch import Ensemble
+    B = len(states)
+    return Ensemble([ConvectionConfig(**WORKLOAD) for _ in range(B)], states,
+                    np.full(B, np.inf), prec=prec)
 
 
 def _timed(ens, steps, warmup):
@@ -71,14 +89,25 @@ def _timed(ens, steps, warmup):
     return n, time.perf_counter() - t0
 
 
-def cpu_baseline(nsteps=4):
-    """The oracle (numpy/scipy FP64 restatement) on the host cores: the first
-    ``nsteps`` steps of the plain config-1 run."""
+def phase_weights(log):
+    """Share of the recorded run's steps that each checkpoint represents
+    (nearest checkpoint by step-cost class)."""
+    it = log[:, 2]
+    rescue = log[:, 4] > 0
+    early = (~rescue) & (np.arange(len(log)) < np.argmax(rescue)) if rescue.any() else ~rescue
+    hard = rescue | (it >= 700)
+    late = ~(early | hard)
+    w = np.array([early.sum(), hard.sum(), late.sum()], dtype=float)
+    return w / w.sum()
+
+
+def cpu_baseline(state, nsteps=2):
+    """The oracle (numpy/scipy FP64 restatement) on the host cores: ``nsteps``
+    steps of the config-1 run from the early checkpoint."""
     import oracle
     m = oracle.build_oracle(WORKLOAD)
-    s = m.seed(amp=3e-2)
-    for _ in range(2):
-        s = m.step(s, oracle.models.policy_dt(m, s))
+    s = np.array(state, dtype=complex)
+    s = m.step(s, oracle.models.policy_dt(m, s))
     t0 = time.perf_counter()
     for _ in range(nsteps):
         s = m.step(s, oracle.models.policy_dt(m, s))
@@ -89,48 +118,76 @@ def cpu_baseline(nsteps=4):
     except Exception:
         cores = os.cpu_count() or 1
     return dict(value=nsteps / el, unit="steps/s", cores=cores, kind="port",
-                sample="%d steps (after 2 warm-up steps) of the config-1 run "
-                       "(64x96 eta_T deta=1e3, amp 3e-2) on the oracle" % nsteps)
+                sample="%d steps of the config-1 run from its step-%d checkpoint (after "
+                       "1 warm-up step) on the oracle; the device rates of that "
+                       "phase are early_plain/early_batched" % (nsteps, CHECKPOINTS[0]))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--phases", default="early,hard,late")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--make-states", action="store_true")
+    ap.add_argument("--states-out", default=STATES)
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: every run of this benchmark fits one MI355X "
-              "and runs shard by run across GPUs without communication; run it with "
+        print("bench.py measures one GPU: the runs of this benchmark are independent "
+              "and shard by run across GPUs without communication; run it with "
               "--gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     from specmc_amd import runtime
     runtime.require_device()
-    # plain FP64 single-run stepper
-    plain = _ensemble(1, np.array([3e-2]))
-    n1, t1 = _timed(plain, a.plain_steps, a.warmup)
-    plain_rate = n1 / t1
-    del plain
-    torch.cuda.empty_cache()
-    # batched form
-    _, amps = _configs(a.batch)
-    ens = _ensemble(a.batch, amps)
-    n, el = _timed(ens, a.steps, a.warmup)
-    value = n / el
+    if a.make_states:
+        make_states(a.states_out)
+        return
+    if not os.path.exists(STATES):
+        raise SystemExit("missing %s: run `python bench.py --make-states` first" % STATES)
+    z = np.load(STATES, allow_pickle=False)
+    log = z["log"]
+    w = phase_weights(log)
+    names = ("early", "hard", "late")
+    rng = np.random.default_rng(1)
+    res = {}
+    for ph, step in zip(names, CHECKPOINTS):
+        if ph not in a.phases.split(","):
+            continue
+        s0 = z["s%d" % step]
+        plain = _ensemble([s0])
+        n1, t1 = _timed(plain, a.steps, a.warmup)
+        del plain
+        torch.cuda.empty_cache()
+        sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
+        ens = _ensemble(sts)
+        n2, t2 = _timed(ens, a.steps, a.warmup)
+        res[ph] = dict(plain=n1 / t1, batched=n2 / t2, ms_batched_step=1e3 * t2 / a.steps,
+                       bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
+                       rescues=int(ens.eng.n_rescue.sum()))
+        del ens
+        torch.cuda.empty_cache()
+        print("phase %s: %s" % (ph, json.dumps(res[ph])), file=sys.stderr, flush=True)
+    ws = np.array([w[names.index(p)] for p in res])
+    ws = ws / ws.sum()
+    # whole-run rates: time per step is the weighted harmonic combination
+    plain_rate = 1.0 / sum(wi / res[p]["plain"] for wi, p in zip(ws, res))
+    value = 1.0 / sum(wi / res[p]["batched"] for wi, p in zip(ws, res))
     out = dict(metric="batched steps/s at 64x96 eta_T deta=1e3 vs plain FP64 GPU",
                value=value, unit="run-steps/s", n_gpus=1, steps=a.steps,
-               warmup=a.warmup, ms_per_step=1e3 * el / a.steps, higher_is_better=True,
-               scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
-               config=dict(workload="64x96 eta_T deta_T=1e3 Ra=1e4 bfbt, seeded amp",
-                           batch=a.batch, Nx=64, Nz=96, precision="fp64"),
-               plain_fp64_steps_per_s=plain_rate,
-               speedup_vs_plain=value / plain_rate)
+               warmup=a.warmup, ms_per_step=1e3 * a.batch / value,
+               higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               data="synthetic",
+               config=dict(workload="config 1: 64x96 eta_T deta_T=1e3 Ra=1e4 Lx=2 bfbt, "
+                                    "tstop=5e-3 trajectory phases",
+                           batch=a.batch, Nx=64, Nz=96, precision="fp64",
+                           phase_weights=dict(zip(names, [float(x) for x in w]))),
+               plain_fp64_steps_per_s=plain_rate, speedup_vs_plain=value / plain_rate,
+               phases=res)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(z["s%d" % CHECKPOINTS[0]])
     print(json.dumps(out), flush=True)
 
 

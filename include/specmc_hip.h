@@ -75,6 +75,21 @@ int smx_lu_solve(int prec, int nmat, int N, const void *LU, const int *piv,
                  const void *kscale, const void *bscale, int per_run,
                  int shared, const int *active, void *stream);
 
+/* Batched explicit inverses (in-place Gauss-Jordan with partial row
+ * pivoting, column-major N x N) and their application
+ *   x = inv(A_m) (scale * b) with the tau entries of b zeroed when tau_zero,
+ * scale / run / shared conventions as smx_lu_solve.  The FGMRES
+ * preconditioners of the BFBT route (SpectralBFBTPrecond.apply,
+ * bfbt.py:185-191) and of the implicit theta solve (ImplicitThetaSolver.precond,
+ * implicit_theta.py:452-474) apply the same per-mode operators as a
+ * bandwidth-bound matrix-vector product. */
+int smx_invert(int prec, int nmat, int N, void *A, int per_run,
+               const int *active, void *stream);
+int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const void *rhs,
+                  void *x, int tau_zero, double rhs_scale, const void *kscale,
+                  const void *bscale, int per_run, int shared,
+                  const int *active, void *stream);
+
 /* Chebyshev-tau operators, column-major, per (run, mode):
  *   kind 0: D2 - (k^2 + lam_b) I, rows N-2, N-1 = (-1)^n, 1
  *           (ChebTau._assemble, rbc_galerkin.py:164-169);
@@ -92,6 +107,20 @@ int smx_tau_assemble(int prec, int kind, int B, int Nk, int N,
  * from coefficient vectors a[b] (cheb_mult_matrix, vv_stokes.py:81-109). */
 int smx_cheb_mult(int prec, int B, int N, const void *a, void *M,
                   const int *active, void *stream);
+
+/* Dense Schur matrix of one run in real form (CoupledStokes.assemble_schur,
+ * stokes_coupled.py:242-289), used by the BFBT rescue route
+ * (coupled_model.py:613-654).  From the per-shift Chebyshev multiplication
+ * matrices M_d and P = D2 M, Qm = M D2, R = D M D ([Nk][N][N] row-major):
+ * schur_blocks writes A_ij = (P_d - k_i^2 M_d) + s(P_d' - k_i^2 M_d') into the
+ * dense row-major A (ld = Nk*N) and the bracket U_ij that multiplies G_j into
+ * U [Nk][Nk][N][N] (d = |i - j|, d' = i + j, s = partner sign); the caller
+ * adds U_ij G_j with smx_gemm; schur_tau writes the tau rows. */
+int smx_schur_blocks(int prec, int Nk, int N, const void *M, const void *P,
+                     const void *Qm, const void *R, const void *kx, double sgn,
+                     void *A, void *U, void *stream);
+int smx_schur_tau(int prec, int Nk, int N, const void *V0, const void *V1,
+                  void *A, void *stream);
 
 /* The viscosity law on the padded half grid (coupled_model.py:333-399):
  * T = 1 - z + theta, counters of T outside [0,1] / [0,2], T clipped to

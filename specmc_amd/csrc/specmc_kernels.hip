@@ -418,6 +418,127 @@ static bool lds_optin(K kern, size_t bytes) {
                              (int)bytes) == hipSuccess;
 }
 
+#define LUS_MAXN 512
+
+// In-place Gauss-Jordan inversion with partial (row) pivoting, LDS-resident,
+// column-major N x N matrices (one block per matrix).  The preconditioners of
+// the FGMRES loops are applied as inv(A) r (smx_inv_apply) instead of two
+// sequential triangular sweeps.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gj_invert(T *Am, int N, int per_run, const int *active) {
+  extern __shared__ unsigned char smem_raw[];
+  T *A = (T *)smem_raw;                 // [col][row]
+  int *sw = (int *)(A + N * N);         // row swap of each step
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  T *G = Am + (long)m * N * N;
+  __shared__ T rv[8];
+  __shared__ int ri[8];
+  __shared__ int s_piv;
+  __shared__ T s_inv;
+  const int tid = threadIdx.x, nt = blockDim.x, NN = N * N;
+  for (int e = tid; e < NN; e += nt) A[e] = G[e];
+  __syncthreads();
+  for (int c = 0; c < N; ++c) {
+    T best = T(-1);
+    int bi = N;
+    for (int r = c + tid; r < N; r += nt) {
+      T v = fabs(A[c * N + r]);
+      if (v > best) { best = v; bi = r; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      T ov = __shfl_xor(best, o, 64);
+      int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      T b = rv[0];
+      int i = ri[0];
+      for (int w = 1; w < (nt >> 6); ++w)
+        if (rv[w] > b || (rv[w] == b && ri[w] < i)) { b = rv[w]; i = ri[w]; }
+      if (i >= N) i = c;
+      s_piv = i;
+      sw[c] = i;
+    }
+    __syncthreads();
+    const int pr = s_piv;
+    if (pr != c)
+      for (int col = tid; col < N; col += nt) {
+        T t = A[col * N + c];
+        A[col * N + c] = A[col * N + pr];
+        A[col * N + pr] = t;
+      }
+    __syncthreads();
+    if (tid == 0) s_inv = T(1) / A[c * N + c];
+    __syncthreads();
+    const T inv = s_inv;
+    // pivot row: a_cq /= a_cc, the pivot entry becomes 1/a_cc
+    for (int q = tid; q < N; q += nt) A[q * N + c] = (q == c) ? inv : A[q * N + c] * inv;
+    __syncthreads();
+    // eliminate column c from the other rows; column c takes -a_rc / a_cc
+    for (int e = tid; e < NN; e += nt) {
+      int q = e / N, r = e % N;
+      if (r == c) continue;
+      T f = A[c * N + r];
+      if (q == c) continue;
+      A[q * N + r] -= f * A[q * N + c];
+    }
+    __syncthreads();
+    for (int r = tid; r < N; r += nt)
+      if (r != c) A[c * N + r] = -A[c * N + r] * inv;
+    __syncthreads();
+  }
+  // undo the row swaps as column swaps, last step first
+  for (int c = N - 1; c >= 0; --c) {
+    const int pr = sw[c];
+    if (pr != c)
+      for (int r = tid; r < N; r += nt) {
+        T t = A[c * N + r];
+        A[c * N + r] = A[pr * N + r];
+        A[pr * N + r] = t;
+      }
+    __syncthreads();
+  }
+  for (int e = tid; e < NN; e += nt) G[e] = A[e];
+}
+
+// x[m] = inv[m] (scale * b[m]) (b[N-2] = b[N-1] = 0 when tau_zero); column-major inv.
+// One block per system: the 4 wavefronts split the columns, lanes the rows.
+template <typename T>
+__global__ void __launch_bounds__(256) k_inv_apply(const T *Inv, const T *rhs, T *x, int N,
+                                                   int tau_zero, T scale, const T *kscale,
+                                                   const T *bscale, int per_run, int shared,
+                                                   const int *active) {
+  __shared__ T bs[LUS_MAXN];
+  __shared__ T part[4][LUS_MAXN];
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  const long mi = shared ? (m % per_run) : m;
+  const T *A = Inv + mi * (long)N * N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  T s = kscale ? scale * kscale[m % per_run] : scale;
+  if (bscale) s = s * bscale[m / per_run];
+  for (int r = tid; r < N; r += 256) {
+    T v = rhs[(long)m * N + r] * s;
+    if (tau_zero && r >= N - 2) v = T(0);
+    bs[r] = v;
+  }
+  __syncthreads();
+  const int cpw = (N + 3) / 4, c0 = wv * cpw, c1 = min(N, c0 + cpw);
+  for (int r0 = 0; r0 < N; r0 += 64) {
+    const int r = r0 + lane;
+    T acc = 0;
+    if (r < N)
+      for (int c = c0; c < c1; ++c) acc = fma(A[(long)c * N + r], bs[c], acc);
+    if (r < N) part[wv][r] = acc;
+  }
+  __syncthreads();
+  for (int r = tid; r < N; r += 256)
+    x[(long)m * N + r] = (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+}
+
 extern "C" int smx_lu_factor(int prec, int nmat, int N, void *LU, int *piv,
                              int per_run, const int *active, void *stream) {
   if (nmat <= 0) return 0;
@@ -444,7 +565,6 @@ extern "C" int smx_lu_factor(int prec, int nmat, int N, void *LU, int *piv,
 
 // one wavefront per system; LU column-major
 #define LUS_WAVES 4
-#define LUS_MAXN 512
 template <typename T>
 __global__ void __launch_bounds__(64 * LUS_WAVES)
 k_lu_solve(const T *LU, const int *piv, const T *rhs, T *x, int nmat, int N, int tau_zero,
@@ -533,6 +653,45 @@ extern "C" int smx_lu_solve(int prec, int nmat, int N, const void *LU, const int
   return 0;
 }
 
+
+extern "C" int smx_invert(int prec, int nmat, int N, void *A, int per_run, const int *active,
+                          void *stream) {
+  if (nmat <= 0) return 0;
+  if (N <= 0 || per_run <= 0) return fail_msg("smx_invert: bad sizes");
+  const size_t sh = (size_t)N * N * (prec == 0 ? 8 : 4) + (size_t)N * 4;
+  if (prec == 0) {
+    if (!lds_optin(k_gj_invert<double>, sh)) return fail_msg("smx_invert: matrix exceeds LDS");
+    hipLaunchKernelGGL(k_gj_invert<double>, dim3(nmat), dim3(256), sh, STREAM(stream),
+                       (double *)A, N, per_run, active);
+  } else {
+    if (!lds_optin(k_gj_invert<float>, sh)) return fail_msg("smx_invert: matrix exceeds LDS");
+    hipLaunchKernelGGL(k_gj_invert<float>, dim3(nmat), dim3(256), sh, STREAM(stream),
+                       (float *)A, N, per_run, active);
+  }
+  CHECK_LAUNCH("smx_invert");
+  return 0;
+}
+
+extern "C" int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const void *rhs,
+                             void *x, int tau_zero, double rhs_scale, const void *kscale,
+                             const void *bscale, int per_run, int shared, const int *active,
+                             void *stream) {
+  if (nmat <= 0) return 0;
+  if (N <= 0 || N > LUS_MAXN || per_run <= 0) return fail_msg("smx_inv_apply: bad sizes");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_inv_apply<double>, dim3(nmat), dim3(256), 0, STREAM(stream),
+                       (const double *)Inv, (const double *)rhs, (double *)x, N, tau_zero,
+                       rhs_scale, (const double *)kscale, (const double *)bscale, per_run,
+                       shared, active);
+  else
+    hipLaunchKernelGGL(k_inv_apply<float>, dim3(nmat), dim3(256), 0, STREAM(stream),
+                       (const float *)Inv, (const float *)rhs, (float *)x, N, tau_zero,
+                       (float)rhs_scale, (const float *)kscale, (const float *)bscale,
+                       per_run, shared, active);
+  CHECK_LAUNCH("smx_inv_apply");
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // operator assembly (column-major out[b][k][col*N + row])
 // ---------------------------------------------------------------------------
@@ -613,6 +772,84 @@ extern "C" int smx_cheb_mult(int prec, int B, int N, const void *a, void *M,
     hipLaunchKernelGGL(k_cheb_mult<float>, grid, dim3(256), 0, STREAM(stream), N,
                        (const float *)a, (float *)M, active);
   CHECK_LAUNCH("smx_cheb_mult");
+  return 0;
+}
+
+// Dense Schur blocks (stokes_coupled.py:242-289) for one run, real form.
+// M, P = D2 M, Qm = M D2, R = D M D are [Nk][N][N] (row-major) per x-shift d.
+// For block (i, j): d = i - j (direct half, M_{-d} = M_d for a real eta) and,
+// when j >= 1 and i + j < Nk, d' = i + j (partner half, sign s):
+//   A_ij  = (P_d - k_i^2 M_d) + s (P_d' - k_i^2 M_d')
+//   U_ij  = (2k_j^2 P_d + 2k_i^2 Q_d - 4k_i k_j R_d)
+//           + s (2k_j^2 P_d' + 2k_i^2 Q_d' + 4k_i k_j R_d')
+// A is written into the dense row-major matrix (ld = Nk*N); U into [Nk][Nk][N][N].
+// The tau rows (N-2, N-1 of every block row) are set separately.
+template <typename T>
+__global__ void k_schur_blocks(int Nk, int N, const T *M, const T *P, const T *Qm, const T *R,
+                               const T *kx, T sgn, T *A, T *U) {
+  const int ij = blockIdx.y, i = ij / Nk, j = ij % Nk;
+  const long NN = (long)N * N, n = (long)Nk * N;
+  const int d = i >= j ? i - j : j - i;
+  const int dp = i + j;
+  const bool part = (j >= 1) && (dp < Nk) && (sgn != T(0));
+  const T ki = kx[i], kj = kx[j];
+  const T ki2 = ki * ki, kj2 = kj * kj;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < NN; e += (long)gridDim.x * blockDim.x) {
+    const int a = (int)(e / N), b = (int)(e % N);
+    const long od = (long)d * NN + e;
+    T av = P[od] - ki2 * M[od];
+    T uv = (T(2) * kj2 * P[od] + T(2) * ki2 * Qm[od]) - T(4) * ki * kj * R[od];
+    if (part) {
+      const long op = (long)dp * NN + e;
+      av = av + sgn * (P[op] - ki2 * M[op]);
+      uv = uv + sgn * ((T(2) * kj2 * P[op] + T(2) * ki2 * Qm[op]) + T(4) * ki * kj * R[op]);
+    }
+    A[((long)i * N + a) * n + (long)j * N + b] = av;
+    U[(long)ij * NN + e] = uv;
+  }
+}
+
+// tau rows of the dense Schur matrix: rows i*N + N-2 / N-1 are V0 / V1 on the
+// diagonal block and zero elsewhere (stokes_coupled.py:282-289)
+template <typename T>
+__global__ void k_schur_tau(int Nk, int N, const T *V0, const T *V1, T *A) {
+  const int i = blockIdx.y, which = blockIdx.z;
+  const long n = (long)Nk * N;
+  T *row = A + ((long)i * N + N - 2 + which) * n;
+  const T *v = which ? V1 : V0;
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+    long jb = c / N;
+    row[c] = (jb == i) ? v[c % N] : T(0);
+  }
+}
+
+extern "C" int smx_schur_blocks(int prec, int Nk, int N, const void *M, const void *P,
+                                const void *Qm, const void *R, const void *kx, double sgn,
+                                void *A, void *U, void *stream) {
+  if ((long)Nk * Nk > 65535) return fail_msg("smx_schur_blocks: Nk too large");
+  dim3 grid(cdiv((long)N * N, 256 * 4), Nk * Nk);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_schur_blocks<double>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const double *)M, (const double *)P, (const double *)Qm,
+                       (const double *)R, (const double *)kx, sgn, (double *)A, (double *)U);
+  else
+    hipLaunchKernelGGL(k_schur_blocks<float>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const float *)M, (const float *)P, (const float *)Qm,
+                       (const float *)R, (const float *)kx, (float)sgn, (float *)A, (float *)U);
+  CHECK_LAUNCH("smx_schur_blocks");
+  return 0;
+}
+
+extern "C" int smx_schur_tau(int prec, int Nk, int N, const void *V0, const void *V1, void *A,
+                             void *stream) {
+  dim3 grid(cdiv((long)Nk * N, 256), Nk, 2);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_schur_tau<double>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const double *)V0, (const double *)V1, (double *)A);
+  else
+    hipLaunchKernelGGL(k_schur_tau<float>, grid, dim3(256), 0, STREAM(stream), Nk, N,
+                       (const float *)V0, (const float *)V1, (float *)A);
+  CHECK_LAUNCH("smx_schur_tau");
   return 0;
 }
 

@@ -587,28 +587,34 @@ class Engine:
                          self.p(v), self.p(t["V0"]), self.p(t["V1"]), self.p(out),
                          self.p(active), s)
 
-    def assemble_schur(self, r, chunk=256):
-        """Dense real Schur matrix of run ``r`` (the imaginary-to-imaginary block
-        of CoupledStokes.assemble_schur, stokes_coupled.py:242-289), built by
-        applying the operator to identity columns in chunks."""
-        n = self.n
-        W = _Workspace(self, min(chunk, n))
-        At = torch.empty(n, n, dtype=self.dtype, device=self.dev)
-        eta = self.eta_S[r]
-        for j0 in range(0, n, W.B):
-            c = min(W.B, n - j0)
-            W.vin.zero_()
-            idx = torch.arange(c, device=self.dev)
-            W.vin[idx, j0 + idx] = 1.0
-            self.schur_matvec(W.vin, W.out, None, W=W, eta=eta, eta_bs=0)
-            At[j0:j0 + c] = W.out[:c]
-        A = At.t().contiguous()
-        # x-mean block: the stream-function class has v_0 = 0 and b_0 = 0, and
-        # the reference's (0, 0) block is decoupled from every other mode
-        # (the two halves of the convolution cancel for a real eta), so it only
-        # has to be invertible; its equation rows get the identity.
-        j = torch.arange(self.Nz - 2, device=self.dev)
-        A[j, j] = 1.0
+    def assemble_schur(self, r, out=None):
+        """Dense real Schur matrix of run ``r`` at its current eta (the
+        imaginary-to-imaginary block of CoupledStokes.assemble_schur,
+        stokes_coupled.py:242-289): per-shift Chebyshev multiplication matrices,
+        the block combinations, one batched GEMM against the Green operators
+        and the tau rows."""
+        L, t, N, Nk, n = self.L, self.t, self.Nz, self.Nk, self.n
+        NN = N * N
+        st = self.stream()
+        if getattr(self, "_asm", None) is None:
+            z = lambda *sh: torch.zeros(*sh, dtype=self.dtype, device=self.dev)
+            self._asm = dict(M=z(Nk, NN), P=z(Nk, NN), Q=z(Nk, NN), DM=z(Nk, NN),
+                             R=z(Nk, NN), U=z(Nk * Nk, NN))
+        w = self._asm
+        A = torch.empty(n, n, dtype=self.dtype, device=self.dev) if out is None else out
+        L.cheb_mult(self.prec, Nk, N, self.p(self.eta_c[r]), self.p(w["M"]), None, st)
+        mm = lambda X, sx, Y, sy, C: self.gemm(N, N, N, X, N, 1, Y, N, 1, C, N, 1, nz1=Nk,
+                                               sa=(sx, 0), sb=(sy, 0), sc=(NN, 0))
+        mm(t["D2"], 0, w["M"], NN, w["P"])        # D2 M_d
+        mm(w["M"], NN, t["D2"], 0, w["Q"])        # M_d D2
+        mm(t["D"], 0, w["M"], NN, w["DM"])        # D M_d
+        mm(w["DM"], NN, t["D"], 0, w["R"])        # (D M_d) D
+        L.schur_blocks(self.prec, Nk, N, self.p(w["M"]), self.p(w["P"]), self.p(w["Q"]),
+                       self.p(w["R"]), self.p(t["kx"]), -1.0, self.p(A), self.p(w["U"]), st)
+        # A_ij += U_ij G_j
+        self.gemm(N, N, N, w["U"], N, 1, t["G"], N, 1, A, n, 1, beta=1.0, nz1=Nk, nz2=Nk,
+                  sa=(Nk * NN, NN), sb=(0, NN), sc=(N * n, N))
+        L.schur_tau(self.prec, Nk, N, self.p(t["V0"]), self.p(t["V1"]), self.p(A), st)
         return A
 
     def _stokes_coupled(self, theta, active):
@@ -653,58 +659,70 @@ class Engine:
 
     def _rescue(self, theta, idx):
         """Recompute rejected BFBT solves with the robust route
-        (coupled_model.py:613-654): dense Schur LU at this eta + Picard defect
-        correction against the matrix-free operator, accepted when the true
-        relative residual meets the tolerance, else the direct dense solve
-        (Nk*Nz <= 4096) or a counted weak acceptance."""
+        (coupled_model.py:613-654) for every listed run at once: dense Schur LU
+        at this eta + Picard defect correction against the matrix-free
+        operator, accepted when the true relative residual meets the
+        tolerance, else the direct dense solve (Nk*Nz <= 4096) or a counted
+        weak acceptance."""
         sv, n, B = self.ssol, self.n, self.B
         tol = self.defect_tol
-        for r in idx:
-            A = self.assemble_schur(int(r))
-            LU, piv = torch.linalg.lu_factor(A)
-            b = sv.b[r].clone()
+        bad = [int(r) for r in idx]
+        nb = len(bad)
+        bt = torch.as_tensor(bad, device=self.dev)
+        A = torch.empty(nb, n, n, dtype=self.dtype, device=self.dev)
+        for q, r in enumerate(bad):
+            self.assemble_schur(r, out=A[q])
+        LU, piv = torch.linalg.lu_factor(A)
+        del A
+        mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
+        mask[bt] = 1
 
-            def solve(rhs):
-                return torch.linalg.lu_solve(LU, piv, rhs[:, None])[:, 0]
+        def solve(R):
+            return torch.linalg.lu_solve(LU, piv, R[..., None])[..., 0]
 
-            def afree(x):
-                self.s1.zero_()
-                self.s1[r] = x
-                mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
-                mask[r] = 1
-                self.schur_matvec(self.s1, self.s2, mask)
-                return self.s2[r].clone()
+        def afree(X):
+            self.s1[bt] = X
+            self.schur_matvec(self.s1, self.s2, mask)
+            return self.s2[bt].clone()
 
-            x0 = solve(b)
-            x = x0.clone()
-            bn = max(float(torch.linalg.norm(b)), 1e-300)
-            xn = max(float(torch.linalg.norm(x)), 1e-300)
-            kmax = self.max_K if tol is not None else self.K
-            corr = []
-            for _ in range(kmax):
-                dx = solve(b - afree(x))
-                c = float(torch.linalg.norm(dx)) / xn
-                corr.append(c)
-                x = x + dx
-                if tol is not None and c <= tol:
+        b = sv.b[bt].clone()
+        x0 = solve(b)
+        x = x0.clone()
+        bn = torch.linalg.norm(b, dim=1).clamp_min(1e-300)
+        xn = torch.linalg.norm(x0, dim=1).clamp_min(1e-300)
+        kmax = self.max_K if tol is not None else self.K
+        live = torch.ones(nb, dtype=torch.bool, device=self.dev)
+        last = torch.zeros(nb, dtype=torch.float64, device=self.dev)
+        for _ in range(kmax):
+            dx = solve(b - afree(x))
+            c = torch.linalg.norm(dx, dim=1) / xn
+            x = torch.where(live[:, None], x + dx, x)
+            last = torch.where(live, c.to(last.dtype), last)
+            if tol is not None:
+                live = live & ~(c <= tol)
+                if not bool(live.any()):
                     break
-            last = corr[-1] if corr else 0.0
-            if tol is not None and (not np.isfinite(last) or last > tol):
+        res = (torch.linalg.norm(b - afree(x), dim=1) / bn).cpu().numpy()
+        lastc = last.cpu().numpy()
+        res_direct = None
+        for q, r in enumerate(bad):
+            if tol is not None and (not np.isfinite(lastc[q]) or lastc[q] > tol):
                 self.n_defect_failed[r] += 1
                 self.n_defect_double_fail[r] += 1
-            res = float(torch.linalg.norm(b - afree(x))) / bn
-            self.last_defect_res[r] = res
-            if res <= sv.rtol or not np.isfinite(res):
+            self.last_defect_res[r] = res[q]
+            if res[q] <= sv.rtol or not np.isfinite(res[q]):
                 self.last_route[r] = "bfbt->legacy"
             elif self.Nk * self.Nz > 4096:
                 self.n_rescue_weak[r] += 1
                 self.last_route[r] = "bfbt->legacy(weak)"
             else:
-                x = x0
+                if res_direct is None:
+                    res_direct = (torch.linalg.norm(b - afree(x0), dim=1) / bn).cpu().numpy()
+                x[q] = x0[q]
                 self.n_rescue_direct[r] += 1
-                self.last_defect_res[r] = float(torch.linalg.norm(b - afree(x))) / bn
+                self.last_defect_res[r] = res_direct[q]
                 self.last_route[r] = "bfbt->direct"
-            sv.x[r] = x
+        sv.x[bt] = x
 
     def stokes(self, theta, active=None):
         """(psi, v = -omega) of ``theta`` for the active runs into self.psi /

@@ -22,8 +22,12 @@ SIGNATURES = {
                  I, I, P, I, P, I, I, P],
     "smx_lu_factor": [I, I, I, P, P, I, P, P],
     "smx_lu_solve": [I, I, I, P, P, P, P, I, D, P, P, I, I, P, P],
+    "smx_invert": [I, I, I, P, I, P, P],
+    "smx_inv_apply": [I, I, I, P, P, P, I, D, P, P, I, I, P, P],
     "smx_tau_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P],
     "smx_cheb_mult": [I, I, I, P, P, P, P],
+    "smx_schur_blocks": [I, I, I, P, P, P, P, P, D, P, P, P],
+    "smx_schur_tau": [I, I, I, P, P, P, P],
     "smx_eta_law": [I, I, I, I, P, P, P, P, P, P, P],
     "smx_eta_guard_profile": [I, I, I, I, P, P, P, I, P, P, P],
     "smx_xprod": [I, I, I, I, I, I, I, P, P, P, L, P, P, P, P, P, P],
