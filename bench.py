@@ -68,25 +68,42 @@ def make_states(path=STATES):
     print(json.dumps(dict(steps=info["steps"], t=info["t"], wall=wall, diag=d)), flush=True)
 
 
-def _ensemble(states, prec=0):
+ROOF_KERNEL = "pc_apply"     # the per-mode preconditioner apply of both FGMRES loops
+PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pc_apply_hard_b64_traffic.json")
+
+
+def _ensemble(states, prec=0, pc=None):
     from specmc_amd import ConvectionConfig
     from specmc_amd.batch import Ensemble
     B = len(states)
+    kw = {} if pc is None else dict(pc_apply=pc)
     return Ensemble([ConvectionConfig(**WORKLOAD) for _ in range(B)], states,
-                    np.full(B, np.inf), prec=prec)
+                    np.full(B, np.inf), prec=prec, **kw)
 
 
-def _timed(ens, steps, warmup):
+def pc_unit_bytes(eng):
+    """Algorithmic HBM bytes of one per-mode preconditioner apply: the block
+    (inverse or LU factors, N*N words; pivots for LU) read once, the input
+    and output vectors (N words each)."""
+    N, w = eng.Nz, eng.esize
+    return N * N * w + 2 * N * w + (4 * N if eng.pc_apply == "lu" else 0)
+
+
+def _timed(ens, steps, warmup, clock=None):
     import torch
     for _ in range(warmup):
         ens.step()
     torch.cuda.synchronize()
+    ens.eng.clock = clock
     t0 = time.perf_counter()
     n = 0
     for _ in range(steps):
         n += ens.step()
     torch.cuda.synchronize()
-    return n, time.perf_counter() - t0
+    el = time.perf_counter() - t0
+    ens.eng.clock = None
+    return n, el
 
 
 def phase_weights(log):
@@ -130,6 +147,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--phases", default="early,hard,late")
+    ap.add_argument("--legs", default="plain,batched")
+    ap.add_argument("--pc", default=None, choices=("lu", "inv"),
+                    help="preconditioner block application (default: the engine's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--make-states", action="store_true")
     ap.add_argument("--states-out", default=STATES)
@@ -151,30 +171,43 @@ def main():
     log = z["log"]
     w = phase_weights(log)
     names = ("early", "hard", "late")
+    from specmc_amd.kclock import KernelClock
+    legs = a.legs.split(",")
     rng = np.random.default_rng(1)
+    clock = KernelClock([ROOF_KERNEL])
+    unit_bytes = None
     res = {}
     for ph, step in zip(names, CHECKPOINTS):
         if ph not in a.phases.split(","):
             continue
         s0 = z["s%d" % step]
-        plain = _ensemble([s0])
-        n1, t1 = _timed(plain, a.steps, a.warmup)
-        del plain
-        torch.cuda.empty_cache()
-        sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
-        ens = _ensemble(sts)
-        n2, t2 = _timed(ens, a.steps, a.warmup)
-        res[ph] = dict(plain=n1 / t1, batched=n2 / t2, ms_batched_step=1e3 * t2 / a.steps,
-                       bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
-                       rescues=int(ens.eng.n_rescue.sum()))
-        del ens
-        torch.cuda.empty_cache()
-        print("phase %s: %s" % (ph, json.dumps(res[ph])), file=sys.stderr, flush=True)
+        r = {}
+        if "plain" in legs:
+            plain = _ensemble([s0], pc=a.pc)
+            n1, t1 = _timed(plain, a.steps, a.warmup)
+            r["plain"] = n1 / t1
+            del plain
+            torch.cuda.empty_cache()
+        if "batched" in legs:
+            sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
+            ens = _ensemble(sts, pc=a.pc)
+            n2, t2 = _timed(ens, a.steps, a.warmup, clock)
+            unit_bytes = pc_unit_bytes(ens.eng)
+            r.update(batched=n2 / t2, ms_batched_step=1e3 * t2 / a.steps,
+                     bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
+                     rescues=int(ens.eng.n_rescue.sum()), pc_apply=ens.eng.pc_apply)
+            del ens
+            torch.cuda.empty_cache()
+        res[ph] = r
+        print("phase %s: %s" % (ph, json.dumps(r)), file=sys.stderr, flush=True)
     ws = np.array([w[names.index(p)] for p in res])
     ws = ws / ws.sum()
     # whole-run rates: time per step is the weighted harmonic combination
-    plain_rate = 1.0 / sum(wi / res[p]["plain"] for wi, p in zip(ws, res))
-    value = 1.0 / sum(wi / res[p]["batched"] for wi, p in zip(ws, res))
+    rate = lambda leg: (1.0 / sum(wi / res[p][leg] for wi, p in zip(ws, res))
+                        if all(leg in res[p] for p in res) else None)
+    plain_rate, value = rate("plain"), rate("batched")
+    if value is None:
+        value = plain_rate
     out = dict(metric="batched steps/s at 64x96 eta_T deta=1e3 vs plain FP64 GPU",
                value=value, unit="run-steps/s", n_gpus=1, steps=a.steps,
                warmup=a.warmup, ms_per_step=1e3 * a.batch / value,
@@ -184,8 +217,20 @@ def main():
                                     "tstop=5e-3 trajectory phases",
                            batch=a.batch, Nx=64, Nz=96, precision="fp64",
                            phase_weights=dict(zip(names, [float(x) for x in w]))),
-               plain_fp64_steps_per_s=plain_rate, speedup_vs_plain=value / plain_rate,
+               plain_fp64_steps_per_s=plain_rate,
+               speedup_vs_plain=(value / plain_rate if plain_rate else None),
                phases=res)
+    nl, ms, units = clock.summary(ROOF_KERNEL)
+    if nl:
+        achieved = unit_bytes * units / nl / (ms * 1e-3)
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        out["roofline"] = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9,
+                               unit="GB/s", frac=achieved / PEAK_HBM, traffic=traffic,
+                               kernel=ROOF_KERNEL, launches=nl, mean_launch_us=1e3 * ms,
+                               bytes_per_unit=unit_bytes, units_per_launch=units / nl)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(z["s%d" % CHECKPOINTS[0]])
     print(json.dumps(out), flush=True)

@@ -32,6 +32,7 @@ KIND_CONST, KIND_DEPTH, KIND_COUPLED = "const", "eta_z", "coupled"
 PROFILES = {"harm": 0, "diag": 1, "geom": 2}
 
 MEMO_RTOL = 1e-12          # coupled_model.py:327, 749-765
+PC_APPLY = "inv"           # default application of the per-mode preconditioner blocks
 
 
 class Solver:
@@ -114,9 +115,17 @@ class Engine:
     def __init__(self, kind, Nx, Nz, Lx, params, prec=0, device=None,
                  profile="harm", stokes_rtol=1e-8, stokes_restart=80,
                  stokes_maxiter=800, theta_rtol=1e-10, theta_restart=40,
-                 theta_maxiter=200, defect_tol=1e-8, max_K=8, K=3):
+                 theta_maxiter=200, defect_tol=1e-8, max_K=8, K=3, pc_apply=None):
         self.dev = runtime.require_device() if device is None else device
         self.L = runtime.load()
+        pc_apply = PC_APPLY if pc_apply is None else pc_apply
+        if pc_apply not in ("lu", "inv"):
+            raise ValueError("pc_apply must be 'lu' or 'inv', got %r" % (pc_apply,))
+        # per-mode preconditioner blocks: LU + two triangular sweeps per
+        # apply ("lu"), or an explicit inverse applied as one matrix-vector
+        # product ("inv")
+        self.pc_apply = pc_apply
+        self.clock = None          # a kclock.KernelClock to time selected launches
         self.kind = kind
         self.prec = int(prec)
         self.dtype = torch.float64 if self.prec == 0 else torch.float32
@@ -421,6 +430,36 @@ class Engine:
         L.lu_solve(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv), self.p(self.s1),
                    self.p(self.theta), 1, -1.0, None, None, Nk, 0, self.p(active), s)
 
+    def _pc_factor(self, mats, piv, mask):
+        """Factor the assembled per-mode blocks ``mats`` [B][Nk][N*N] in place
+        (LU with pivots, or explicit inverse)."""
+        B, Nk, N = self.B, self.Nk, self.Nz
+        if self.pc_apply == "inv":
+            self.L.invert(self.prec, B * Nk, N, self.p(mats), Nk, self.p(mask), self.stream())
+        else:
+            self.L.lu_factor(self.prec, B * Nk, N, self.p(mats), self.p(piv), Nk, self.p(mask),
+                             self.stream())
+
+    def _pc_apply(self, mats, piv, vin, zout, scale, mask):
+        """zout_k = block_k^{-1} (scale * vin_k) with the tau entries zeroed."""
+        ck = self.clock
+        if ck is not None and ck.wants("pc_apply"):
+            ev = ck.begin()
+            self._pc_apply_launch(mats, piv, vin, zout, scale, mask)
+            ck.end("pc_apply", ev, self.B * self.Nk if mask is None else mask * self.Nk)
+        else:
+            self._pc_apply_launch(mats, piv, vin, zout, scale, mask)
+
+    def _pc_apply_launch(self, mats, piv, vin, zout, scale, mask):
+        B, Nk, N = self.B, self.Nk, self.Nz
+        if self.pc_apply == "inv":
+            self.L.inv_apply(self.prec, B * Nk, N, self.p(mats), self.p(vin), self.p(zout), 1,
+                             scale, None, None, Nk, 0, self.p(mask), self.stream())
+        else:
+            self.L.lu_solve(self.prec, B * Nk, N, self.p(mats), self.p(piv), self.p(vin),
+                            self.p(zout), 1, scale, None, None, Nk, 0, self.p(mask),
+                            self.stream())
+
     # ---------------------------------------------------- depth-only eta(z)
     def _reduced_op_factor(self, prof_c, lu, piv, active):
         """LU of Lm_k M + 2k^2 [D,[D,M]] G_k with tau rows v(0), v(1) for the
@@ -447,7 +486,7 @@ class Engine:
         L.tau_assemble(self.prec, 1, B, Nk, N, None, self.p(t["kx"]), None, self.p(M0),
                        self.p(T1), self.p(self.rb_CG), self.p(t["V0"]), self.p(t["V1"]),
                        self.p(lu), self.p(active), s)
-        L.lu_factor(self.prec, B * Nk, N, self.p(lu), self.p(piv), Nk, self.p(active), s)
+        self._pc_factor(lu, piv, active)
 
     def _alloc_reduced(self):
         B, Nk, N, dev = self.B, self.Nk, self.Nz, self.dev
@@ -631,8 +670,7 @@ class Engine:
                    self.p(sv.b), self.p(active), s)
 
         def precond(vin, zout, mask):
-            L.lu_solve(self.prec, B * Nk, N, self.p(self.bf_lu), self.p(self.bf_piv),
-                       self.p(vin), self.p(zout), 1, 1.0, None, None, Nk, 0, self.p(mask), s)
+            self._pc_apply(self.bf_lu, self.bf_piv, vin, zout, 1.0, mask)
 
         def op(zin, wout, mask):
             self.schur_matvec(zin, wout, mask)
@@ -669,16 +707,20 @@ class Engine:
         bad = [int(r) for r in idx]
         nb = len(bad)
         bt = torch.as_tensor(bad, device=self.dev)
-        A = torch.empty(nb, n, n, dtype=self.dtype, device=self.dev)
-        for q, r in enumerate(bad):
-            self.assemble_schur(r, out=A[q])
-        LU, piv = torch.linalg.lu_factor(A)
+        # one blocked LU per matrix: the batched factorisation of rocSOLVER
+        # takes its unblocked path at these sizes
+        A = torch.empty(n, n, dtype=self.dtype, device=self.dev)
+        facs = []
+        for r in bad:
+            self.assemble_schur(r, out=A)
+            facs.append(torch.linalg.lu_factor(A))
         del A
         mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
         mask[bt] = 1
 
         def solve(R):
-            return torch.linalg.lu_solve(LU, piv, R[..., None])[..., 0]
+            return torch.stack([torch.linalg.lu_solve(LU, piv, R[q, :, None])[:, 0]
+                                for q, (LU, piv) in enumerate(facs)])
 
         def afree(X):
             self.s1[bt] = X
@@ -839,13 +881,11 @@ class Engine:
             L.tau_assemble(self.prec, 0, B, Nk, N, self.p(t["D2"]), self.p(t["kx"]),
                            self.p(self.hlam), None, None, None, None, None, self.p(self.helm),
                            self.p(mask), s)
-            L.lu_factor(self.prec, B * Nk, N, self.p(self.helm), self.p(self.helm_piv), Nk,
-                        self.p(mask), s)
+            self._pc_factor(self.helm, self.helm_piv, mask)
             self._a0_fact[refac] = a0[refac]
 
         def precond(vin, zout, mask):
-            L.lu_solve(self.prec, B * Nk, N, self.p(self.helm), self.p(self.helm_piv),
-                       self.p(vin), self.p(zout), 1, -1.0, None, None, Nk, 0, self.p(mask), s)
+            self._pc_apply(self.helm, self.helm_piv, vin, zout, -1.0, mask)
 
         def op(zin, wout, mask):
             self.theta_matvec(zin, wout, mask)

@@ -51,7 +51,7 @@ class _DeviceModel:
                  internal_heating=0.0, dissipation_number=0.0,
                  eta_min=1e-3, eta_max=1e3, eta_smooth=0.0, arrhenius_Ts=0.0,
                  precond="bfbt", symmetrize=True, stokes_maxiter=None,
-                 defect_tol=1e-8, max_K=8, K=3, **extra):
+                 defect_tol=1e-8, max_K=8, K=3, pc_apply=None, **extra):
         self.Ra, self.Nx, self.Nz, self.Lx = float(Ra), int(Nx), int(Nz), float(Lx)
         self.internal_heating = float(internal_heating)
         self.dissipation_number = float(dissipation_number)
@@ -66,6 +66,8 @@ class _DeviceModel:
         kw = dict(defect_tol=defect_tol, max_K=int(max_K), K=int(K))
         if stokes_maxiter is not None:
             kw["stokes_maxiter"] = int(stokes_maxiter)
+        if pc_apply is not None:
+            kw["pc_apply"] = pc_apply
         self.eng = Engine(self.kind, self.Nx, self.Nz, self.Lx, [params],
                           prec=self.prec, **kw)
         o = self.eng.ops
