@@ -90,6 +90,21 @@ int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const void *rhs,
                   const void *bscale, int per_run, int shared,
                   const int *active, void *stream);
 
+/* Batched dense LU with partial pivoting, FP64 (prec must be 0), row-major
+ * n x n matrices (matrix q at A + q*mstride), n <= 8192.  piv int[nmat][n]:
+ * at step i rows i and piv[i] were exchanged (0-based, LAPACK order);
+ * info int[nmat]: 0, or 1 + the first column with a zero pivot.  Replaces
+ * sla.lu_factor of the assembled Schur matrix in LaggedStokes.refresh
+ * (lagged_stokes.py:274-294), which the BFBT rescue calls at the current eta
+ * (coupled_model.py:613-654), and the direct dense solve
+ * (coupled_model.py:649-650). */
+int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, int *piv,
+                 int *info, void *stream);
+/* x_q = A_q^{-1} b_q from smx_dense_lu factors; b and x are [nmat][n]
+ * (sla.lu_solve of the defect correction, coupled_model.py:544-554). */
+int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
+                       const int *piv, const void *b, void *x, void *stream);
+
 /* Chebyshev-tau operators, column-major, per (run, mode):
  *   kind 0: D2 - (k^2 + lam_b) I, rows N-2, N-1 = (-1)^n, 1
  *           (ChebTau._assemble, rbc_galerkin.py:164-169);

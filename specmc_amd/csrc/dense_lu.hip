@@ -1,0 +1,431 @@
+// dense_lu.hip -- batched dense LU with partial pivoting and its solves, for
+// the dense Schur systems of the BFBT rescue route (n = Nk*Nz: 3168 at
+// 64x96).  FP64, row-major n x n matrices, matrix q at A + q*mstride.
+//
+// Blocked right-looking factorisation, block width NB = 64:
+//   for each block [j0, j0+jb):
+//     for each sub-panel of PB columns [s0, s0+PB):
+//       k_lud_panel        factor the m x PB sub-panel (rows s0..n-1) with
+//                          partial pivoting; the sub-panel lives in registers
+//                          of one 1024-thread workgroup per matrix
+//       k_lud_block_update apply its swaps to the block's other columns, the
+//                          U rows (unit-lower PB x PB solve) and the rank-PB
+//                          update of the block columns to its right
+//     k_lud_laswp          the block's swaps on every column outside it
+//     k_lud_trsm           U12 = L11^{-1} A12
+//     smx_gemm             A22 -= L21 U12 (K = 64)
+// Row swaps are contiguous in row-major storage, so every kernel reads and
+// writes whole row segments.  Pivots are global row indices: at step i rows i
+// and piv[i] were exchanged (LAPACK getrf order, 0-based).
+#include "smx_common.h"
+#include "specmc_hip.h"
+
+#define LUD_NB 64
+#define LUD_MAXN 8192
+
+// ---------------------------------------------------------------------------
+// sub-panel factorisation: rows s0 + tid + 1024 k (k < R), columns s0..s0+PB-1
+// ---------------------------------------------------------------------------
+template <int R, int PB>
+__global__ void __launch_bounds__(1024) k_lud_panel(double *A, long ms, int n, int s0, int *piv,
+                                                    int *info) {
+  const int q = blockIdx.x;
+  double *M = A + (long)q * ms;
+  int *P = piv + (long)q * n;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  __shared__ double rowp[PB], rowc[PB];
+  __shared__ double rv[16];
+  __shared__ int ri[16];
+  __shared__ int s_p;
+  double v[R][PB];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int r = s0 + tid + k * 1024;
+#pragma unroll
+    for (int c = 0; c < PB; ++c)
+      v[k][c] = (r < n && s0 + c < n) ? M[(long)r * n + s0 + c] : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < PB; ++c) {
+    const int col = s0 + c;
+    if (col >= n) break;
+    // pivot: first row of largest |a| at or below the diagonal
+    double best = -1.0;
+    int bi = n;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int r = s0 + tid + k * 1024;
+      const double a = fabs(v[k][c]);
+      if (r >= col && r < n && a > best) {
+        best = a;
+        bi = r;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      rv[w] = best;
+      ri[w] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double b = rv[0];
+      int i = ri[0];
+      for (int u = 1; u < 16; ++u)
+        if (rv[u] > b || (rv[u] == b && ri[u] < i)) {
+          b = rv[u];
+          i = ri[u];
+        }
+      if (i < col || i >= n) i = col;
+      s_p = i;
+      P[col] = i;
+    }
+    __syncthreads();
+    const int p = s_p;
+    const int oc = col - s0, op = p - s0;
+    if (tid == (op & 1023)) {
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (k == (op >> 10)) {
+#pragma unroll
+          for (int cc = 0; cc < PB; ++cc) rowp[cc] = v[k][cc];
+        }
+    }
+    if (tid == (oc & 1023)) {
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (k == (oc >> 10)) {
+#pragma unroll
+          for (int cc = 0; cc < PB; ++cc) rowc[cc] = v[k][cc];
+        }
+    }
+    __syncthreads();
+    if (p != col) {
+      if (tid == (oc & 1023)) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (k == (oc >> 10)) {
+#pragma unroll
+            for (int cc = 0; cc < PB; ++cc) v[k][cc] = rowp[cc];
+          }
+      }
+      if (tid == (op & 1023)) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (k == (op >> 10)) {
+#pragma unroll
+            for (int cc = 0; cc < PB; ++cc) v[k][cc] = rowc[cc];
+          }
+      }
+    }
+    const double u = rowp[c];
+    if (u != 0.0) {
+      const double inv = 1.0 / u;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int r = s0 + tid + k * 1024;
+        if (r > col && r < n) {
+          const double l = v[k][c] * inv;
+          v[k][c] = l;
+#pragma unroll
+          for (int cc = c + 1; cc < PB; ++cc) v[k][cc] = fma(-l, rowp[cc], v[k][cc]);
+        }
+      }
+    } else if (tid == 0 && info[q] == 0) {
+      info[q] = col + 1;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int r = s0 + tid + k * 1024;
+    if (r < n) {
+#pragma unroll
+      for (int c = 0; c < PB; ++c)
+        if (s0 + c < n) M[(long)r * n + s0 + c] = v[k][c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// after sub-panel s0 of block [j0, j0+jb): one workgroup per PB-column group
+// of the block.  Left groups take the swaps; right groups take the swaps, the
+// U rows and the rank-PB update of rows s0+PB..n-1.
+// ---------------------------------------------------------------------------
+template <int PB>
+__global__ void __launch_bounds__(256) k_lud_block_update(double *A, long ms, int n, int j0,
+                                                          int jb, int s0, const int *piv) {
+  const int q = blockIdx.y;
+  double *M = A + (long)q * ms;
+  const int *P = piv + (long)q * n;
+  const int g0 = j0 + blockIdx.x * PB;
+  if (g0 == s0 || g0 >= j0 + jb) return;
+  const int ncol = min(PB, j0 + jb - g0);
+  const int npiv = min(PB, n - s0);
+  const int tid = threadIdx.x;
+  __shared__ double u[PB][PB];
+  if (tid < ncol) {
+    const int c = g0 + tid;
+    for (int i = 0; i < npiv; ++i) {
+      const int r = s0 + i, pr = P[r];
+      if (pr != r) {
+        const double t = M[(long)r * n + c];
+        M[(long)r * n + c] = M[(long)pr * n + c];
+        M[(long)pr * n + c] = t;
+      }
+    }
+  }
+  if (g0 < s0) return;
+  if (tid < ncol) {
+    const int c = g0 + tid;
+    double x[PB];
+#pragma unroll
+    for (int i = 0; i < PB; ++i) x[i] = (i < npiv) ? M[(long)(s0 + i) * n + c] : 0.0;
+#pragma unroll
+    for (int i = 1; i < PB; ++i) {
+      if (i < npiv) {
+        double a = x[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) a = fma(-M[(long)(s0 + i) * n + s0 + j], x[j], a);
+        x[i] = a;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i)
+      if (i < npiv) {
+        M[(long)(s0 + i) * n + c] = x[i];
+        u[i][tid] = x[i];
+      }
+  }
+  __syncthreads();
+  const int cc = tid % PB, rstep = 256 / PB;
+  if (cc < ncol) {
+    for (int r = s0 + npiv + tid / PB; r < n; r += rstep) {
+      const double *Lr = M + (long)r * n + s0;
+      double a = M[(long)r * n + g0 + cc];
+#pragma unroll
+      for (int j = 0; j < PB; ++j)
+        if (j < npiv) a = fma(-Lr[j], u[j][cc], a);
+      M[(long)r * n + g0 + cc] = a;
+    }
+  }
+}
+
+// the block's row swaps on the columns outside [j0, j0+jb), one thread per column
+__global__ void __launch_bounds__(256) k_lud_laswp(double *A, long ms, int n, int j0, int jb,
+                                                   const int *piv) {
+  const int q = blockIdx.y;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n - jb) return;
+  const int c = t < j0 ? t : t + jb;
+  double *M = A + (long)q * ms;
+  const int *P = piv + (long)q * n;
+  for (int i = j0; i < j0 + jb; ++i) {
+    const int pr = P[i];
+    if (pr != i) {
+      const double x = M[(long)i * n + c];
+      M[(long)i * n + c] = M[(long)pr * n + c];
+      M[(long)pr * n + c] = x;
+    }
+  }
+}
+
+// U12 = L11^{-1} A12 for rows j0..j0+jb-1 and columns j0+jb..n-1
+__global__ void __launch_bounds__(256) k_lud_trsm(double *A, long ms, int n, int j0, int jb) {
+  const int q = blockIdx.y;
+  double *M = A + (long)q * ms;
+  __shared__ double L[LUD_NB][LUD_NB + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < jb * jb; e += 256) {
+    const int i = e / jb, j = e % jb;
+    L[i][j] = M[(long)(j0 + i) * n + j0 + j];
+  }
+  __syncthreads();
+  const int c = j0 + jb + blockIdx.x * 256 + tid;
+  if (c >= n) return;
+  double x[LUD_NB];
+#pragma unroll
+  for (int i = 0; i < LUD_NB; ++i) x[i] = (i < jb) ? M[(long)(j0 + i) * n + c] : 0.0;
+#pragma unroll
+  for (int i = 1; i < LUD_NB; ++i) {
+    if (i < jb) {
+      double a = x[i];
+#pragma unroll
+      for (int j = 0; j < i; ++j) a = fma(-L[i][j], x[j], a);
+      x[i] = a;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < LUD_NB; ++i)
+    if (i < jb) M[(long)(j0 + i) * n + c] = x[i];
+}
+
+__global__ void k_lud_zero(int *v, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) v[i] = 0;
+}
+
+template <int R, int PB>
+static int lud_run(int nmat, int n, double *A, long ms, int *piv, int *info, hipStream_t s) {
+  for (int j0 = 0; j0 < n; j0 += LUD_NB) {
+    const int jb = min(LUD_NB, n - j0);
+    for (int s0 = j0; s0 < j0 + jb; s0 += PB) {
+      hipLaunchKernelGGL((k_lud_panel<R, PB>), dim3(nmat), dim3(1024), 0, s, A, ms, n, s0, piv,
+                         info);
+      hipLaunchKernelGGL(k_lud_block_update<PB>, dim3(cdiv(jb, PB), nmat), dim3(256), 0, s, A,
+                         ms, n, j0, jb, s0, piv);
+    }
+    if (n - jb > 0)
+      hipLaunchKernelGGL(k_lud_laswp, dim3(cdiv(n - jb, 256), nmat), dim3(256), 0, s, A, ms, n,
+                         j0, jb, piv);
+    const int nr = n - j0 - jb;
+    if (nr > 0) {
+      hipLaunchKernelGGL(k_lud_trsm, dim3(cdiv(nr, 256), nmat), dim3(256), 0, s, A, ms, n, j0,
+                         jb);
+      CHECK_LAUNCH("smx_dense_lu");
+      const long lda = n;
+      int rc = smx_gemm(0, nr, nr, jb, -1.0, 1.0,
+                        A + (long)(j0 + jb) * lda + j0, lda, 1, ms, 0,
+                        A + (long)j0 * lda + j0 + jb, lda, 1, ms, 0,
+                        A + (long)(j0 + jb) * lda + j0 + jb, lda, 1, ms, 0,
+                        nmat, 1, nullptr, 1, nullptr, 0, 1, (void *)s);
+      if (rc) return rc;
+    }
+  }
+  CHECK_LAUNCH("smx_dense_lu");
+  return 0;
+}
+
+extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, int *piv,
+                            int *info, void *stream) {
+  if (nmat <= 0) return 0;
+  if (prec != 0) return fail_msg("smx_dense_lu: FP64 only");
+  if (n <= 0 || n > LUD_MAXN || mstride < (long)n * n)
+    return fail_msg("smx_dense_lu: bad sizes (1 <= n <= 8192, mstride >= n*n)");
+  hipStream_t s = STREAM(stream);
+  hipLaunchKernelGGL(k_lud_zero, dim3(cdiv(nmat, 256)), dim3(256), 0, s, info, nmat);
+  double *a = (double *)A;
+  if (n <= 1024) return lud_run<1, 8>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 2048) return lud_run<2, 8>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 4096) return lud_run<4, 8>(nmat, n, a, mstride, piv, info, s);
+  return lud_run<8, 4>(nmat, n, a, mstride, piv, info, s);
+}
+
+// ---------------------------------------------------------------------------
+// solve: y = P b, L y' = y (unit lower), U x = y'; one 1024-thread workgroup
+// per matrix with the vector in LDS; 64-row blocks: a matrix-vector product
+// against the solved part (four rows per wavefront at a time) and a
+// wavefront-serial solve of the diagonal block staged in LDS.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_lud_solve(const double *A, long ms, int n,
+                                                    const int *piv, const double *b, double *x) {
+  extern __shared__ unsigned char smem_raw[];
+  double *y = (double *)smem_raw;
+  double *Db = y + n;                       // [64][65]
+  const int q = blockIdx.x;
+  const double *M = A + (long)q * ms;
+  const int *P = piv + (long)q * n;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int i = tid; i < n; i += 1024) y[i] = b[(long)q * n + i];
+  __syncthreads();
+  if (tid == 0)
+    for (int i = 0; i < n; ++i) {
+      const int p = P[i];
+      if (p != i && p >= 0 && p < n) {
+        const double t = y[i];
+        y[i] = y[p];
+        y[p] = t;
+      }
+    }
+  __syncthreads();
+  const int nblk = (n + 63) / 64;
+  for (int bk = 0; bk < nblk; ++bk) {
+    const int i0 = bk * 64, bs = min(64, n - i0);
+    // y[i0 + t] -= L[i0 + t, 0:i0] . y[0:i0]
+    {
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      const double *rows[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) rows[u] = M + (long)(i0 + min(w + 16 * u, bs - 1)) * n;
+      for (int j = lane; j < i0; j += 64) {
+        const double yj = y[j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double a = wave_sum(acc[u]);
+        if (lane == 0 && w + 16 * u < bs) y[i0 + w + 16 * u] -= a;
+      }
+    }
+    for (int e = tid; e < bs * bs; e += 1024) {
+      const int r = e / bs, c = e % bs;
+      Db[r * 65 + c] = M[(long)(i0 + r) * n + i0 + c];
+    }
+    __syncthreads();
+    if (w == 0) {
+      double yv = lane < bs ? y[i0 + lane] : 0.0;
+      for (int t = 0; t < bs; ++t) {
+        const double yt = __shfl(yv, t, 64);
+        if (lane > t && lane < bs) yv = fma(-Db[lane * 65 + t], yt, yv);
+      }
+      if (lane < bs) y[i0 + lane] = yv;
+    }
+    __syncthreads();
+  }
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int i0 = bk * 64, bs = min(64, n - i0), j1 = i0 + bs;
+    {
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      const double *rows[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) rows[u] = M + (long)(i0 + min(w + 16 * u, bs - 1)) * n;
+      for (int j = j1 + lane; j < n; j += 64) {
+        const double yj = y[j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double a = wave_sum(acc[u]);
+        if (lane == 0 && w + 16 * u < bs) y[i0 + w + 16 * u] -= a;
+      }
+    }
+    for (int e = tid; e < bs * bs; e += 1024) {
+      const int r = e / bs, c = e % bs;
+      Db[r * 65 + c] = M[(long)(i0 + r) * n + i0 + c];
+    }
+    __syncthreads();
+    if (w == 0) {
+      double yv = lane < bs ? y[i0 + lane] : 0.0;
+      for (int t = bs - 1; t >= 0; --t) {
+        if (lane == t) yv = yv / Db[t * 65 + t];
+        const double yt = __shfl(yv, t, 64);
+        if (lane < t) yv = fma(-Db[lane * 65 + t], yt, yv);
+      }
+      if (lane < bs) y[i0 + lane] = yv;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += 1024) x[(long)q * n + i] = y[i];
+}
+
+extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
+                                  const int *piv, const void *b, void *x, void *stream) {
+  if (nmat <= 0) return 0;
+  if (prec != 0) return fail_msg("smx_dense_lu_solve: FP64 only");
+  if (n <= 0 || n > LUD_MAXN || mstride < (long)n * n)
+    return fail_msg("smx_dense_lu_solve: bad sizes (1 <= n <= 8192, mstride >= n*n)");
+  const size_t sh = (size_t)n * 8 + 64 * 65 * 8;
+  if (!lds_optin(k_lud_solve, sh)) return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
+  hipLaunchKernelGGL(k_lud_solve, dim3(nmat), dim3(1024), sh, STREAM(stream),
+                     (const double *)LU, mstride, n, piv, (const double *)b, (double *)x);
+  CHECK_LAUNCH("smx_dense_lu_solve");
+  return 0;
+}

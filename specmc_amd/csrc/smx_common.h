@@ -1,0 +1,104 @@
+// smx_common.h -- helpers shared by the kernel sources of libspecmc_hip.so:
+// error reporting behind smx_last_error() and block-wide reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+extern thread_local char g_err[512];
+
+static inline int fail(const char *where, hipError_t e) {
+  snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+  return (int)e == 0 ? -1 : (int)e;
+}
+static inline int fail_msg(const char *msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return -1;
+}
+#define CHECK_LAUNCH(name)                                                   \
+  do {                                                                       \
+    hipError_t _e = hipGetLastError();                                       \
+    if (_e != hipSuccess) return fail(name, _e);                             \
+  } while (0)
+#define STREAM(s) ((hipStream_t)(s))
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// block reductions (wavefront shuffles, then LDS across wavefronts)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// sum over the block; every thread receives the result
+template <typename T>
+__device__ T block_sum(T v, T *red) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+template <typename T>
+__device__ T block_max(T v, T *red) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = red[0];
+  for (int i = 1; i < nw; ++i) s = fmax(s, red[i]);
+  return s;
+}
+template <typename T>
+__device__ T block_min(T v, T *red) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_min(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = red[0];
+  for (int i = 1; i < nw; ++i) s = fmin(s, red[i]);
+  return s;
+}
+
+// dynamic LDS beyond 64 KB needs a per-kernel opt-in
+static inline size_t lds_limit() {
+  static size_t lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+      lim = (size_t)v;
+    else
+      lim = 64 * 1024;
+  }
+  return lim;
+}
+
+template <typename K>
+static inline bool lds_optin(K kern, size_t bytes) {
+  if (bytes <= 64 * 1024) return true;
+  if (bytes > lds_limit()) return false;
+  return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes) == hipSuccess;
+}
+

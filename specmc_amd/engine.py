@@ -707,20 +707,22 @@ class Engine:
         bad = [int(r) for r in idx]
         nb = len(bad)
         bt = torch.as_tensor(bad, device=self.dev)
-        # one blocked LU per matrix: the batched factorisation of rocSOLVER
-        # takes its unblocked path at these sizes
-        A = torch.empty(n, n, dtype=self.dtype, device=self.dev)
-        facs = []
-        for r in bad:
-            self.assemble_schur(r, out=A)
-            facs.append(torch.linalg.lu_factor(A))
-        del A
+        L, st = self.L, self.stream()
+        A = torch.empty(nb, n, n, dtype=self.dtype, device=self.dev)
+        for q, r in enumerate(bad):
+            self.assemble_schur(r, out=A[q])
+        piv = torch.empty(nb, n, dtype=torch.int32, device=self.dev)
+        info = torch.empty(nb, dtype=torch.int32, device=self.dev)
+        L.dense_lu(self.prec, nb, n, self.p(A), n * n, self.p(piv), self.p(info), st)
         mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
         mask[bt] = 1
 
         def solve(R):
-            return torch.stack([torch.linalg.lu_solve(LU, piv, R[q, :, None])[:, 0]
-                                for q, (LU, piv) in enumerate(facs)])
+            R = R.contiguous()
+            X = torch.empty_like(R)
+            L.dense_lu_solve(self.prec, nb, n, self.p(A), n * n, self.p(piv), self.p(R),
+                             self.p(X), st)
+            return X
 
         def afree(X):
             self.s1[bt] = X
