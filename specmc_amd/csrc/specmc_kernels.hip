@@ -103,20 +103,40 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   }
 }
 
+// Tile choice: the largest tile that still gives every CU several
+// workgroups (the batched shapes of one step are small: K <= 3Nz/2 and
+// M*N per problem of a few 10^5), else the smallest tile.
+struct GemmTile { int tm, tn; };
+static const GemmTile kGemmTiles[] = {{128, 96}, {128, 64}, {64, 96}, {64, 64},
+                                      {64, 32}, {32, 64}, {32, 32}};
+static const int kGemmWantWG = 1024;
+
 template <typename T>
 static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
-  // tile choice: cover N without half-empty tiles; tall problems use TM = 128
-  const int N = g.N;
-  const bool tall = (long)g.M * nz >= 4096;
-  int tn = (N <= 32) ? 32 : (N % 64 == 0 || N > 96 ? 64 : (N <= 48 ? 48 : 96));
-  if (N > 96 && N % 96 == 0 && N % 64 != 0) tn = 96;
+  const int M = g.M, N = g.N;
+  int pick = -1;
+  long best_waste = -1;
+  for (int t = 0; t < (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0])); ++t) {
+    const int tm = kGemmTiles[t].tm, tn = kGemmTiles[t].tn;
+    const long wg = (long)cdiv(M, tm) * cdiv(N, tn) * nz;
+    if (wg < kGemmWantWG) continue;
+    // among tiles with enough workgroups prefer little padding, then size
+    const long waste = (long)cdiv(M, tm) * tm * cdiv(N, tn) * tn - (long)M * N;
+    if (pick < 0 || waste * 8 < best_waste) { pick = t; best_waste = waste; }
+  }
+  if (pick < 0) pick = (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0])) - 1;
 #define GL(TM_, TN_)                                                                      \
-  hipLaunchKernelGGL((k_gemm<T, TM_, TN_>), dim3(cdiv(N, TN_), cdiv(g.M, TM_), nz),        \
+  hipLaunchKernelGGL((k_gemm<T, TM_, TN_>), dim3(cdiv(N, TN_), cdiv(M, TM_), nz),          \
                      dim3(256), 0, st, g)
-  if (tn == 32) { if (tall) GL(128, 32); else GL(64, 32); }
-  else if (tn == 48) GL(64, 48);
-  else if (tn == 96) { if (tall) GL(128, 96); else GL(64, 96); }
-  else { if (tall) GL(128, 64); else GL(64, 64); }
+  switch (pick) {
+    case 0: GL(128, 96); break;
+    case 1: GL(128, 64); break;
+    case 2: GL(64, 96); break;
+    case 3: GL(64, 64); break;
+    case 4: GL(64, 32); break;
+    case 5: GL(32, 64); break;
+    default: GL(32, 32); break;
+  }
 #undef GL
 }
 

@@ -51,8 +51,11 @@ class Solver:
         self.g, self.y = z(B, m + 1), z(B, m)
         self.ctl = torch.zeros(10, B, dtype=torch.int32, device=dev)
         self.sc = torch.zeros(8, B, dtype=torch.float64, device=dev)
-        self.cnt = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.poll = 1
+        # unfinished-run counts of the last two rounds: device slots, and
+        # pinned host copies read one round late
+        self.cnt = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.cnt_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
+        self.cnt_ev = [torch.cuda.Event(), torch.cuda.Event()]
         self.rounds = 0
 
     def act_pre(self):
@@ -70,21 +73,31 @@ class Solver:
         mi = self.maxiter if maxiter is None else int(maxiter)
         L.fg_init(P, B, n, m, e.p(self.b), e.p(self.x), e.p(self.V), e.p(self.g),
                   e.p(self.ctl), e.p(self.sc), self.rtol, 0.0, mi, e.p(active), s)
+        # Round r is enqueued before the host waits for the count of round
+        # r-1, so the device never idles on the host's check; a run that is
+        # done skips every kernel of a round (the masks), so the one round
+        # enqueued after the last run finished is a no-op.
         rounds = 0
         while True:
-            for _ in range(self.poll):
-                L.fg_cycle_end(P, B, n, m, e.p(self.x), e.p(self.Z), e.p(self.V),
-                               e.p(self.H), e.p(self.g), e.p(self.y), e.p(self.vin),
-                               e.p(self.zbuf), e.p(self.ctl), s)
-                precond(self.vin, self.zbuf, self.act_pre())
-                op(self.zbuf, self.w, self.act_mv())
-                L.fg_post(P, B, n, m, e.p(self.b), e.p(self.w), e.p(self.zbuf),
-                          e.p(self.Z), e.p(self.V), e.p(self.H), e.p(self.cs),
-                          e.p(self.sn), e.p(self.g), e.p(self.ctl), e.p(self.sc), s)
-                rounds += 1
-            L.fg_count(B, e.p(self.ctl), e.p(self.cnt), s)
-            if int(self.cnt.item()) == 0:
-                break
+            slot = rounds & 1
+            L.fg_cycle_end(P, B, n, m, e.p(self.x), e.p(self.Z), e.p(self.V),
+                           e.p(self.H), e.p(self.g), e.p(self.y), e.p(self.vin),
+                           e.p(self.zbuf), e.p(self.ctl), s)
+            precond(self.vin, self.zbuf, self.act_pre())
+            op(self.zbuf, self.w, self.act_mv())
+            L.fg_post(P, B, n, m, e.p(self.b), e.p(self.w), e.p(self.zbuf),
+                      e.p(self.Z), e.p(self.V), e.p(self.H), e.p(self.cs),
+                      e.p(self.sn), e.p(self.g), e.p(self.ctl), e.p(self.sc), s)
+            L.fg_count(B, e.p(self.ctl), e.p(self.cnt, slot), s)
+            self.cnt_host[slot:slot + 1].copy_(self.cnt[slot:slot + 1], non_blocking=True)
+            self.cnt_ev[slot].record()
+            rounds += 1
+            if rounds >= 2:
+                prev = slot ^ 1
+                self.cnt_ev[prev].synchronize()
+                if int(self.cnt_host[prev]) == 0:
+                    break
+        self.cnt_ev[(rounds - 1) & 1].synchronize()
         self.rounds += rounds
         return rounds
 
@@ -461,7 +474,7 @@ class Engine:
                             self.stream())
 
     # ---------------------------------------------------- depth-only eta(z)
-    def _reduced_op_factor(self, prof_c, lu, piv, active):
+    def _reduced_op_factor(self, prof_c, lu, piv, active, as_pc=False):
         """LU of Lm_k M + 2k^2 [D,[D,M]] G_k with tau rows v(0), v(1) for the
         Chebyshev profile ``prof_c`` [B][N] of every run (vv_stokes.py:266-307,
         bfbt.py:157-181)."""
@@ -486,7 +499,10 @@ class Engine:
         L.tau_assemble(self.prec, 1, B, Nk, N, None, self.p(t["kx"]), None, self.p(M0),
                        self.p(T1), self.p(self.rb_CG), self.p(t["V0"]), self.p(t["V1"]),
                        self.p(lu), self.p(active), s)
-        self._pc_factor(lu, piv, active)
+        if as_pc:          # a preconditioner: factored the way _pc_apply applies it
+            self._pc_factor(lu, piv, active)
+        else:              # the depth-only Stokes operator itself: LU
+            L.lu_factor(self.prec, B * Nk, N, self.p(lu), self.p(piv), Nk, self.p(active), s)
 
     def _alloc_reduced(self):
         B, Nk, N, dev = self.B, self.Nk, self.Nz, self.dev
@@ -664,7 +680,7 @@ class Engine:
         self._eta_of(theta, active)
         self.gemm(B, N, N, self.prof, N, 1, t["Qprof"], 1, N, self.prof_c, N, 1,
                   active=active, mask_mode=1, rpr=1)
-        self._reduced_op_factor(self.prof_c, self.bf_lu, self.bf_piv, active)
+        self._reduced_op_factor(self.prof_c, self.bf_lu, self.bf_piv, active, as_pc=True)
         sv = self.ssol
         L.rowscale(self.prec, B, Nk, N, self.p(theta), self.p(t["kx"]), 1.0, self.p(self.Ra), 1,
                    self.p(sv.b), self.p(active), s)

@@ -236,3 +236,36 @@ def test_dense_lu_matches_lapack(lib, n):
         assert np.array_equal(pv[q], p_ref)
         assert rel(F[q], lu) < 1e-10
         assert r < 1e-10, r
+
+
+def test_batched_runs_match_solo_runs(lib):
+    """Runs of different physics advancing together in one Ensemble (per-run
+    dt, retirement at their own tstop) reproduce their solo marches."""
+    from specmc_amd import ConvectionConfig, api
+    from specmc_amd.batch import Ensemble
+    from specmc_amd.core import bench_ic
+    base = dict(Nx=16, Nz=24, Lx=2.0, precond="bfbt")
+    cfgs = [ConvectionConfig(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, **base),
+            ConvectionConfig(viscosity_model="eta_T", deta_T=1e2, Ra=2e4, internal_heating=1.0,
+                             **base),
+            ConvectionConfig(viscosity_model="arrhenius", deta_T=1e3, arrhenius_Ts=1.0,
+                             arrhenius_E=2.0 * np.log(1e3), Ra=1e4, **base)]
+    amps, tstops = (3e-2, 2e-2, 2.5e-2), (3e-4, 2e-4, 2.5e-4)
+    solo = []
+    th0 = []
+    for c, a, ts in zip(cfgs, amps, tstops):
+        m = api.build_model(c)
+        t0 = bench_ic.conductive_theta(m, amp=a)
+        th0.append(t0)
+        s, info = api.march_adaptive(m, t0, ts)
+        solo.append((s, info["steps"], m.diagnostics(s)))
+    ens = Ensemble(cfgs, th0, np.array(tstops))
+    ens.march()
+    got = ens.states()
+    d = ens.diagnostics()
+    for q, (s, steps, dg) in enumerate(solo):
+        err = rel(got[q], s)
+        print("run %d: steps %d/%d rel err %.2e" % (q, ens.steps[q], steps, err), flush=True)
+        assert ens.steps[q] == steps
+        assert err < 1e-12
+        assert abs(d[q][0] - dg["Nu_top"]) <= 1e-12 * abs(dg["Nu_top"])
