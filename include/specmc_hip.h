@@ -53,6 +53,11 @@ int smx_gemm(int prec, int M, int N, int K, double alpha, double beta,
              const int *active, int mask_mode, int rows_per_run,
              void *stream);
 
+/* FP64 GEMMs of smx_gemm run on the matrix cores (v_mfma_f64_16x16x4f64)
+ * when on != 0 (the default), on the vector FMA kernel otherwise.  Both give
+ * the same products to roundoff; the switch is process-wide. */
+int smx_set_gemm_mfma(int on);
+
 /* Batched LU with partial pivoting, in place, column-major N x N matrices
  * (LAPACK getrf semantics).  Replaces scipy.linalg.lu_factor in
  * ChebTau._factors_for (rbc_galerkin.py:119-144), VVStokesReduced

@@ -56,6 +56,20 @@ __device__ T block_sum(T v, T *red) {
   for (int i = 0; i < nw; ++i) s += red[i];
   return s;
 }
+// sum over the block with one barrier: successive calls alternate between
+// the two halves of red2 (2 x 16 entries), so a call never overwrites the
+// partials the previous call's readers may still be reading
+template <typename T>
+__device__ T block_sum_db(T v, T *red2, int parity) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  T *red = red2 + 16 * (parity & 1);
+  v = wave_sum(v);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
 template <typename T>
 __device__ T block_max(T v, T *red) {
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;

@@ -103,6 +103,100 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   }
 }
 
+// FP64 matrix-core GEMM, same arguments and masking as k_gemm.  Four
+// wavefronts in a 2 x 2 arrangement; each owns a (TM/2) x (TN/2) sub-tile made
+// of 16 x 16 blocks computed by v_mfma_f64_16x16x4f64.  Operand layout of that
+// instruction: lane l supplies A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16];
+// accumulator element r of lane l is C[l / 16 + 4 r][l % 16] (measured on
+// gfx950; the FP32 16x16x4 form instead holds C[4 (l / 16) + r][l % 16]).
+typedef double v4f64 __attribute__((ext_vector_type(4)));
+
+template <int TM, int TN>
+__global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
+  constexpr int WM = TM / 2, WN = TN / 2, BM = WM / 16, BN = WN / 16, KC = 16;
+  __shared__ double As[KC][TM + 1];
+  __shared__ double Bs[KC][TN + 1];
+  const int z = blockIdx.z, z1 = z / g.nz2, z2 = z % g.nz2;
+  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  if (g.active) {
+    if (g.mask_mode == 2 && !g.active[z1]) return;
+    if (g.mask_mode == 3 && !g.active[z2]) return;
+    if (g.mask_mode == 1) {
+      int ilast = min(i0 + TM - 1, g.M - 1);
+      int r0 = i0 / g.rows_per_run, r1 = ilast / g.rows_per_run;
+      bool any = false;
+      for (int r = r0; r <= r1; ++r) any |= (g.active[r] != 0);
+      if (!any) return;
+    }
+  }
+  const double *A = (const double *)g.A + z1 * g.sa1 + z2 * g.sa2;
+  const double *Bm = (const double *)g.B + z1 * g.sb1 + z2 * g.sb2;
+  double *C = (double *)g.C + z1 * g.sc1 + z2 * g.sc2;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * WM, wc = (w & 1) * WN;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4f64 acc[BM][BN];
+#pragma unroll
+  for (int a = 0; a < BM; ++a)
+#pragma unroll
+    for (int b = 0; b < BN; ++b) acc[a][b] = v4f64{0.0, 0.0, 0.0, 0.0};
+  const bool a_kfast = (g.acs == 1), b_jfast = (g.bcs == 1);
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
+#pragma unroll
+    for (int e0 = 0; e0 < KC * TM; e0 += 256) {
+      int e = e0 + tid, ii, kk;
+      if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
+      int i = i0 + ii, k = k0 + kk;
+      As[kk][ii] = (i < g.M && k < g.K) ? A[(long)i * g.ars + (long)k * g.acs] : 0.0;
+    }
+#pragma unroll
+    for (int e0 = 0; e0 < KC * TN; e0 += 256) {
+      int e = e0 + tid, jj, kb;
+      if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
+      int j = j0 + jj, kq = k0 + kb;
+      Bs[kb][jj] = (j < g.N && kq < g.K) ? Bm[(long)kq * g.brs + (long)j * g.bcs] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC; ks += 4) {
+      double a[BM], b[BN];
+#pragma unroll
+      for (int q = 0; q < BM; ++q) a[q] = As[ks + lk][wr + 16 * q + lr];
+#pragma unroll
+      for (int q = 0; q < BN; ++q) b[q] = Bs[ks + lk][wc + 16 * q + lr];
+#pragma unroll
+      for (int p = 0; p < BM; ++p)
+#pragma unroll
+        for (int q = 0; q < BN; ++q)
+          acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p], b[q], acc[p][q], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const double alpha = g.alpha, beta = g.beta;
+  const double *rs = (const double *)g.rscale;
+#pragma unroll
+  for (int p = 0; p < BM; ++p) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + wr + 16 * p + lk + 4 * r;
+      if (i >= g.M) continue;
+      if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
+      const double al = rs ? alpha * rs[i % g.rperiod] : alpha;
+#pragma unroll
+      for (int q = 0; q < BN; ++q) {
+        const int j = j0 + wc + 16 * q + lr;
+        if (j >= g.N) continue;
+        const long o = (long)i * g.crs + (long)j * g.ccs;
+        double v = al * acc[p][q][r];
+        if (beta != 0.0) v += beta * C[o];
+        C[o] = v;
+      }
+    }
+  }
+}
+
+static bool g_gemm_mfma = true;     // FP64 GEMMs on the matrix cores
+
 // Tile choice: the largest tile that still gives every CU several
 // workgroups (the batched shapes of one step are small: K <= 3Nz/2 and
 // M*N per problem of a few 10^5), else the smallest tile.
@@ -114,17 +208,29 @@ static const int kGemmWantWG = 1024;
 template <typename T>
 static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
   const int M = g.M, N = g.N;
+  if (sizeof(T) == 8 && g_gemm_mfma) {
+    // 64 x 64 tiles when they give every CU work and pad little, else 32 x 32
+    const long wg64 = (long)cdiv(M, 64) * cdiv(N, 64) * nz;
+    const long pad64 = (long)cdiv(M, 64) * 64 * cdiv(N, 64) * 64;
+    if (wg64 >= 512 && pad64 * 4 <= (long)M * N * 5)
+      hipLaunchKernelGGL((k_gemm_mfma64<64, 64>), dim3(cdiv(N, 64), cdiv(M, 64), nz), dim3(256),
+                         0, st, g);
+    else
+      hipLaunchKernelGGL((k_gemm_mfma64<32, 32>), dim3(cdiv(N, 32), cdiv(M, 32), nz), dim3(256),
+                         0, st, g);
+    return;
+  }
+  const int nt = (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0]));
+  // the largest tile with enough workgroups and at most 25% padding; else
+  // the tile with the most workgroups
   int pick = -1;
-  long best_waste = -1;
-  for (int t = 0; t < (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0])); ++t) {
+  for (int t = 0; t < nt && pick < 0; ++t) {
     const int tm = kGemmTiles[t].tm, tn = kGemmTiles[t].tn;
     const long wg = (long)cdiv(M, tm) * cdiv(N, tn) * nz;
-    if (wg < kGemmWantWG) continue;
-    // among tiles with enough workgroups prefer little padding, then size
-    const long waste = (long)cdiv(M, tm) * tm * cdiv(N, tn) * tn - (long)M * N;
-    if (pick < 0 || waste * 8 < best_waste) { pick = t; best_waste = waste; }
+    const long padded = (long)cdiv(M, tm) * tm * cdiv(N, tn) * tn;
+    if (wg >= kGemmWantWG && padded * 4 <= (long)M * N * 5) pick = t;
   }
-  if (pick < 0) pick = (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0])) - 1;
+  if (pick < 0) pick = nt - 1;
 #define GL(TM_, TN_)                                                                      \
   hipLaunchKernelGGL((k_gemm<T, TM_, TN_>), dim3(cdiv(N, TN_), cdiv(M, TM_), nz),          \
                      dim3(256), 0, st, g)
@@ -141,6 +247,10 @@ static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
 }
 
 extern "C" int smx_version(void) { return SMX_VERSION; }
+extern "C" int smx_set_gemm_mfma(int on) {
+  g_gemm_mfma = (on != 0);
+  return 0;
+}
 extern "C" const char *smx_last_error(void) { return g_err; }
 
 extern "C" int smx_gemm(int prec, int M, int N, int K, double alpha, double beta,
@@ -1644,22 +1754,58 @@ __global__ void __launch_bounds__(1024) k_fg_post(int B, long n, int m, const T 
       T *Zj = Zb + ((long)r * m + j) * n;
       for (long e = tid; e < n; e += nt) Zj[e] = zr[e];
     }
-    for (int i = 0; i <= j; ++i) {
-      const T *vi = Vr + (long)i * n;
-      double s = 0;
+    if constexpr (E > 0) {
+      // modified Gram-Schmidt with v_{i+1} loaded while h_i is reduced
+      __shared__ double red2[32];
+      __syncthreads();
+      T vc[E], vn[E];
 #pragma unroll
-      for (int q = 0; q < cnt; ++q) {
+      for (int q = 0; q < E; ++q) {
         long e = tid + (long)q * nt;
-        if (q < per && e < n) s += (double)vi[e] * (double)W.get(q, e);
+        vc[q] = (q < per && e < n) ? Vr[e] : T(0);
       }
-      s = block_sum(s, red);
-      const T h = (T)s;
+      for (int i = 0; i <= j; ++i) {
+        if (i < j) {
+          const T *vnx = Vr + (long)(i + 1) * n;
 #pragma unroll
-      for (int q = 0; q < cnt; ++q) {
-        long e = tid + (long)q * nt;
-        if (q < per && e < n) W.set(q, e, W.get(q, e) - h * vi[e]);
+          for (int q = 0; q < E; ++q) {
+            long e = tid + (long)q * nt;
+            vn[q] = (q < per && e < n) ? vnx[e] : T(0);
+          }
+        }
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+          long e = tid + (long)q * nt;
+          if (q < per && e < n) s += (double)vc[q] * (double)W.v[q];
+        }
+        s = block_sum_db(s, red2, i);
+        const T h = (T)s;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+          long e = tid + (long)q * nt;
+          if (q < per && e < n) W.v[q] = W.v[q] - h * vc[q];
+        }
+        if (tid == 0) Hr[i * m + j] = h;
+#pragma unroll
+        for (int q = 0; q < E; ++q) vc[q] = vn[q];
       }
-      if (tid == 0) Hr[i * m + j] = h;
+    } else {
+      for (int i = 0; i <= j; ++i) {
+        const T *vi = Vr + (long)i * n;
+        double s = 0;
+        for (int q = 0; q < cnt; ++q) {
+          long e = tid + (long)q * nt;
+          if (q < per && e < n) s += (double)vi[e] * (double)W.get(q, e);
+        }
+        s = block_sum(s, red);
+        const T h = (T)s;
+        for (int q = 0; q < cnt; ++q) {
+          long e = tid + (long)q * nt;
+          if (q < per && e < n) W.set(q, e, W.get(q, e) - h * vi[e]);
+        }
+        if (tid == 0) Hr[i * m + j] = h;
+      }
     }
     double s2 = 0;
 #pragma unroll

@@ -269,3 +269,47 @@ def test_batched_runs_match_solo_runs(lib):
         assert ens.steps[q] == steps
         assert err < 1e-12
         assert abs(d[q][0] - dg["Nu_top"]) <= 1e-12 * abs(dg["Nu_top"])
+
+
+@pytest.mark.parametrize("mfma", [1, 0])
+def test_gemm_variants(lib, mfma):
+    """smx_gemm on the matrix cores and on the vector FMA kernel: transposed
+    operands, two-level batching, row scaling, beta and run masks, against
+    float64 products on the host."""
+    from specmc_amd.engine import Engine
+    e = Engine("const", 16, 24, 2.0, [dict(Ra=1e4)])
+    lib.set_gemm_mfma(mfma)
+    try:
+        rng = np.random.default_rng(11)
+        for (M, N, K, nz1, nz2) in ((37, 71, 29, 1, 1), (130, 96, 145, 3, 2),
+                                    (2112, 145, 96, 1, 1), (64, 96, 96, 1, 33),
+                                    (300, 260, 64, 2, 1)):
+            A = rng.normal(size=(nz1, nz2, K, M))          # stored k-major: A(i,k) at [k][i]
+            Bm = rng.normal(size=(nz1, nz2, K, N))
+            C0 = rng.normal(size=(nz1, nz2, M, N))
+            rs = rng.normal(size=7)
+            dA, dB, dC = (torch.tensor(x, device=e.dev) for x in (A, Bm, C0))
+            lib.gemm(0, M, N, K, 0.5, -1.5, dA.data_ptr(), 1, M, nz2 * K * M, K * M,
+                     dB.data_ptr(), N, 1, nz2 * K * N, K * N,
+                     dC.data_ptr(), N, 1, nz2 * M * N, M * N, nz1, nz2,
+                     torch.tensor(rs, device=e.dev).data_ptr(), 7, None, 0, 1, e.stream())
+            torch.cuda.synchronize()
+            want = 0.5 * rs[np.arange(M) % 7][:, None] * np.einsum("abki,abkj->abij", A, Bm) \
+                - 1.5 * C0
+            assert rel(dC.cpu().numpy(), want) < 1e-13, (M, N, K)
+        # masks: rows of inactive runs keep their values
+        M, N, K, rpr = 96, 40, 33, 24
+        act = torch.tensor([1, 0, 1, 0], dtype=torch.int32, device=e.dev)
+        A = rng.normal(size=(M, K))
+        Bm = rng.normal(size=(K, N))
+        C0 = rng.normal(size=(M, N))
+        dA, dB, dC = (torch.tensor(x, device=e.dev) for x in (A, Bm, C0))
+        lib.gemm(0, M, N, K, 1.0, 0.0, dA.data_ptr(), K, 1, 0, 0, dB.data_ptr(), N, 1, 0, 0,
+                 dC.data_ptr(), N, 1, 0, 0, 1, 1, None, 1, act.data_ptr(), 1, rpr, e.stream())
+        torch.cuda.synchronize()
+        want = C0.copy()
+        on = np.repeat(act.cpu().numpy().astype(bool), rpr)
+        want[on] = (A @ Bm)[on]
+        assert rel(dC.cpu().numpy(), want) < 1e-13
+    finally:
+        lib.set_gemm_mfma(1)
