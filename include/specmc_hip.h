@@ -53,10 +53,11 @@ int smx_gemm(int prec, int M, int N, int K, double alpha, double beta,
              const int *active, int mask_mode, int rows_per_run,
              void *stream);
 
-/* FP64 GEMMs of smx_gemm run on the matrix cores (v_mfma_f64_16x16x4f64)
- * when on != 0 (the default), on the vector FMA kernel otherwise.  Both give
- * the same products to roundoff; the switch is process-wide. */
-int smx_set_gemm_mfma(int on);
+/* Kernel family of the FP64 GEMMs of smx_gemm, process-wide: 0 vector FMA
+ * tiles, 1 (default) matrix cores (v_mfma_f64_16x16x4f64) with 16-wide K
+ * chunks, 2 matrix cores with the whole K extent staged at once when
+ * K <= 160.  All give the same products to roundoff. */
+int smx_set_gemm_mfma(int mode);
 
 /* Batched LU with partial pivoting, in place, column-major N x N matrices
  * (LAPACK getrf semantics).  Replaces scipy.linalg.lu_factor in

@@ -54,19 +54,41 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     for (int b = 0; b < PN; ++b) acc[a][b] = 0;
   const bool a_kfast = (g.acs == 1), b_jfast = (g.bcs == 1);
   for (int k0 = 0; k0 < g.K; k0 += 16) {
+    // all loads of the chunk are issued before any LDS store (clamped
+    // addresses, zero-selected afterwards), so they are in flight together
+    {
+      constexpr int NA = 16 * TM / 256, NB = 16 * TN / 256;
+      T ra[NA], rb[NB];
 #pragma unroll
-    for (int e0 = 0; e0 < 16 * TM; e0 += 256) {
-      int e = e0 + tid, ii, kk;
-      if (a_kfast) { ii = e >> 4; kk = e & 15; } else { kk = e / TM; ii = e % TM; }
-      int i = i0 + ii, k = k0 + kk;
-      As[kk][ii] = (i < g.M && k < g.K) ? A[(long)i * g.ars + (long)k * g.acs] : T(0);
-    }
+      for (int u = 0; u < NA; ++u) {
+        int e = u * 256 + tid, ii, kk;
+        if (a_kfast) { ii = e >> 4; kk = e & 15; } else { kk = e / TM; ii = e % TM; }
+        const int i = i0 + ii, k = k0 + kk;
+        const bool ok = (i < g.M && k < g.K);
+        ra[u] = A[ok ? (long)i * g.ars + (long)k * g.acs : 0];
+        ra[u] = ok ? ra[u] : T(0);
+      }
 #pragma unroll
-    for (int e0 = 0; e0 < 16 * TN; e0 += 256) {
-      int e = e0 + tid, jj, kb;
-      if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e >> 4; kb = e & 15; }
-      int j = j0 + jj, kq = k0 + kb;
-      Bs[kb][jj] = (j < g.N && kq < g.K) ? Bm[(long)kq * g.brs + (long)j * g.bcs] : T(0);
+      for (int u = 0; u < NB; ++u) {
+        int e = u * 256 + tid, jj, kb;
+        if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e >> 4; kb = e & 15; }
+        const int j = j0 + jj, kq = k0 + kb;
+        const bool ok = (j < g.N && kq < g.K);
+        rb[u] = Bm[ok ? (long)kq * g.brs + (long)j * g.bcs : 0];
+        rb[u] = ok ? rb[u] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < NA; ++u) {
+        int e = u * 256 + tid, ii, kk;
+        if (a_kfast) { ii = e >> 4; kk = e & 15; } else { kk = e / TM; ii = e % TM; }
+        As[kk][ii] = ra[u];
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        int e = u * 256 + tid, jj, kb;
+        if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e >> 4; kb = e & 15; }
+        Bs[kb][jj] = rb[u];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -111,9 +133,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 // gfx950; the FP32 16x16x4 form instead holds C[4 (l / 16) + r][l % 16]).
 typedef double v4f64 __attribute__((ext_vector_type(4)));
 
-template <int TM, int TN>
+template <int TM, int TN, int KC>
 __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
-  constexpr int WM = TM / 2, WN = TN / 2, BM = WM / 16, BN = WN / 16, KC = 16;
+  constexpr int WM = TM / 2, WN = TN / 2, BM = WM / 16, BN = WN / 16;
   __shared__ double As[KC][TM + 1];
   __shared__ double Bs[KC][TN + 1];
   const int z = blockIdx.z, z1 = z / g.nz2, z2 = z % g.nz2;
@@ -142,19 +164,39 @@ __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
     for (int b = 0; b < BN; ++b) acc[a][b] = v4f64{0.0, 0.0, 0.0, 0.0};
   const bool a_kfast = (g.acs == 1), b_jfast = (g.bcs == 1);
   for (int k0 = 0; k0 < g.K; k0 += KC) {
+    {
+      constexpr int NA = KC * TM / 256, NB = KC * TN / 256;
+      double ra[NA], rb[NB];
 #pragma unroll
-    for (int e0 = 0; e0 < KC * TM; e0 += 256) {
-      int e = e0 + tid, ii, kk;
-      if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
-      int i = i0 + ii, k = k0 + kk;
-      As[kk][ii] = (i < g.M && k < g.K) ? A[(long)i * g.ars + (long)k * g.acs] : 0.0;
-    }
+      for (int u = 0; u < NA; ++u) {
+        int e = u * 256 + tid, ii, kk;
+        if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
+        const int i = i0 + ii, k = k0 + kk;
+        const bool ok = (i < g.M && k < g.K);
+        ra[u] = A[ok ? (long)i * g.ars + (long)k * g.acs : 0];
+        ra[u] = ok ? ra[u] : 0.0;
+      }
 #pragma unroll
-    for (int e0 = 0; e0 < KC * TN; e0 += 256) {
-      int e = e0 + tid, jj, kb;
-      if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
-      int j = j0 + jj, kq = k0 + kb;
-      Bs[kb][jj] = (j < g.N && kq < g.K) ? Bm[(long)kq * g.brs + (long)j * g.bcs] : 0.0;
+      for (int u = 0; u < NB; ++u) {
+        int e = u * 256 + tid, jj, kb;
+        if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
+        const int j = j0 + jj, kq = k0 + kb;
+        const bool ok = (j < g.N && kq < g.K);
+        rb[u] = Bm[ok ? (long)kq * g.brs + (long)j * g.bcs : 0];
+        rb[u] = ok ? rb[u] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < NA; ++u) {
+        int e = u * 256 + tid, ii, kk;
+        if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
+        As[kk][ii] = ra[u];
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        int e = u * 256 + tid, jj, kb;
+        if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
+        Bs[kb][jj] = rb[u];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -195,7 +237,9 @@ __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
   }
 }
 
-static bool g_gemm_mfma = true;     // FP64 GEMMs on the matrix cores
+// FP64 GEMM kernel family: 0 vector FMA, 1 matrix cores with 16-wide K
+// chunks (default), 2 matrix cores with single-chunk staging of small K
+static int g_gemm_mfma = 1;
 
 // Tile choice: the largest tile that still gives every CU several
 // workgroups (the batched shapes of one step are small: K <= 3Nz/2 and
@@ -208,16 +252,29 @@ static const int kGemmWantWG = 1024;
 template <typename T>
 static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
   const int M = g.M, N = g.N;
-  if (sizeof(T) == 8 && g_gemm_mfma) {
+  if (sizeof(T) == 8 && g_gemm_mfma > 0) {
     // 64 x 64 tiles when they give every CU work and pad little, else 32 x 32
     const long wg64 = (long)cdiv(M, 64) * cdiv(N, 64) * nz;
     const long pad64 = (long)cdiv(M, 64) * 64 * cdiv(N, 64) * 64;
-    if (wg64 >= 512 && pad64 * 4 <= (long)M * N * 5)
-      hipLaunchKernelGGL((k_gemm_mfma64<64, 64>), dim3(cdiv(N, 64), cdiv(M, 64), nz), dim3(256),
-                         0, st, g);
-    else
-      hipLaunchKernelGGL((k_gemm_mfma64<32, 32>), dim3(cdiv(N, 32), cdiv(M, 32), nz), dim3(256),
-                         0, st, g);
+    if (wg64 >= 512 && pad64 * 4 <= (long)M * N * 5) {
+      hipLaunchKernelGGL((k_gemm_mfma64<64, 64, 16>), dim3(cdiv(N, 64), cdiv(M, 64), nz),
+                         dim3(256), 0, st, g);
+      return;
+    }
+    // small problems: 32 x 32 tiles with the whole K extent staged in one
+    // chunk when K <= 160 (one memory latency per launch instead of one per
+    // 16-wide chunk)
+    const dim3 grid(cdiv(N, 32), cdiv(M, 32), nz);
+#define GK(KC_) hipLaunchKernelGGL((k_gemm_mfma64<32, 32, KC_>), grid, dim3(256), 0, st, g)
+    const int K = g_gemm_mfma == 1 ? 1 << 30 : g.K;
+    if (K <= 32) { GK(32); return; }
+    if (K <= 48) { GK(48); return; }
+    if (K <= 64) { GK(64); return; }
+    if (K <= 96) { GK(96); return; }
+    if (K <= 128) { GK(128); return; }
+    if (K <= 160) { GK(160); return; }
+#undef GK
+    hipLaunchKernelGGL((k_gemm_mfma64<32, 32, 16>), grid, dim3(256), 0, st, g);
     return;
   }
   const int nt = (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0]));
@@ -247,8 +304,9 @@ static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
 }
 
 extern "C" int smx_version(void) { return SMX_VERSION; }
-extern "C" int smx_set_gemm_mfma(int on) {
-  g_gemm_mfma = (on != 0);
+extern "C" int smx_set_gemm_mfma(int mode) {
+  if (mode < 0 || mode > 2) return fail_msg("smx_set_gemm_mfma: mode must be 0, 1 or 2");
+  g_gemm_mfma = mode;
   return 0;
 }
 extern "C" const char *smx_last_error(void) { return g_err; }

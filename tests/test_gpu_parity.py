@@ -271,7 +271,7 @@ def test_batched_runs_match_solo_runs(lib):
         assert abs(d[q][0] - dg["Nu_top"]) <= 1e-12 * abs(dg["Nu_top"])
 
 
-@pytest.mark.parametrize("mfma", [1, 0])
+@pytest.mark.parametrize("mfma", [2, 1, 0])
 def test_gemm_variants(lib, mfma):
     """smx_gemm on the matrix cores and on the vector FMA kernel: transposed
     operands, two-level batching, row scaling, beta and run masks, against
@@ -281,7 +281,7 @@ def test_gemm_variants(lib, mfma):
     lib.set_gemm_mfma(mfma)
     try:
         rng = np.random.default_rng(11)
-        for (M, N, K, nz1, nz2) in ((37, 71, 29, 1, 1), (130, 96, 145, 3, 2),
+        for (M, N, K, nz1, nz2) in ((37, 71, 29, 1, 1), (130, 96, 145, 3, 2), (33, 96, 190, 5, 1),
                                     (2112, 145, 96, 1, 1), (64, 96, 96, 1, 33),
                                     (300, 260, 64, 2, 1)):
             A = rng.normal(size=(nz1, nz2, K, M))          # stored k-major: A(i,k) at [k][i]
