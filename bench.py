@@ -73,11 +73,13 @@ PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pc_apply_hard_b64_traffic.json")
 
 
-def _ensemble(states, prec=0, pc=None):
+def _ensemble(states, prec=0, pc=None, ortho=None):
     from specmc_amd import ConvectionConfig
     from specmc_amd.batch import Ensemble
     B = len(states)
     kw = {} if pc is None else dict(pc_apply=pc)
+    if ortho is not None:
+        kw["ortho"] = ortho
     return Ensemble([ConvectionConfig(**WORKLOAD) for _ in range(B)], states,
                     np.full(B, np.inf), prec=prec, **kw)
 
@@ -150,6 +152,8 @@ def main():
     ap.add_argument("--legs", default="plain,batched")
     ap.add_argument("--pc", default=None, choices=("lu", "inv"),
                     help="preconditioner block application (default: the engine's)")
+    ap.add_argument("--ortho", default=None, choices=("mgs", "bgs2"),
+                    help="FGMRES orthogonalisation (default: the engine's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--make-states", action="store_true")
     ap.add_argument("--states-out", default=STATES)
@@ -183,19 +187,20 @@ def main():
         s0 = z["s%d" % step]
         r = {}
         if "plain" in legs:
-            plain = _ensemble([s0], pc=a.pc)
+            plain = _ensemble([s0], pc=a.pc, ortho=a.ortho)
             n1, t1 = _timed(plain, a.steps, a.warmup)
             r["plain"] = n1 / t1
             del plain
             torch.cuda.empty_cache()
         if "batched" in legs:
             sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
-            ens = _ensemble(sts, pc=a.pc)
+            ens = _ensemble(sts, pc=a.pc, ortho=a.ortho)
             n2, t2 = _timed(ens, a.steps, a.warmup, clock)
             unit_bytes = pc_unit_bytes(ens.eng)
             r.update(batched=n2 / t2, ms_batched_step=1e3 * t2 / a.steps,
                      bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
-                     rescues=int(ens.eng.n_rescue.sum()), pc_apply=ens.eng.pc_apply)
+                     rescues=int(ens.eng.n_rescue.sum()), pc_apply=ens.eng.pc_apply,
+                     ortho=ens.eng.ortho)
             del ens
             torch.cuda.empty_cache()
         res[ph] = r

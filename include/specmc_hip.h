@@ -249,8 +249,11 @@ int smx_maxabs(int prec, int B, long n_per_run, const void *x, double *out,
  * (double[B][8]) live in device memory; see the implementation for the
  * layout.  fg_init: x = 0, r = b, beta = ||b||, tol = max(atol, rtol*beta).
  * fg_cycle_end: y = H^{-1} g, x += Z y, then the inputs of this round's
- * preconditioner (V_j) and operator (x) apply.  fg_post: Z_j store, MGS +
- * Givens + the stop tests, or the restart from the true residual.  The input
+ * preconditioner (V_j) and operator (x) apply.  fg_post: Z_j store, the
+ * orthogonalisation (ortho 0: modified Gram-Schmidt in the reference's order;
+ * 1: block Gram-Schmidt with reorthogonalisation, two passes over blocks of
+ * basis vectors, for n <= 8192) + Givens + the stop tests, or the restart
+ * from the true residual.  The input
  * pointer of the next operator apply (Z_j or x).  fg_count: number of runs
  * not yet finished (written to *out).  ctl is int[10][B] and sc double[8][B]
  * (struct of arrays); ctl + 5*B / ctl + 6*B are the preconditioner and
@@ -263,7 +266,8 @@ int smx_fg_cycle_end(int prec, int B, long n, int m, void *x, const void *Zb,
                      void *zout, int *ctl, void *stream);
 int smx_fg_post(int prec, int B, long n, int m, const void *b, void *w,
                 const void *zin, void *Zb, void *Vb, void *H, void *cs,
-                void *sn, void *g, int *ctl, double *sc, void *stream);
+                void *sn, void *g, int *ctl, double *sc, int ortho,
+                void *stream);
 int smx_fg_count(int B, const int *ctl, int *out, void *stream);
 
 /* CFL dt (rbc_galerkin.py:465-471, cfl_policy.py:64-74): per run

@@ -1789,8 +1789,13 @@ struct WVec {
   __device__ __forceinline__ int count() const { return E > 0 ? E : per; }
 };
 
-template <typename T, int E>
-__global__ void __launch_bounds__(1024) k_fg_post(int B, long n, int m, const T *bvec, T *w,
+// ORTHO 0: modified Gram-Schmidt in the reference's order (1024 threads);
+// 1: block Gram-Schmidt with reorthogonalisation (512 threads; blocks of 64/E
+// basis vectors held in registers: one reduction of the block's dot products
+// per block, MGS between blocks, everything twice, H = sum of both passes)
+template <typename T, int E, int ORTHO>
+__global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, int m,
+                                                                const T *bvec, T *w,
                                                   const T *zin, T *Zb, T *Vb, T *H, T *cs,
                                                   T *sn, T *g, int *ctl, double *sc) {
   const int r = blockIdx.x;
@@ -1812,7 +1817,63 @@ __global__ void __launch_bounds__(1024) k_fg_post(int B, long n, int m, const T 
       T *Zj = Zb + ((long)r * m + j) * n;
       for (long e = tid; e < n; e += nt) Zj[e] = zr[e];
     }
-    if constexpr (E > 0) {
+    if constexpr (E > 0 && ORTHO == 1) {
+      constexpr int CH = 64 / E;
+      __shared__ double redc[2][16][CH];
+      const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+      int par = 0;
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int i0 = 0; i0 <= j; i0 += CH) {
+          T vv[CH][E];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            const bool okc = (i0 + c <= j);
+            const T *vi = Vr + (long)(okc ? i0 + c : 0) * n;
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+              long e = tid + (long)q * nt;
+              const bool ok = okc && q < per && e < n;
+              vv[c][q] = vi[ok ? e : 0];
+              vv[c][q] = ok ? vv[c][q] : T(0);
+            }
+          }
+          double part[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            double a = 0;
+#pragma unroll
+            for (int q = 0; q < E; ++q) a += (double)vv[c][q] * (double)W.v[q];
+            part[c] = wave_sum(a);
+          }
+          if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) redc[par][wv][c] = part[c];
+          }
+          __syncthreads();
+          T h[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            double a = 0;
+            for (int u = 0; u < nw; ++u) a += redc[par][u][c];
+            h[c] = (T)a;
+          }
+          par ^= 1;
+#pragma unroll
+          for (int q = 0; q < E; ++q) {
+            T x = W.v[q];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) x -= h[c] * vv[c][q];
+            W.v[q] = x;
+          }
+          if (tid == 0) {
+            for (int c = 0; c < CH && i0 + c <= j; ++c) {
+              T *hp = Hr + (long)(i0 + c) * m + j;
+              *hp = pass == 0 ? h[c] : *hp + h[c];
+            }
+          }
+        }
+      }
+    } else if constexpr (E > 0) {
       // modified Gram-Schmidt with v_{i+1} loaded while h_i is reduced
       __shared__ double red2[32];
       __syncthreads();
@@ -1980,23 +2041,28 @@ __global__ void __launch_bounds__(1024) k_fg_post(int B, long n, int m, const T 
 template <typename T>
 static void launch_fg_post(int B, long n, int m, const void *b, void *w, const void *zin,
                            void *Zb, void *Vb, void *H, void *cs, void *sn, void *g, int *ctl,
-                           double *sc, hipStream_t st) {
-#define FGP(E)                                                                              \
-  hipLaunchKernelGGL((k_fg_post<T, E>), dim3(B), dim3(1024), 0, st, B, n, m, (const T *)b,  \
-                     (T *)w, (const T *)zin, (T *)Zb, (T *)Vb, (T *)H, (T *)cs, (T *)sn,    \
-                     (T *)g, ctl, sc)
-  if (n <= 4L * 1024) FGP(4);
-  else if (n <= 8L * 1024) FGP(8);
-  else FGP(0);
+                           double *sc, int ortho, hipStream_t st) {
+#define FGP(E, O)                                                                             \
+  hipLaunchKernelGGL((k_fg_post<T, E, O>), dim3(B), dim3(O ? 512 : 1024), 0, st, B, n, m,     \
+                     (const T *)b, (T *)w, (const T *)zin, (T *)Zb, (T *)Vb, (T *)H, (T *)cs, \
+                     (T *)sn, (T *)g, ctl, sc)
+  if (n <= 4L * 1024) { if (ortho) FGP(8, 1); else FGP(4, 0); }
+  else if (n <= 8L * 1024) { if (ortho) FGP(16, 1); else FGP(8, 0); }
+  else FGP(0, 0);
 #undef FGP
 }
 
 extern "C" int smx_fg_post(int prec, int B, long n, int m, const void *b, void *w,
                            const void *zin, void *Zb, void *Vb, void *H, void *cs, void *sn,
-                           void *g, int *ctl, double *sc, void *stream) {
+                           void *g, int *ctl, double *sc, int ortho, void *stream) {
   if (B <= 0) return 0;
-  if (prec == 0) launch_fg_post<double>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, STREAM(stream));
-  else launch_fg_post<float>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, STREAM(stream));
+  if (ortho != 0 && ortho != 1) return fail_msg("smx_fg_post: ortho must be 0 or 1");
+  if (prec == 0)
+    launch_fg_post<double>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho,
+                           STREAM(stream));
+  else
+    launch_fg_post<float>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho,
+                          STREAM(stream));
   CHECK_LAUNCH("smx_fg_post");
   return 0;
 }

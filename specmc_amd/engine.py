@@ -33,6 +33,8 @@ PROFILES = {"harm": 0, "diag": 1, "geom": 2}
 
 MEMO_RTOL = 1e-12          # coupled_model.py:327, 749-765
 PC_APPLY = "inv"           # default application of the per-mode preconditioner blocks
+ORTHO = {"mgs": 0, "bgs2": 1}
+ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
 
 
 class Solver:
@@ -53,6 +55,7 @@ class Solver:
         self.sc = torch.zeros(8, B, dtype=torch.float64, device=dev)
         # unfinished-run counts of the last two rounds: device slots, and
         # pinned host copies read one round late
+        self.ortho = ORTHO[eng.ortho]
         self.cnt = torch.zeros(2, dtype=torch.int32, device=dev)
         self.cnt_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
         self.cnt_ev = [torch.cuda.Event(), torch.cuda.Event()]
@@ -87,7 +90,7 @@ class Solver:
             op(self.zbuf, self.w, self.act_mv())
             L.fg_post(P, B, n, m, e.p(self.b), e.p(self.w), e.p(self.zbuf),
                       e.p(self.Z), e.p(self.V), e.p(self.H), e.p(self.cs),
-                      e.p(self.sn), e.p(self.g), e.p(self.ctl), e.p(self.sc), s)
+                      e.p(self.sn), e.p(self.g), e.p(self.ctl), e.p(self.sc), self.ortho, s)
             L.fg_count(B, e.p(self.ctl), e.p(self.cnt, slot), s)
             self.cnt_host[slot:slot + 1].copy_(self.cnt[slot:slot + 1], non_blocking=True)
             self.cnt_ev[slot].record()
@@ -128,7 +131,8 @@ class Engine:
     def __init__(self, kind, Nx, Nz, Lx, params, prec=0, device=None,
                  profile="harm", stokes_rtol=1e-8, stokes_restart=80,
                  stokes_maxiter=800, theta_rtol=1e-10, theta_restart=40,
-                 theta_maxiter=200, defect_tol=1e-8, max_K=8, K=3, pc_apply=None):
+                 theta_maxiter=200, defect_tol=1e-8, max_K=8, K=3, pc_apply=None,
+                 ortho=None):
         self.dev = runtime.require_device() if device is None else device
         self.L = runtime.load()
         pc_apply = PC_APPLY if pc_apply is None else pc_apply
@@ -138,6 +142,9 @@ class Engine:
         # apply ("lu"), or an explicit inverse applied as one matrix-vector
         # product ("inv")
         self.pc_apply = pc_apply
+        self.ortho = ORTHO_DEFAULT if ortho is None else ortho
+        if self.ortho not in ORTHO:
+            raise ValueError("ortho must be one of %s, got %r" % (sorted(ORTHO), ortho))
         self.clock = None          # a kclock.KernelClock to time selected launches
         self.kind = kind
         self.prec = int(prec)

@@ -51,7 +51,7 @@ class _DeviceModel:
                  internal_heating=0.0, dissipation_number=0.0,
                  eta_min=1e-3, eta_max=1e3, eta_smooth=0.0, arrhenius_Ts=0.0,
                  precond="bfbt", symmetrize=True, stokes_maxiter=None,
-                 defect_tol=1e-8, max_K=8, K=3, pc_apply=None, **extra):
+                 defect_tol=1e-8, max_K=8, K=3, pc_apply=None, ortho=None, **extra):
         self.Ra, self.Nx, self.Nz, self.Lx = float(Ra), int(Nx), int(Nz), float(Lx)
         self.internal_heating = float(internal_heating)
         self.dissipation_number = float(dissipation_number)
@@ -68,6 +68,8 @@ class _DeviceModel:
             kw["stokes_maxiter"] = int(stokes_maxiter)
         if pc_apply is not None:
             kw["pc_apply"] = pc_apply
+        if ortho is not None:
+            kw["ortho"] = ortho
         self.eng = Engine(self.kind, self.Nx, self.Nz, self.Lx, [params],
                           prec=self.prec, **kw)
         o = self.eng.ops

@@ -46,7 +46,7 @@ SIGNATURES = {
     "smx_maxabs": [I, I, L, P, P, P, P],
     "smx_fg_init": [I, I, L, I, P, P, P, P, P, P, D, D, I, P, P],
     "smx_fg_cycle_end": [I, I, L, I, P, P, P, P, P, P, P, P, P, P],
-    "smx_fg_post": [I, I, L, I, P, P, P, P, P, P, P, P, P, P, P, P],
+    "smx_fg_post": [I, I, L, I, P, P, P, P, P, P, P, P, P, P, P, I, P],
     "smx_fg_count": [I, P, P, P],
     "smx_cfl_dt": [I, P, P, P, P, P],
 }
