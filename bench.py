@@ -68,9 +68,9 @@ def make_states(path=STATES):
     print(json.dumps(dict(steps=info["steps"], t=info["t"], wall=wall, diag=d)), flush=True)
 
 
-ROOF_KERNEL = "pc_apply"     # the per-mode preconditioner apply of both FGMRES loops
+ROOF_KERNEL = "fg_post"      # FGMRES Arnoldi step (Z_j store, MGS, Givens) of both solves
 PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pc_apply_hard_b64_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fg_post_c1_b64_traffic.json")
 
 
 def _ensemble(states, prec=0, pc=None, ortho=None):
@@ -84,12 +84,11 @@ def _ensemble(states, prec=0, pc=None, ortho=None):
                     np.full(B, np.inf), prec=prec, **kw)
 
 
-def pc_unit_bytes(eng):
-    """Algorithmic HBM bytes of one per-mode preconditioner apply: the block
-    (inverse or LU factors, N*N words; pivots for LU) read once, the input
-    and output vectors (N words each)."""
-    N, w = eng.Nz, eng.esize
-    return N * N * w + 2 * N * w + (4 * N if eng.pc_apply == "lu" else 0)
+def unit_bytes(eng):
+    """Algorithmic HBM bytes per unit the fg_post kernel counts: one vector of
+    length n = Nk*Nz (an Arnoldi step j reads z, w and V_0..V_j and writes Z_j
+    and V_{j+1}: j + 5 vectors; a restart reads the residual and writes V_0)."""
+    return eng.n * eng.esize
 
 
 def _timed(ens, steps, warmup, clock=None):
@@ -97,14 +96,16 @@ def _timed(ens, steps, warmup, clock=None):
     for _ in range(warmup):
         ens.step()
     torch.cuda.synchronize()
-    ens.eng.clock = clock
+    if clock is not None:
+        clock.start()
     t0 = time.perf_counter()
     n = 0
     for _ in range(steps):
         n += ens.step()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    ens.eng.clock = None
+    if clock is not None:
+        clock.stop()
     return n, el
 
 
@@ -155,6 +156,9 @@ def main():
     ap.add_argument("--ortho", default=None, choices=("mgs", "bgs2"),
                     help="FGMRES orthogonalisation (default: the engine's)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="copy MIB MiB device-to-device once first (a known-size "
+                         "reference for hardware byte counters)")
     ap.add_argument("--make-states", action="store_true")
     ap.add_argument("--states-out", default=STATES)
     a = ap.parse_args()
@@ -171,6 +175,12 @@ def main():
         return
     if not os.path.exists(STATES):
         raise SystemExit("missing %s: run `python bench.py --make-states` first" % STATES)
+    if a.copy_probe > 0:
+        x = torch.ones(a.copy_probe << 17, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     z = np.load(STATES, allow_pickle=False)
     log = z["log"]
     w = phase_weights(log)
@@ -178,8 +188,8 @@ def main():
     from specmc_amd.kclock import KernelClock
     legs = a.legs.split(",")
     rng = np.random.default_rng(1)
-    clock = KernelClock([ROOF_KERNEL])
-    unit_bytes = None
+    clock = KernelClock(ROOF_KERNEL)
+    ubytes = None
     res = {}
     for ph, step in zip(names, CHECKPOINTS):
         if ph not in a.phases.split(","):
@@ -196,7 +206,7 @@ def main():
             sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
             ens = _ensemble(sts, pc=a.pc, ortho=a.ortho)
             n2, t2 = _timed(ens, a.steps, a.warmup, clock)
-            unit_bytes = pc_unit_bytes(ens.eng)
+            ubytes = unit_bytes(ens.eng)
             r.update(batched=n2 / t2, ms_batched_step=1e3 * t2 / a.steps,
                      bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
                      rescues=int(ens.eng.n_rescue.sum()), pc_apply=ens.eng.pc_apply,
@@ -225,9 +235,9 @@ def main():
                plain_fp64_steps_per_s=plain_rate,
                speedup_vs_plain=(value / plain_rate if plain_rate else None),
                phases=res)
-    nl, ms, units = clock.summary(ROOF_KERNEL)
+    nl, ms, units = clock.summary()
     if nl:
-        achieved = unit_bytes * units / nl / (ms * 1e-3)
+        achieved = ubytes * units / nl / (ms * 1e-3)
         traffic = None
         if os.path.exists(TRAFFIC_FILE):
             with open(TRAFFIC_FILE) as f:
@@ -235,7 +245,7 @@ def main():
         out["roofline"] = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9,
                                unit="GB/s", frac=achieved / PEAK_HBM, traffic=traffic,
                                kernel=ROOF_KERNEL, launches=nl, mean_launch_us=1e3 * ms,
-                               bytes_per_unit=unit_bytes, units_per_launch=units / nl)
+                               bytes_per_unit=ubytes, units_per_launch=units / nl)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(z["s%d" % CHECKPOINTS[0]])
     print(json.dumps(out), flush=True)

@@ -303,6 +303,87 @@ static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
 #undef GL
 }
 
+// ---------------------------------------------------------------------------
+// launch clock: HIP events recorded on the launch stream immediately around
+// every launch of one chosen entry point (tag 1: smx_fg_post, 2: smx_inv_apply
+// and smx_lu_solve), for measuring a kernel's average duration inside a timed
+// region without host synchronisation
+// ---------------------------------------------------------------------------
+static int g_clk_tag = 0, g_clk_cap = 0, g_clk_n = 0;
+static hipEvent_t *g_clk_ev = nullptr;
+// algorithmic units processed by the clocked launches, counted by the kernels
+// themselves (fg_post: vectors of length n moved; inv_apply: systems solved)
+static unsigned long long *g_clk_units = nullptr;
+
+static inline bool clk_on(int tag) { return g_clk_tag == tag && g_clk_n < g_clk_cap; }
+static inline void clk_begin(hipStream_t st) { hipEventRecord(g_clk_ev[2 * g_clk_n], st); }
+static inline void clk_end(hipStream_t st) {
+  hipEventRecord(g_clk_ev[2 * g_clk_n + 1], st);
+  ++g_clk_n;
+}
+
+static inline unsigned long long *clk_units(int tag) {
+  return clk_on(tag) ? g_clk_units : nullptr;
+}
+
+static void clk_free() {
+  for (int i = 0; i < 2 * g_clk_cap; ++i) hipEventDestroy(g_clk_ev[i]);
+  free(g_clk_ev);
+  g_clk_ev = nullptr;
+  if (g_clk_units) hipFree(g_clk_units);
+  g_clk_units = nullptr;
+  g_clk_cap = g_clk_n = g_clk_tag = 0;
+}
+
+extern "C" int smx_clock_start(int tag, int capacity) {
+  clk_free();
+  if (tag <= 0 || capacity <= 0) return 0;
+  g_clk_ev = (hipEvent_t *)calloc((size_t)2 * capacity, sizeof(hipEvent_t));
+  if (!g_clk_ev) return fail_msg("smx_clock_start: out of host memory");
+  for (int i = 0; i < 2 * capacity; ++i) {
+    hipError_t e = hipEventCreate(&g_clk_ev[i]);
+    if (e != hipSuccess) {
+      g_clk_cap = i / 2;
+      clk_free();
+      return fail("smx_clock_start", e);
+    }
+  }
+  g_clk_cap = capacity;
+  hipError_t e = hipMalloc((void **)&g_clk_units, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(g_clk_units, 0, sizeof(unsigned long long));
+  if (e != hipSuccess) {
+    clk_free();
+    return fail("smx_clock_start", e);
+  }
+  g_clk_tag = tag;
+  return 0;
+}
+
+extern "C" int smx_clock_stop(double *total_ms, int *count, long long *units) {
+  double tot = 0.0;
+  if (units) {
+    *units = 0;
+    if (g_clk_units) {
+      unsigned long long u = 0;
+      hipError_t e = hipMemcpy(&u, g_clk_units, sizeof(u), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return fail("smx_clock_stop", e);
+      *units = (long long)u;
+    }
+  }
+  for (int i = 0; i < g_clk_n; ++i) {
+    hipError_t e = hipEventSynchronize(g_clk_ev[2 * i + 1]);
+    if (e != hipSuccess) return fail("smx_clock_stop", e);
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, g_clk_ev[2 * i], g_clk_ev[2 * i + 1]);
+    if (e != hipSuccess) return fail("smx_clock_stop", e);
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (count) *count = g_clk_n;
+  clk_free();
+  return 0;
+}
+
 extern "C" int smx_version(void) { return SMX_VERSION; }
 extern "C" int smx_set_gemm_mfma(int mode) {
   if (mode < 0 || mode > 2) return fail_msg("smx_set_gemm_mfma: mode must be 0, 1 or 2");
@@ -602,11 +683,13 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_inv_apply(const T *Inv, const T *rhs, T *x, int N,
                                                    int tau_zero, T scale, const T *kscale,
                                                    const T *bscale, int per_run, int shared,
-                                                   const int *active) {
+                                                   const int *active,
+                                                   unsigned long long *units) {
   __shared__ T bs[LUS_MAXN];
   __shared__ T part[4][LUS_MAXN];
   const int m = blockIdx.x;
   if (active && !active[m / per_run]) return;
+  if (units && threadIdx.x == 0) atomicAdd(units, 1ull);
   const long mi = shared ? (m % per_run) : m;
   const T *A = Inv + mi * (long)N * N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -770,16 +853,19 @@ extern "C" int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const v
                              void *stream) {
   if (nmat <= 0) return 0;
   if (N <= 0 || N > LUS_MAXN || per_run <= 0) return fail_msg("smx_inv_apply: bad sizes");
+  const bool clk = clk_on(2);
+  if (clk) clk_begin(STREAM(stream));
   if (prec == 0)
     hipLaunchKernelGGL(k_inv_apply<double>, dim3(nmat), dim3(256), 0, STREAM(stream),
                        (const double *)Inv, (const double *)rhs, (double *)x, N, tau_zero,
                        rhs_scale, (const double *)kscale, (const double *)bscale, per_run,
-                       shared, active);
+                       shared, active, clk_units(2));
   else
     hipLaunchKernelGGL(k_inv_apply<float>, dim3(nmat), dim3(256), 0, STREAM(stream),
                        (const float *)Inv, (const float *)rhs, (float *)x, N, tau_zero,
                        (float)rhs_scale, (const float *)kscale, (const float *)bscale,
-                       per_run, shared, active);
+                       per_run, shared, active, clk_units(2));
+  if (clk) clk_end(STREAM(stream));
   CHECK_LAUNCH("smx_inv_apply");
   return 0;
 }
@@ -1797,10 +1883,15 @@ template <typename T, int E, int ORTHO>
 __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, int m,
                                                                 const T *bvec, T *w,
                                                   const T *zin, T *Zb, T *Vb, T *H, T *cs,
-                                                  T *sn, T *g, int *ctl, double *sc) {
+                                                  T *sn, T *g, int *ctl, double *sc,
+                                                  unsigned long long *units) {
   const int r = blockIdx.x;
   const int mode = ctl[0 * B + r];
   if (mode != FG_ARN && mode != FG_RES && mode != FG_FIN) return;
+  // vectors moved: Arnoldi step j reads z, w, V_0..V_j, writes Z_j, V_{j+1};
+  // a restart or finish reads the residual and writes V_0
+  if (units && threadIdx.x == 0)
+    atomicAdd(units, (unsigned long long)(mode == FG_ARN ? ctl[1 * B + r] + 5 : 2));
   __shared__ double red[16];
   __shared__ int s_next;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -2041,11 +2132,11 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
 template <typename T>
 static void launch_fg_post(int B, long n, int m, const void *b, void *w, const void *zin,
                            void *Zb, void *Vb, void *H, void *cs, void *sn, void *g, int *ctl,
-                           double *sc, int ortho, hipStream_t st) {
+                           double *sc, int ortho, unsigned long long *units, hipStream_t st) {
 #define FGP(E, O)                                                                             \
   hipLaunchKernelGGL((k_fg_post<T, E, O>), dim3(B), dim3(O ? 512 : 1024), 0, st, B, n, m,     \
                      (const T *)b, (T *)w, (const T *)zin, (T *)Zb, (T *)Vb, (T *)H, (T *)cs, \
-                     (T *)sn, (T *)g, ctl, sc)
+                     (T *)sn, (T *)g, ctl, sc, units)
   if (n <= 4L * 1024) { if (ortho) FGP(8, 1); else FGP(4, 0); }
   else if (n <= 8L * 1024) { if (ortho) FGP(16, 1); else FGP(8, 0); }
   else FGP(0, 0);
@@ -2057,12 +2148,16 @@ extern "C" int smx_fg_post(int prec, int B, long n, int m, const void *b, void *
                            void *g, int *ctl, double *sc, int ortho, void *stream) {
   if (B <= 0) return 0;
   if (ortho != 0 && ortho != 1) return fail_msg("smx_fg_post: ortho must be 0 or 1");
+  const bool clk = clk_on(1);
+  if (clk) clk_begin(STREAM(stream));
+  unsigned long long *units = clk_units(1);
   if (prec == 0)
-    launch_fg_post<double>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho,
+    launch_fg_post<double>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho, units,
                            STREAM(stream));
   else
-    launch_fg_post<float>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho,
+    launch_fg_post<float>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho, units,
                           STREAM(stream));
+  if (clk) clk_end(STREAM(stream));
   CHECK_LAUNCH("smx_fg_post");
   return 0;
 }

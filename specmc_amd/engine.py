@@ -145,7 +145,6 @@ class Engine:
         self.ortho = ORTHO_DEFAULT if ortho is None else ortho
         if self.ortho not in ORTHO:
             raise ValueError("ortho must be one of %s, got %r" % (sorted(ORTHO), ortho))
-        self.clock = None          # a kclock.KernelClock to time selected launches
         self.kind = kind
         self.prec = int(prec)
         self.dtype = torch.float64 if self.prec == 0 else torch.float32
@@ -462,15 +461,6 @@ class Engine:
 
     def _pc_apply(self, mats, piv, vin, zout, scale, mask):
         """zout_k = block_k^{-1} (scale * vin_k) with the tau entries zeroed."""
-        ck = self.clock
-        if ck is not None and ck.wants("pc_apply"):
-            ev = ck.begin()
-            self._pc_apply_launch(mats, piv, vin, zout, scale, mask)
-            ck.end("pc_apply", ev, self.B * self.Nk if mask is None else mask * self.Nk)
-        else:
-            self._pc_apply_launch(mats, piv, vin, zout, scale, mask)
-
-    def _pc_apply_launch(self, mats, piv, vin, zout, scale, mask):
         B, Nk, N = self.B, self.Nk, self.Nz
         if self.pc_apply == "inv":
             self.L.inv_apply(self.prec, B * Nk, N, self.p(mats), self.p(vin), self.p(zout), 1,

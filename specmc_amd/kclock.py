@@ -1,45 +1,43 @@
-"""HIP-event timing of selected kernel launches on the engine's stream.
+"""Launch timing of one kernel inside a timed region.
 
-An ``Engine`` with ``clock`` set records an event pair around every launch of
-the named kernels, together with the number of units (systems, rows) the
-launch processes as a device-side count, so a benchmark can report a
-kernel's average launch duration and its achieved bandwidth from the same
-timed region without host synchronisation inside it.
+``KernelClock(name)`` arms the library's launch clock (``smx_clock_start``):
+HIP events are recorded on the launch stream immediately before and after
+every launch of that entry point, from the C side, and the kernel itself
+counts the algorithmic units it processes in a device counter.  Nothing is
+added on the host per launch and nothing synchronises inside the timed
+region; ``stop`` collects the spans and the count.
 """
+import ctypes
+
 import torch
+
+from . import runtime
+
+TAGS = {"fg_post": 1, "pc_apply": 2}
 
 
 class KernelClock:
-    def __init__(self, names):
-        self.names = set(names)
-        self.spans = {k: [] for k in self.names}
-        self.units = {k: [] for k in self.names}
+    def __init__(self, name, capacity=60000):
+        if name not in TAGS:
+            raise ValueError("no launch clock for %r (have %s)" % (name, sorted(TAGS)))
+        self.name, self.capacity = name, int(capacity)
+        self.launches, self.total_ms, self.total_units = 0, 0.0, 0
 
-    def wants(self, name):
-        return name in self.names
+    def start(self):
+        runtime.load().clock_start(TAGS[self.name], self.capacity)
 
-    def begin(self):
-        ev = torch.cuda.Event(enable_timing=True)
-        ev.record()
-        return ev
-
-    def end(self, name, ev0, units):
-        """``units``: int, or a device tensor whose sum is the unit count."""
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev1.record()
-        self.spans[name].append((ev0, ev1))
-        self.units[name].append(units if isinstance(units, int) else units.sum())
-
-    def summary(self, name):
-        """-> (launches, mean launch ms, total units) after a synchronize."""
+    def stop(self):
         torch.cuda.synchronize()
-        sp = self.spans[name]
-        if not sp:
+        ms, n, u = ctypes.c_double(0.0), ctypes.c_int(0), ctypes.c_longlong(0)
+        runtime.load().clock_stop(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(u))
+        if n.value >= self.capacity:
+            raise RuntimeError("launch clock full: raise its capacity above %d" % self.capacity)
+        self.launches += n.value
+        self.total_ms += ms.value
+        self.total_units += u.value
+
+    def summary(self):
+        """-> (launches, mean launch ms, total units) over every start/stop."""
+        if not self.launches:
             return 0, 0.0, 0
-        ms = sum(a.elapsed_time(b) for a, b in sp)
-        u = 0
-        dev = [x for x in self.units[name] if not isinstance(x, int)]
-        if dev:
-            u += int(torch.stack([d.to(torch.int64) for d in dev]).sum().item())
-        u += sum(x for x in self.units[name] if isinstance(x, int))
-        return len(sp), ms / len(sp), u
+        return self.launches, self.total_ms / self.launches, self.total_units

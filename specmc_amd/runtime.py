@@ -23,6 +23,8 @@ SIGNATURES = {
     "smx_lu_factor": [I, I, I, P, P, I, P, P],
     "smx_lu_solve": [I, I, I, P, P, P, P, I, D, P, P, I, I, P, P],
     "smx_invert": [I, I, I, P, I, P, P],
+    "smx_clock_start": [I, I],
+    "smx_clock_stop": [P, P, P],
     "smx_set_gemm_mfma": [I],
     "smx_dense_lu": [I, I, I, P, L, P, P, P],
     "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P],
