@@ -148,7 +148,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--phases", default="early,hard,late")
     ap.add_argument("--legs", default="plain,batched")
     ap.add_argument("--pc", default=None, choices=("lu", "inv"),

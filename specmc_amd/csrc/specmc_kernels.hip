@@ -1717,6 +1717,7 @@ extern "C" int smx_cfl_dt(int B, const double *umax, const double *params, doubl
 // H: [B][(m+1)*m] row-major (i, j) -> i*m + j
 // ---------------------------------------------------------------------------
 enum { FG_ARN = 0, FG_CYC = 1, FG_RES = 2, FG_FIN = 3, FG_DONE = 4 };
+#define FG_MAXM 128   // largest restart length (Krylov dimension) supported
 
 template <typename T>
 __global__ void __launch_bounds__(1024) k_fg_init(int B, long n, int m, const T *b, T *x, T *Vb,
@@ -1903,6 +1904,15 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
     const int j = ctl[1 * B + r];
     T *Vr = Vb + (long)r * (m + 1) * n;
     T *Hr = H + (long)r * (m + 1) * m;
+    // column j of H and the previous rotations live in LDS for this step
+    __shared__ T hcol[FG_MAXM + 1], scs[FG_MAXM], ssn[FG_MAXM];
+    {
+      const T *csr = cs + (long)r * m, *snr = sn + (long)r * m;
+      for (int i = tid; i < j; i += nt) {
+        scs[i] = csr[i];
+        ssn[i] = snr[i];
+      }
+    }
     {
       const T *zr = zin + (long)r * n;
       T *Zj = Zb + ((long)r * m + j) * n;
@@ -1957,10 +1967,8 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
             W.v[q] = x;
           }
           if (tid == 0) {
-            for (int c = 0; c < CH && i0 + c <= j; ++c) {
-              T *hp = Hr + (long)(i0 + c) * m + j;
-              *hp = pass == 0 ? h[c] : *hp + h[c];
-            }
+            for (int c = 0; c < CH && i0 + c <= j; ++c)
+              hcol[i0 + c] = pass == 0 ? h[c] : hcol[i0 + c] + h[c];
           }
         }
       }
@@ -1996,7 +2004,7 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
           long e = tid + (long)q * nt;
           if (q < per && e < n) W.v[q] = W.v[q] - h * vc[q];
         }
-        if (tid == 0) Hr[i * m + j] = h;
+        if (tid == 0) hcol[i] = h;
 #pragma unroll
         for (int q = 0; q < E; ++q) vc[q] = vn[q];
       }
@@ -2014,7 +2022,7 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
           long e = tid + (long)q * nt;
           if (q < per && e < n) W.set(q, e, W.get(q, e) - h * vi[e]);
         }
-        if (tid == 0) Hr[i * m + j] = h;
+        if (tid == 0) hcol[i] = h;
       }
     }
     double s2 = 0;
@@ -2027,15 +2035,17 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
     const double hn = sqrt(s2);
     if (tid == 0) {
       T *csr = cs + (long)r * m, *snr = sn + (long)r * m, *gr = g + (long)r * (m + 1);
-      Hr[(j + 1) * m + j] = (T)hn;
+      hcol[j + 1] = (T)hn;
+      // the previous rotations applied to column j, in LDS
+      T hi = hcol[0];
       for (int i = 0; i < j; ++i) {
-        T a = Hr[i * m + j], bb = Hr[(i + 1) * m + j];
-        T t = csr[i] * a + snr[i] * bb;
-        Hr[(i + 1) * m + j] = -snr[i] * a + csr[i] * bb;
-        Hr[i * m + j] = t;
+        const T a = hi, bb = hcol[i + 1];
+        hcol[i] = scs[i] * a + ssn[i] * bb;
+        hi = -ssn[i] * a + scs[i] * bb;
       }
+      hcol[j] = hi;
       // Givens (krylov_fgmres.py:42-57) for real entries
-      T a = Hr[j * m + j], bb = Hr[(j + 1) * m + j];
+      T a = hcol[j], bb = hcol[j + 1];
       T aa = fabs(a), ab = fabs(bb), c, s, rr;
       if (ab == T(0)) { c = T(1); s = T(0); rr = a; }
       else if (aa == T(0)) { c = T(0); s = bb / ab; rr = ab; }
@@ -2047,8 +2057,8 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
       }
       csr[j] = c;
       snr[j] = s;
-      Hr[j * m + j] = rr;
-      Hr[(j + 1) * m + j] = T(0);
+      hcol[j] = rr;
+      hcol[j + 1] = T(0);
       T t = c * gr[j] + s * gr[j + 1];
       gr[j + 1] = -s * gr[j] + c * gr[j + 1];
       gr[j] = t;
@@ -2070,6 +2080,7 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
       s_next = (rn <= sc[1 * B + r] || hn == 0.0) ? 0 : 1;
     }
     __syncthreads();
+    for (int i = tid; i <= j + 1; i += nt) Hr[(long)i * m + j] = hcol[i];
     if (s_next) {
       T *vn = Vr + (long)(j + 1) * n;
       const T hv = (T)hn;
@@ -2148,6 +2159,7 @@ extern "C" int smx_fg_post(int prec, int B, long n, int m, const void *b, void *
                            void *g, int *ctl, double *sc, int ortho, void *stream) {
   if (B <= 0) return 0;
   if (ortho != 0 && ortho != 1) return fail_msg("smx_fg_post: ortho must be 0 or 1");
+  if (m <= 0 || m > FG_MAXM) return fail_msg("smx_fg_post: restart length must be 1..128");
   const bool clk = clk_on(1);
   if (clk) clk_begin(STREAM(stream));
   unsigned long long *units = clk_units(1);
