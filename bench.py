@@ -70,7 +70,7 @@ def make_states(path=STATES):
 
 ROOF_KERNEL = "fg_post"      # FGMRES Arnoldi step (Z_j store, MGS, Givens) of both solves
 PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fg_post_c1_b64_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fg_post_c1_b256_traffic.json")
 
 
 def _ensemble(states, prec=0, pc=None, ortho=None):
