@@ -73,15 +73,29 @@ PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fg_post_c1_b256_traffic.json")
 
 
-def _ensemble(states, prec=0, pc=None, ortho=None):
+def _ensemble(states, prec=0, pc=None, ortho=None, precision="fp64", graph_rounds=0):
     from specmc_amd import ConvectionConfig
     from specmc_amd.batch import Ensemble
     B = len(states)
-    kw = {} if pc is None else dict(pc_apply=pc)
+    kw = dict(graph_rounds=graph_rounds)
+    if pc is not None:
+        kw["pc_apply"] = pc
     if ortho is not None:
         kw["ortho"] = ortho
     return Ensemble([ConvectionConfig(**WORKLOAD) for _ in range(B)], states,
-                    np.full(B, np.inf), prec=prec, **kw)
+                    np.full(B, np.inf), prec=prec, precision=precision, **kw)
+
+
+def _diag_errors(ens, ref_states, ref_diag):
+    """Relative errors of the states and of Nu_top / Vrms of ``ens`` against
+    FP64 results of the same inputs: max over runs."""
+    got = ens.states()
+    d = ens.diagnostics()
+    eth = max(float(np.linalg.norm(got[q] - ref_states[q]) / np.linalg.norm(ref_states[q]))
+              for q in range(len(got)))
+    enu = float(np.max(np.abs(d[:, 0] - ref_diag[:, 0]) / np.abs(ref_diag[:, 0])))
+    evr = float(np.max(np.abs(d[:, 3] - ref_diag[:, 3]) / np.abs(ref_diag[:, 3])))
+    return dict(theta_l2=eth, Nu=enu, Vrms=evr)
 
 
 def unit_bytes(eng):
@@ -143,6 +157,36 @@ def cpu_baseline(state, nsteps=2):
                        "phase are early_plain/early_batched" % (nsteps, CHECKPOINTS[0]))
 
 
+def served_leg(z, log, w, B, value, horizon, load, nreq, precision, graph_rounds=0,
+               seed=5):
+    """A Poisson stream of config-1 requests resumed from the phase checkpoints
+    (phase drawn with the phase weights, state perturbed by 1e-3 relative),
+    each marching ``horizon`` steps' worth of simulated time, arriving at
+    ``load`` x the completion rate the batched form sustains, served by one
+    bucket of ``B`` slots with continuous admission and retirement."""
+    from specmc_amd import ConvectionConfig
+    from specmc_amd.serve import Request, Server
+    rng = np.random.default_rng(seed)
+    dt = np.diff(np.r_[0.0, log[:, 1]])
+    rate = load * value / horizon
+    arr = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    ph = rng.choice(len(CHECKPOINTS), size=nreq, p=w)
+    cfg = ConvectionConfig(**WORKLOAD)
+    reqs = []
+    for i in range(nreq):
+        ck = CHECKPOINTS[ph[i]]
+        s0 = np.real(z["s%d" % ck]) * (1.0 + 1e-3 * rng.normal())
+        reqs.append(Request(i, cfg, s0, horizon * dt[ck - 1], arr[i]))
+    srv = Server(slots=B, precision=precision, diagnostics=False, graph_rounds=graph_rounds)
+    srv.submit(reqs)
+    srv.run()
+    st = srv.stats()
+    st.update(rate_per_s=rate, horizon_steps=horizon, load=load, slots=B,
+              phases=[int((ph == k).sum()) for k in range(len(CHECKPOINTS))])
+    st.pop("buckets", None)
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,11 +194,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--phases", default="early,hard,late")
-    ap.add_argument("--legs", default="plain,batched")
+    ap.add_argument("--legs", default="plain,batched,mixed,served",
+                    help="plain: one FP64 run; batched: B FP64 runs; mixed: B runs in mixed "
+                         "precision (the reported value); served: Poisson stream")
     ap.add_argument("--pc", default=None, choices=("lu", "inv"),
                     help="preconditioner block application (default: the engine's)")
     ap.add_argument("--ortho", default=None, choices=("mgs", "bgs2"),
                     help="FGMRES orthogonalisation (default: the engine's)")
+    ap.add_argument("--graph-rounds", type=int, default=4,
+                    help="FGMRES rounds per HIP graph replay in the mixed and served legs "
+                         "(0: eager launches)")
+    ap.add_argument("--served-horizon", type=int, default=4)
+    ap.add_argument("--served-load", type=float, default=0.8)
+    ap.add_argument("--served-requests", type=int, default=0,
+                    help="requests of the served leg (default: the batch size)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device once first (a known-size "
@@ -195,6 +248,7 @@ def main():
         if ph not in a.phases.split(","):
             continue
         s0 = z["s%d" % step]
+        sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
         r = {}
         if "plain" in legs:
             plain = _ensemble([s0], pc=a.pc, ortho=a.ortho)
@@ -202,8 +256,8 @@ def main():
             r["plain"] = n1 / t1
             del plain
             torch.cuda.empty_cache()
+        ref = None
         if "batched" in legs:
-            sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
             ens = _ensemble(sts, pc=a.pc, ortho=a.ortho)
             n2, t2 = _timed(ens, a.steps, a.warmup, clock)
             ubytes = unit_bytes(ens.eng)
@@ -211,6 +265,19 @@ def main():
                      bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
                      rescues=int(ens.eng.n_rescue.sum()), pc_apply=ens.eng.pc_apply,
                      ortho=ens.eng.ortho)
+            if "mixed" in legs:
+                ref = (ens.states(), ens.diagnostics())
+            del ens
+            torch.cuda.empty_cache()
+        if "mixed" in legs:
+            ens = _ensemble(sts, pc=a.pc, ortho=a.ortho, precision="mixed",
+                            graph_rounds=a.graph_rounds)
+            n3, t3 = _timed(ens, a.steps, a.warmup)
+            r.update(mixed=n3 / t3, ms_mixed_step=1e3 * t3 / a.steps,
+                     rescues_fp32=int(ens.eng.n_rescue_fp32.sum()),
+                     rescues_fp32_redone=int(ens.eng.n_rescue_fp32_redo.sum()))
+            if ref is not None:
+                r["mixed_vs_fp64"] = _diag_errors(ens, *ref)
             del ens
             torch.cuda.empty_cache()
         res[ph] = r
@@ -219,22 +286,30 @@ def main():
     ws = ws / ws.sum()
     # whole-run rates: time per step is the weighted harmonic combination
     rate = lambda leg: (1.0 / sum(wi / res[p][leg] for wi, p in zip(ws, res))
-                        if all(leg in res[p] for p in res) else None)
-    plain_rate, value = rate("plain"), rate("batched")
-    if value is None:
-        value = plain_rate
-    out = dict(metric="batched steps/s at 64x96 eta_T deta=1e3 vs plain FP64 GPU",
+                        if res and all(leg in res[p] for p in res) else None)
+    plain_rate, fp64_rate, mixed_rate = rate("plain"), rate("batched"), rate("mixed")
+    value = next(v for v in (mixed_rate, fp64_rate, plain_rate, 0.0) if v is not None)
+    dtype = "f64+f32" if mixed_rate is not None else "f64"
+    out = dict(metric="batched steps/s at 64x96 eta_T deta=1e3 vs plain FP64 GPU; "
+                      "p50/p99 run latency",
                value=value, unit="run-steps/s", n_gpus=1, steps=a.steps,
-               warmup=a.warmup, ms_per_step=1e3 * a.batch / value,
-               higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               warmup=a.warmup, ms_per_step=(1e3 * a.batch / value if value else None),
+               higher_is_better=True, scaling="weak", vs_baseline=None, dtype=dtype,
                data="synthetic",
-               config=dict(workload="config 1: 64x96 eta_T deta_T=1e3 Ra=1e4 Lx=2 bfbt, "
-                                    "tstop=5e-3 trajectory phases",
-                           batch=a.batch, Nx=64, Nz=96, precision="fp64",
+               config=dict(workload="64x96 eta_T deta_T=1e3 Ra=1e4 unit box (Lx=2 carrier), "
+                                    "BFBT route, tstop=5e-3 trajectory phases",
+                           batch=a.batch, Nx=64, Nz=96,
+                           precision=("mixed: FP64 Krylov/operators/blocks, FP32 dense "
+                                      "rescue LU under FP64 defect correction"
+                                      if mixed_rate is not None else "fp64"),
                            phase_weights=dict(zip(names, [float(x) for x in w]))),
-               plain_fp64_steps_per_s=plain_rate,
+               plain_fp64_steps_per_s=plain_rate, batched_fp64_steps_per_s=fp64_rate,
+               batched_mixed_steps_per_s=mixed_rate,
                speedup_vs_plain=(value / plain_rate if plain_rate else None),
                phases=res)
+    errs = [res[p]["mixed_vs_fp64"] for p in res if "mixed_vs_fp64" in res[p]]
+    if errs:
+        out["mixed_error_vs_fp64"] = {k: max(e[k] for e in errs) for k in errs[0]}
     nl, ms, units = clock.summary()
     if nl:
         achieved = ubytes * units / nl / (ms * 1e-3)
@@ -246,6 +321,13 @@ def main():
                                unit="GB/s", frac=achieved / PEAK_HBM, traffic=traffic,
                                kernel=ROOF_KERNEL, launches=nl, mean_launch_us=1e3 * ms,
                                bytes_per_unit=ubytes, units_per_launch=units / nl)
+    if "served" in legs and value:
+        sv = served_leg(z, log, w, a.batch, value, a.served_horizon, a.served_load,
+                        a.served_requests or a.batch,
+                        "mixed" if mixed_rate is not None else "fp64", a.graph_rounds)
+        out["served"] = sv
+        out["p50_s"], out["p99_s"] = sv["p50_s"], sv["p99_s"]
+        print("served: %s" % json.dumps(sv), file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(z["s%d" % CHECKPOINTS[0]])
     print(json.dumps(out), flush=True)

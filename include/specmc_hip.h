@@ -65,11 +65,38 @@ int smx_gemm(int prec, int M, int N, int K, double alpha, double beta,
              const int *active, int mask_mode, int rows_per_run,
              void *stream);
 
-/* Kernel family of the FP64 GEMMs of smx_gemm, process-wide: 0 vector FMA
- * tiles, 1 (default) matrix cores (v_mfma_f64_16x16x4f64) with 16-wide K
- * chunks, 2 matrix cores with the whole K extent staged at once when
- * K <= 160.  All give the same products to roundoff. */
+/* smx_gemm with a role tag (no counterpart in the reference): the role only
+ * names the kernel symbol, so kernel traces separate the work: 0 z-transform
+ * or z-derivative (what smx_gemm launches), 1 x-transform (the half-grid
+ * DFTs), 2 per-mode operators (Green applies, BFBT / reduced-operator
+ * builds), 3 dense-matrix work of the rescue route (Schur assembly, LU
+ * trailing update). */
+int smx_gemm_ex(int role, int prec, int M, int N, int K, double alpha, double beta,
+                const void *A, long ars, long acs, long sa1, long sa2,
+                const void *B, long brs, long bcs, long sb1, long sb2,
+                void *C, long crs, long ccs, long sc1, long sc2,
+                int nz1, int nz2, const void *rscale, int rperiod,
+                const int *active, int mask_mode, int rows_per_run,
+                void *stream);
+
+/* Kernel family of the GEMMs of smx_gemm without a tuned choice,
+ * process-wide: 0 vector FMA tiles, 1 (default) matrix cores
+ * (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) with 16-wide K chunks, 2
+ * matrix cores with the whole K extent staged at once when K <= 160.  All
+ * give the same products to roundoff. */
 int smx_set_gemm_mfma(int mode);
+
+/* Autotuner table (no counterpart in the reference; the choices are measured
+ * by specmc_amd.autotune): the kernel used for GEMMs of precision `prec`,
+ * shape M x N x K, nz batch entries and operand layout (a_kfast: acs == 1,
+ * b_jfast: bcs == 1).  code: 0..6 vector FMA tiles 128x96, 128x64, 64x96,
+ * 64x64, 64x32, 32x64, 32x32; 10 matrix cores 64x64 (K chunks of 16);
+ * 11 matrix cores 32x32 (K chunks of 16); 12 matrix cores 32x32 with the
+ * whole K (<= 160) staged at once; a negative code removes the entry.
+ * smx_gemm_tune_clear empties the table.  Process-wide. */
+int smx_gemm_tune(int prec, int M, int N, int K, int nz, int a_kfast, int b_jfast,
+                  int code);
+int smx_gemm_tune_clear(void);
 
 /* Batched LU with partial pivoting, in place, column-major N x N matrices
  * (LAPACK getrf semantics).  Replaces scipy.linalg.lu_factor in

@@ -86,10 +86,16 @@ def model_kwargs(cfg):
     return dict(model_spec(cfg)["kwargs"])
 
 
-def build_model(cfg: ConvectionConfig):
-    """Build the device model for ``cfg`` and check its class (api.py:102-110)."""
+def build_model(cfg: ConvectionConfig, precision="fp64"):
+    """Build the device model for ``cfg`` and check its class (api.py:102-110).
+
+    ``precision`` is an extension of this package: "fp64" (default) runs every
+    kernel in FP64; "mixed" factors the dense rescue systems in FP32 under the
+    FP64 defect correction (the engine's ``PRECISIONS``)."""
     spec = model_spec(cfg)
     kw = dict(spec["kwargs"])
+    if precision != "fp64":
+        kw["precision"] = precision
     if kw.get("precond") == "legacy" and not _warned_legacy[0]:
         _warned_legacy[0] = True
         warnings.warn("precond='legacy' runs the BFBT-preconditioned FGMRES route on "

@@ -1,6 +1,9 @@
 // dense_lu.hip -- batched dense LU with partial pivoting and its solves, for
 // the dense Schur systems of the BFBT rescue route (n = Nk*Nz: 3168 at
-// 64x96).  FP64, row-major n x n matrices, matrix q at A + q*mstride.
+// 64x96).  FP64 or FP32 (T), row-major n x n matrices, matrix q at
+// A + q*mstride.  The FP32 factor serves the mixed-precision rescue: its
+// solves take and return FP64 vectors and the caller's FP64 defect
+// correction against the matrix-free operator refines the result.
 //
 // Blocked right-looking factorisation, block width NB = 64:
 //   for each block [j0, j0+jb):
@@ -26,43 +29,43 @@
 // ---------------------------------------------------------------------------
 // sub-panel factorisation: rows s0 + tid + 1024 k (k < R), columns s0..s0+PB-1
 // ---------------------------------------------------------------------------
-template <int R, int PB>
-__global__ void __launch_bounds__(1024) k_lud_panel(double *A, long ms, int n, int s0, int *piv,
+template <typename T, int R, int PB>
+__global__ void __launch_bounds__(1024) k_lud_panel(T *A, long ms, int n, int s0, int *piv,
                                                     int *info) {
   const int q = blockIdx.x;
-  double *M = A + (long)q * ms;
+  T *M = A + (long)q * ms;
   int *P = piv + (long)q * n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  __shared__ double rowp[PB], rowc[PB];
-  __shared__ double rv[16];
+  __shared__ T rowp[PB], rowc[PB];
+  __shared__ T rv[16];
   __shared__ int ri[16];
   __shared__ int s_p;
-  double v[R][PB];
+  T v[R][PB];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int r = s0 + tid + k * 1024;
 #pragma unroll
     for (int c = 0; c < PB; ++c)
-      v[k][c] = (r < n && s0 + c < n) ? M[(long)r * n + s0 + c] : 0.0;
+      v[k][c] = (r < n && s0 + c < n) ? M[(long)r * n + s0 + c] : T(0);
   }
 #pragma unroll
   for (int c = 0; c < PB; ++c) {
     const int col = s0 + c;
     if (col >= n) break;
     // pivot: first row of largest |a| at or below the diagonal
-    double best = -1.0;
+    T best = T(-1);
     int bi = n;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int r = s0 + tid + k * 1024;
-      const double a = fabs(v[k][c]);
+      const T a = fabs(v[k][c]);
       if (r >= col && r < n && a > best) {
         best = a;
         bi = r;
       }
     }
     for (int o = 32; o > 0; o >>= 1) {
-      const double ob = __shfl_xor(best, o, 64);
+      const T ob = __shfl_xor(best, o, 64);
       const int oi = __shfl_xor(bi, o, 64);
       if (ob > best || (ob == best && oi < bi)) {
         best = ob;
@@ -75,7 +78,7 @@ __global__ void __launch_bounds__(1024) k_lud_panel(double *A, long ms, int n, i
     }
     __syncthreads();
     if (tid == 0) {
-      double b = rv[0];
+      T b = rv[0];
       int i = ri[0];
       for (int u = 1; u < 16; ++u)
         if (rv[u] > b || (rv[u] == b && ri[u] < i)) {
@@ -124,14 +127,14 @@ __global__ void __launch_bounds__(1024) k_lud_panel(double *A, long ms, int n, i
           }
       }
     }
-    const double u = rowp[c];
-    if (u != 0.0) {
-      const double inv = 1.0 / u;
+    const T u = rowp[c];
+    if (u != T(0)) {
+      const T inv = T(1) / u;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const int r = s0 + tid + k * 1024;
         if (r > col && r < n) {
-          const double l = v[k][c] * inv;
+          const T l = v[k][c] * inv;
           v[k][c] = l;
 #pragma unroll
           for (int cc = c + 1; cc < PB; ++cc) v[k][cc] = fma(-l, rowp[cc], v[k][cc]);
@@ -158,24 +161,24 @@ __global__ void __launch_bounds__(1024) k_lud_panel(double *A, long ms, int n, i
 // of the block.  Left groups take the swaps; right groups take the swaps, the
 // U rows and the rank-PB update of rows s0+PB..n-1.
 // ---------------------------------------------------------------------------
-template <int PB>
-__global__ void __launch_bounds__(256) k_lud_block_update(double *A, long ms, int n, int j0,
+template <typename T, int PB>
+__global__ void __launch_bounds__(256) k_lud_block_update(T *A, long ms, int n, int j0,
                                                           int jb, int s0, const int *piv) {
   const int q = blockIdx.y;
-  double *M = A + (long)q * ms;
+  T *M = A + (long)q * ms;
   const int *P = piv + (long)q * n;
   const int g0 = j0 + blockIdx.x * PB;
   if (g0 == s0 || g0 >= j0 + jb) return;
   const int ncol = min(PB, j0 + jb - g0);
   const int npiv = min(PB, n - s0);
   const int tid = threadIdx.x;
-  __shared__ double u[PB][PB];
+  __shared__ T u[PB][PB];
   if (tid < ncol) {
     const int c = g0 + tid;
     for (int i = 0; i < npiv; ++i) {
       const int r = s0 + i, pr = P[r];
       if (pr != r) {
-        const double t = M[(long)r * n + c];
+        const T t = M[(long)r * n + c];
         M[(long)r * n + c] = M[(long)pr * n + c];
         M[(long)pr * n + c] = t;
       }
@@ -184,13 +187,13 @@ __global__ void __launch_bounds__(256) k_lud_block_update(double *A, long ms, in
   if (g0 < s0) return;
   if (tid < ncol) {
     const int c = g0 + tid;
-    double x[PB];
+    T x[PB];
 #pragma unroll
-    for (int i = 0; i < PB; ++i) x[i] = (i < npiv) ? M[(long)(s0 + i) * n + c] : 0.0;
+    for (int i = 0; i < PB; ++i) x[i] = (i < npiv) ? M[(long)(s0 + i) * n + c] : T(0);
 #pragma unroll
     for (int i = 1; i < PB; ++i) {
       if (i < npiv) {
-        double a = x[i];
+        T a = x[i];
 #pragma unroll
         for (int j = 0; j < i; ++j) a = fma(-M[(long)(s0 + i) * n + s0 + j], x[j], a);
         x[i] = a;
@@ -207,8 +210,8 @@ __global__ void __launch_bounds__(256) k_lud_block_update(double *A, long ms, in
   const int cc = tid % PB, rstep = 256 / PB;
   if (cc < ncol) {
     for (int r = s0 + npiv + tid / PB; r < n; r += rstep) {
-      const double *Lr = M + (long)r * n + s0;
-      double a = M[(long)r * n + g0 + cc];
+      const T *Lr = M + (long)r * n + s0;
+      T a = M[(long)r * n + g0 + cc];
 #pragma unroll
       for (int j = 0; j < PB; ++j)
         if (j < npiv) a = fma(-Lr[j], u[j][cc], a);
@@ -218,18 +221,19 @@ __global__ void __launch_bounds__(256) k_lud_block_update(double *A, long ms, in
 }
 
 // the block's row swaps on the columns outside [j0, j0+jb), one thread per column
-__global__ void __launch_bounds__(256) k_lud_laswp(double *A, long ms, int n, int j0, int jb,
+template <typename T>
+__global__ void __launch_bounds__(256) k_lud_laswp(T *A, long ms, int n, int j0, int jb,
                                                    const int *piv) {
   const int q = blockIdx.y;
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= n - jb) return;
   const int c = t < j0 ? t : t + jb;
-  double *M = A + (long)q * ms;
+  T *M = A + (long)q * ms;
   const int *P = piv + (long)q * n;
   for (int i = j0; i < j0 + jb; ++i) {
     const int pr = P[i];
     if (pr != i) {
-      const double x = M[(long)i * n + c];
+      const T x = M[(long)i * n + c];
       M[(long)i * n + c] = M[(long)pr * n + c];
       M[(long)pr * n + c] = x;
     }
@@ -237,10 +241,11 @@ __global__ void __launch_bounds__(256) k_lud_laswp(double *A, long ms, int n, in
 }
 
 // U12 = L11^{-1} A12 for rows j0..j0+jb-1 and columns j0+jb..n-1
-__global__ void __launch_bounds__(256) k_lud_trsm(double *A, long ms, int n, int j0, int jb) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_lud_trsm(T *A, long ms, int n, int j0, int jb) {
   const int q = blockIdx.y;
-  double *M = A + (long)q * ms;
-  __shared__ double L[LUD_NB][LUD_NB + 1];
+  T *M = A + (long)q * ms;
+  __shared__ T L[LUD_NB][LUD_NB + 1];
   const int tid = threadIdx.x;
   for (int e = tid; e < jb * jb; e += 256) {
     const int i = e / jb, j = e % jb;
@@ -249,13 +254,13 @@ __global__ void __launch_bounds__(256) k_lud_trsm(double *A, long ms, int n, int
   __syncthreads();
   const int c = j0 + jb + blockIdx.x * 256 + tid;
   if (c >= n) return;
-  double x[LUD_NB];
+  T x[LUD_NB];
 #pragma unroll
-  for (int i = 0; i < LUD_NB; ++i) x[i] = (i < jb) ? M[(long)(j0 + i) * n + c] : 0.0;
+  for (int i = 0; i < LUD_NB; ++i) x[i] = (i < jb) ? M[(long)(j0 + i) * n + c] : T(0);
 #pragma unroll
   for (int i = 1; i < LUD_NB; ++i) {
     if (i < jb) {
-      double a = x[i];
+      T a = x[i];
 #pragma unroll
       for (int j = 0; j < i; ++j) a = fma(-L[i][j], x[j], a);
       x[i] = a;
@@ -271,26 +276,26 @@ __global__ void k_lud_zero(int *v, int n) {
   if (i < n) v[i] = 0;
 }
 
-template <int R, int PB>
-static int lud_run(int nmat, int n, double *A, long ms, int *piv, int *info, hipStream_t s) {
+template <typename T, int R, int PB>
+static int lud_run(int nmat, int n, T *A, long ms, int *piv, int *info, hipStream_t s) {
   for (int j0 = 0; j0 < n; j0 += LUD_NB) {
     const int jb = min(LUD_NB, n - j0);
     for (int s0 = j0; s0 < j0 + jb; s0 += PB) {
-      hipLaunchKernelGGL((k_lud_panel<R, PB>), dim3(nmat), dim3(1024), 0, s, A, ms, n, s0, piv,
-                         info);
-      hipLaunchKernelGGL(k_lud_block_update<PB>, dim3(cdiv(jb, PB), nmat), dim3(256), 0, s, A,
-                         ms, n, j0, jb, s0, piv);
+      hipLaunchKernelGGL((k_lud_panel<T, R, PB>), dim3(nmat), dim3(1024), 0, s, A, ms, n, s0,
+                         piv, info);
+      hipLaunchKernelGGL((k_lud_block_update<T, PB>), dim3(cdiv(jb, PB), nmat), dim3(256), 0, s,
+                         A, ms, n, j0, jb, s0, piv);
     }
     if (n - jb > 0)
-      hipLaunchKernelGGL(k_lud_laswp, dim3(cdiv(n - jb, 256), nmat), dim3(256), 0, s, A, ms, n,
-                         j0, jb, piv);
+      hipLaunchKernelGGL(k_lud_laswp<T>, dim3(cdiv(n - jb, 256), nmat), dim3(256), 0, s, A, ms,
+                         n, j0, jb, piv);
     const int nr = n - j0 - jb;
     if (nr > 0) {
-      hipLaunchKernelGGL(k_lud_trsm, dim3(cdiv(nr, 256), nmat), dim3(256), 0, s, A, ms, n, j0,
-                         jb);
+      hipLaunchKernelGGL(k_lud_trsm<T>, dim3(cdiv(nr, 256), nmat), dim3(256), 0, s, A, ms, n,
+                         j0, jb);
       CHECK_LAUNCH("smx_dense_lu");
       const long lda = n;
-      int rc = smx_gemm(0, nr, nr, jb, -1.0, 1.0,
+      int rc = smx_gemm_ex(3, sizeof(T) == 8 ? 0 : 1, nr, nr, jb, -1.0, 1.0,
                         A + (long)(j0 + jb) * lda + j0, lda, 1, ms, 0,
                         A + (long)j0 * lda + j0 + jb, lda, 1, ms, 0,
                         A + (long)(j0 + jb) * lda + j0 + jb, lda, 1, ms, 0,
@@ -305,40 +310,49 @@ static int lud_run(int nmat, int n, double *A, long ms, int *piv, int *info, hip
 extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, int *piv,
                             int *info, void *stream) {
   if (nmat <= 0) return 0;
-  if (prec != 0) return fail_msg("smx_dense_lu: FP64 only");
+  if (prec != 0 && prec != 1) return fail_msg("smx_dense_lu: prec must be 0 (FP64) or 1 (FP32)");
   if (n <= 0 || n > LUD_MAXN || mstride < (long)n * n)
     return fail_msg("smx_dense_lu: bad sizes (1 <= n <= 8192, mstride >= n*n)");
   hipStream_t s = STREAM(stream);
   hipLaunchKernelGGL(k_lud_zero, dim3(cdiv(nmat, 256)), dim3(256), 0, s, info, nmat);
+  if (prec == 1) {
+    float *a = (float *)A;
+    if (n <= 1024) return lud_run<float, 1, 8>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 2048) return lud_run<float, 2, 8>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 4096) return lud_run<float, 4, 8>(nmat, n, a, mstride, piv, info, s);
+    return lud_run<float, 8, 4>(nmat, n, a, mstride, piv, info, s);
+  }
   double *a = (double *)A;
-  if (n <= 1024) return lud_run<1, 8>(nmat, n, a, mstride, piv, info, s);
-  if (n <= 2048) return lud_run<2, 8>(nmat, n, a, mstride, piv, info, s);
-  if (n <= 4096) return lud_run<4, 8>(nmat, n, a, mstride, piv, info, s);
-  return lud_run<8, 4>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 1024) return lud_run<double, 1, 8>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 2048) return lud_run<double, 2, 8>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 4096) return lud_run<double, 4, 8>(nmat, n, a, mstride, piv, info, s);
+  return lud_run<double, 8, 4>(nmat, n, a, mstride, piv, info, s);
 }
 
 // ---------------------------------------------------------------------------
 // solve: y = P b, L y' = y (unit lower), U x = y'; one 1024-thread workgroup
 // per matrix with the vector in LDS; 64-row blocks: a matrix-vector product
 // against the solved part (four rows per wavefront at a time) and a
-// wavefront-serial solve of the diagonal block staged in LDS.
+// wavefront-serial solve of the diagonal block staged in LDS.  T is the
+// factor's type and the arithmetic type; TV the type of b and x.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_lud_solve(const double *A, long ms, int n,
-                                                    const int *piv, const double *b, double *x) {
+template <typename T, typename TV>
+__global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
+                                                    const int *piv, const TV *b, TV *x) {
   extern __shared__ unsigned char smem_raw[];
-  double *y = (double *)smem_raw;
-  double *Db = y + n;                       // [64][65]
+  T *y = (T *)smem_raw;
+  T *Db = y + n;                            // [64][65]
   const int q = blockIdx.x;
-  const double *M = A + (long)q * ms;
+  const T *M = A + (long)q * ms;
   const int *P = piv + (long)q * n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  for (int i = tid; i < n; i += 1024) y[i] = b[(long)q * n + i];
+  for (int i = tid; i < n; i += 1024) y[i] = (T)b[(long)q * n + i];
   __syncthreads();
   if (tid == 0)
     for (int i = 0; i < n; ++i) {
       const int p = P[i];
       if (p != i && p >= 0 && p < n) {
-        const double t = y[i];
+        const T t = y[i];
         y[i] = y[p];
         y[p] = t;
       }
@@ -349,18 +363,18 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const double *A, long ms, in
     const int i0 = bk * 64, bs = min(64, n - i0);
     // y[i0 + t] -= L[i0 + t, 0:i0] . y[0:i0]
     {
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      const double *rows[4];
+      T acc[4] = {T(0), T(0), T(0), T(0)};
+      const T *rows[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) rows[u] = M + (long)(i0 + min(w + 16 * u, bs - 1)) * n;
       for (int j = lane; j < i0; j += 64) {
-        const double yj = y[j];
+        const T yj = y[j];
 #pragma unroll
         for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const double a = wave_sum(acc[u]);
+        const T a = wave_sum(acc[u]);
         if (lane == 0 && w + 16 * u < bs) y[i0 + w + 16 * u] -= a;
       }
     }
@@ -370,9 +384,9 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const double *A, long ms, in
     }
     __syncthreads();
     if (w == 0) {
-      double yv = lane < bs ? y[i0 + lane] : 0.0;
+      T yv = lane < bs ? y[i0 + lane] : T(0);
       for (int t = 0; t < bs; ++t) {
-        const double yt = __shfl(yv, t, 64);
+        const T yt = __shfl(yv, t, 64);
         if (lane > t && lane < bs) yv = fma(-Db[lane * 65 + t], yt, yv);
       }
       if (lane < bs) y[i0 + lane] = yv;
@@ -382,18 +396,18 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const double *A, long ms, in
   for (int bk = nblk - 1; bk >= 0; --bk) {
     const int i0 = bk * 64, bs = min(64, n - i0), j1 = i0 + bs;
     {
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      const double *rows[4];
+      T acc[4] = {T(0), T(0), T(0), T(0)};
+      const T *rows[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) rows[u] = M + (long)(i0 + min(w + 16 * u, bs - 1)) * n;
       for (int j = j1 + lane; j < n; j += 64) {
-        const double yj = y[j];
+        const T yj = y[j];
 #pragma unroll
         for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const double a = wave_sum(acc[u]);
+        const T a = wave_sum(acc[u]);
         if (lane == 0 && w + 16 * u < bs) y[i0 + w + 16 * u] -= a;
       }
     }
@@ -403,29 +417,46 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const double *A, long ms, in
     }
     __syncthreads();
     if (w == 0) {
-      double yv = lane < bs ? y[i0 + lane] : 0.0;
+      T yv = lane < bs ? y[i0 + lane] : T(0);
       for (int t = bs - 1; t >= 0; --t) {
         if (lane == t) yv = yv / Db[t * 65 + t];
-        const double yt = __shfl(yv, t, 64);
+        const T yt = __shfl(yv, t, 64);
         if (lane < t) yv = fma(-Db[lane * 65 + t], yt, yv);
       }
       if (lane < bs) y[i0 + lane] = yv;
     }
     __syncthreads();
   }
-  for (int i = tid; i < n; i += 1024) x[(long)q * n + i] = y[i];
+  for (int i = tid; i < n; i += 1024) x[(long)q * n + i] = (TV)y[i];
 }
 
 extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
                                   const int *piv, const void *b, void *x, void *stream) {
   if (nmat <= 0) return 0;
-  if (prec != 0) return fail_msg("smx_dense_lu_solve: FP64 only");
+  if (prec < 0 || prec > 2)
+    return fail_msg("smx_dense_lu_solve: prec must be 0 (FP64), 1 (FP32) or 2 (FP32 factor, "
+                    "FP64 vectors)");
   if (n <= 0 || n > LUD_MAXN || mstride < (long)n * n)
     return fail_msg("smx_dense_lu_solve: bad sizes (1 <= n <= 8192, mstride >= n*n)");
-  const size_t sh = (size_t)n * 8 + 64 * 65 * 8;
-  if (!lds_optin(k_lud_solve, sh)) return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
-  hipLaunchKernelGGL(k_lud_solve, dim3(nmat), dim3(1024), sh, STREAM(stream),
-                     (const double *)LU, mstride, n, piv, (const double *)b, (double *)x);
+  const size_t es = prec == 0 ? 8 : 4;
+  const size_t sh = (size_t)n * es + 64 * 65 * es;
+  hipStream_t st = STREAM(stream);
+  if (prec == 0) {
+    if (!lds_optin(k_lud_solve<double, double>, sh))
+      return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
+    hipLaunchKernelGGL((k_lud_solve<double, double>), dim3(nmat), dim3(1024), sh, st,
+                       (const double *)LU, mstride, n, piv, (const double *)b, (double *)x);
+  } else if (prec == 1) {
+    if (!lds_optin(k_lud_solve<float, float>, sh))
+      return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
+    hipLaunchKernelGGL((k_lud_solve<float, float>), dim3(nmat), dim3(1024), sh, st,
+                       (const float *)LU, mstride, n, piv, (const float *)b, (float *)x);
+  } else {
+    if (!lds_optin(k_lud_solve<float, double>, sh))
+      return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
+    hipLaunchKernelGGL((k_lud_solve<float, double>), dim3(nmat), dim3(1024), sh, st,
+                       (const float *)LU, mstride, n, piv, (const double *)b, (double *)x);
+  }
   CHECK_LAUNCH("smx_dense_lu_solve");
   return 0;
 }

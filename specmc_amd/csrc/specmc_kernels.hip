@@ -25,7 +25,11 @@ struct GemmArgs {
   const int *active; int mask_mode, rows_per_run;
 };
 
-template <typename T, int TM, int TN>
+// R: role of the product, part of the kernel symbol only (so kernel traces
+// separate them): 0 z-transform / z-derivative, 1 x-transform, 2 per-mode
+// operators (Green applies, block builds), 3 dense-matrix work of the rescue
+// (assembly, LU update)
+template <typename T, int TM, int TN, int R>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   constexpr int PM = TM / 16, PN = TN / 16;
   __shared__ T As[16][TM + 1];
@@ -125,19 +129,39 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   }
 }
 
-// FP64 matrix-core GEMM, same arguments and masking as k_gemm.  Four
-// wavefronts in a 2 x 2 arrangement; each owns a (TM/2) x (TN/2) sub-tile made
-// of 16 x 16 blocks computed by v_mfma_f64_16x16x4f64.  Operand layout of that
-// instruction: lane l supplies A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16];
-// accumulator element r of lane l is C[l / 16 + 4 r][l % 16] (measured on
-// gfx950; the FP32 16x16x4 form instead holds C[4 (l / 16) + r][l % 16]).
+// Matrix-core GEMM, same arguments and masking as k_gemm.  Four wavefronts in
+// a 2 x 2 arrangement; each owns a (TM/2) x (TN/2) sub-tile made of 16 x 16
+// blocks computed by v_mfma_f64_16x16x4f64 (T = double) or
+// v_mfma_f32_16x16x4f32 (T = float).  Operand layout of both: lane l supplies
+// A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]; accumulator element r of
+// lane l is C[l / 16 + 4 r][l % 16] for the FP64 form and
+// C[4 (l / 16) + r][l % 16] for the FP32 form (measured on gfx950).
 typedef double v4f64 __attribute__((ext_vector_type(4)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
 
-template <int TM, int TN, int KC>
-__global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<double> {
+  typedef v4f64 V;
+  static __device__ __forceinline__ V step(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator element r for lane-row group lk
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct MfmaAcc<float> {
+  typedef v4f32 V;
+  static __device__ __forceinline__ V step(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+template <typename T, int TM, int TN, int KC, int R>
+__global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   constexpr int WM = TM / 2, WN = TN / 2, BM = WM / 16, BN = WN / 16;
-  __shared__ double As[KC][TM + 1];
-  __shared__ double Bs[KC][TN + 1];
+  typedef typename MfmaAcc<T>::V VT;
+  __shared__ T As[KC][TM + 1];
+  __shared__ T Bs[KC][TN + 1];
   const int z = blockIdx.z, z1 = z / g.nz2, z2 = z % g.nz2;
   const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
   if (g.active) {
@@ -151,22 +175,22 @@ __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
       if (!any) return;
     }
   }
-  const double *A = (const double *)g.A + z1 * g.sa1 + z2 * g.sa2;
-  const double *Bm = (const double *)g.B + z1 * g.sb1 + z2 * g.sb2;
-  double *C = (double *)g.C + z1 * g.sc1 + z2 * g.sc2;
+  const T *A = (const T *)g.A + z1 * g.sa1 + z2 * g.sa2;
+  const T *Bm = (const T *)g.B + z1 * g.sb1 + z2 * g.sb2;
+  T *C = (T *)g.C + z1 * g.sc1 + z2 * g.sc2;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = (w >> 1) * WM, wc = (w & 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
-  v4f64 acc[BM][BN];
+  VT acc[BM][BN];
 #pragma unroll
   for (int a = 0; a < BM; ++a)
 #pragma unroll
-    for (int b = 0; b < BN; ++b) acc[a][b] = v4f64{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < BN; ++b) acc[a][b] = VT{T(0), T(0), T(0), T(0)};
   const bool a_kfast = (g.acs == 1), b_jfast = (g.bcs == 1);
   for (int k0 = 0; k0 < g.K; k0 += KC) {
     {
       constexpr int NA = KC * TM / 256, NB = KC * TN / 256;
-      double ra[NA], rb[NB];
+      T ra[NA], rb[NB];
 #pragma unroll
       for (int u = 0; u < NA; ++u) {
         int e = u * 256 + tid, ii, kk;
@@ -174,7 +198,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
         const int i = i0 + ii, k = k0 + kk;
         const bool ok = (i < g.M && k < g.K);
         ra[u] = A[ok ? (long)i * g.ars + (long)k * g.acs : 0];
-        ra[u] = ok ? ra[u] : 0.0;
+        ra[u] = ok ? ra[u] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
@@ -183,7 +207,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
         const int j = j0 + jj, kq = k0 + kb;
         const bool ok = (j < g.N && kq < g.K);
         rb[u] = Bm[ok ? (long)kq * g.brs + (long)j * g.bcs : 0];
-        rb[u] = ok ? rb[u] : 0.0;
+        rb[u] = ok ? rb[u] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < NA; ++u) {
@@ -201,7 +225,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < KC; ks += 4) {
-      double a[BM], b[BN];
+      T a[BM], b[BN];
 #pragma unroll
       for (int q = 0; q < BM; ++q) a[q] = As[ks + lk][wr + 16 * q + lr];
 #pragma unroll
@@ -210,35 +234,35 @@ __global__ void __launch_bounds__(256) k_gemm_mfma64(GemmArgs g) {
       for (int p = 0; p < BM; ++p)
 #pragma unroll
         for (int q = 0; q < BN; ++q)
-          acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p], b[q], acc[p][q], 0, 0, 0);
+          acc[p][q] = MfmaAcc<T>::step(a[p], b[q], acc[p][q]);
     }
     __syncthreads();
   }
-  const double alpha = g.alpha, beta = g.beta;
-  const double *rs = (const double *)g.rscale;
+  const T alpha = (T)g.alpha, beta = (T)g.beta;
+  const T *rs = (const T *)g.rscale;
 #pragma unroll
   for (int p = 0; p < BM; ++p) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = i0 + wr + 16 * p + lk + 4 * r;
+      const int i = i0 + wr + 16 * p + MfmaAcc<T>::row(lk, r);
       if (i >= g.M) continue;
       if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
-      const double al = rs ? alpha * rs[i % g.rperiod] : alpha;
+      const T al = rs ? alpha * rs[i % g.rperiod] : alpha;
 #pragma unroll
       for (int q = 0; q < BN; ++q) {
         const int j = j0 + wc + 16 * q + lr;
         if (j >= g.N) continue;
         const long o = (long)i * g.crs + (long)j * g.ccs;
-        double v = al * acc[p][q][r];
-        if (beta != 0.0) v += beta * C[o];
+        T v = al * acc[p][q][r];
+        if (beta != T(0)) v += beta * C[o];
         C[o] = v;
       }
     }
   }
 }
 
-// FP64 GEMM kernel family: 0 vector FMA, 1 matrix cores with 16-wide K
-// chunks (default), 2 matrix cores with single-chunk staging of small K
+// GEMM kernel family: 0 vector FMA, 1 matrix cores with 16-wide K chunks
+// (default), 2 matrix cores with single-chunk staging of small K
 static int g_gemm_mfma = 1;
 
 // Tile choice: the largest tile that still gives every CU several
@@ -249,24 +273,42 @@ static const GemmTile kGemmTiles[] = {{128, 96}, {128, 64}, {64, 96}, {64, 64},
                                       {64, 32}, {32, 64}, {32, 32}};
 static const int kGemmWantWG = 1024;
 
-template <typename T>
-static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
+// Explicit kernel choices per problem shape, set by the autotuner
+// (smx_gemm_tune).  Codes: 0..6 vector FMA with tile kGemmTiles[code];
+// 10 matrix cores 64 x 64, K chunks of 16; 11 matrix cores 32 x 32, K chunks
+// of 16; 12 matrix cores 32 x 32, the whole K (<= 160) staged in one chunk.
+struct GemmTune { int prec, M, N, K, nz, akf, bjf, code; };
+#define GEMM_TUNE_MAX 256
+static GemmTune g_tune[GEMM_TUNE_MAX];
+static int g_ntune = 0;
+
+static int tune_lookup(int prec, const GemmArgs &g, int nz) {
+  const int akf = g.acs == 1, bjf = g.bcs == 1;
+  for (int i = 0; i < g_ntune; ++i) {
+    const GemmTune &t = g_tune[i];
+    if (t.prec == prec && t.M == g.M && t.N == g.N && t.K == g.K && t.nz == nz &&
+        t.akf == akf && t.bjf == bjf)
+      return t.code;
+  }
+  return -1;
+}
+
+static bool gemm_code_ok(int code, int K) {
+  return (code >= 0 && code <= 6) || code == 10 || code == 11 || (code == 12 && K <= 160);
+}
+
+template <typename T, int R>
+static void launch_gemm_code(const GemmArgs &g, int nz, hipStream_t st, int code) {
   const int M = g.M, N = g.N;
-  if (sizeof(T) == 8 && g_gemm_mfma > 0) {
-    // 64 x 64 tiles when they give every CU work and pad little, else 32 x 32
-    const long wg64 = (long)cdiv(M, 64) * cdiv(N, 64) * nz;
-    const long pad64 = (long)cdiv(M, 64) * 64 * cdiv(N, 64) * 64;
-    if (wg64 >= 512 && pad64 * 4 <= (long)M * N * 5) {
-      hipLaunchKernelGGL((k_gemm_mfma64<64, 64, 16>), dim3(cdiv(N, 64), cdiv(M, 64), nz),
-                         dim3(256), 0, st, g);
-      return;
-    }
-    // small problems: 32 x 32 tiles with the whole K extent staged in one
-    // chunk when K <= 160 (one memory latency per launch instead of one per
-    // 16-wide chunk)
+  if (code == 10) {
+    hipLaunchKernelGGL((k_gemm_mfma<T, 64, 64, 16, R>), dim3(cdiv(N, 64), cdiv(M, 64), nz),
+                       dim3(256), 0, st, g);
+    return;
+  }
+  if (code == 11 || code == 12) {
     const dim3 grid(cdiv(N, 32), cdiv(M, 32), nz);
-#define GK(KC_) hipLaunchKernelGGL((k_gemm_mfma64<32, 32, KC_>), grid, dim3(256), 0, st, g)
-    const int K = g_gemm_mfma == 1 ? 1 << 30 : g.K;
+#define GK(KC_) hipLaunchKernelGGL((k_gemm_mfma<T, 32, 32, KC_, R>), grid, dim3(256), 0, st, g)
+    const int K = code == 11 ? 1 << 30 : g.K;
     if (K <= 32) { GK(32); return; }
     if (K <= 48) { GK(48); return; }
     if (K <= 64) { GK(64); return; }
@@ -274,24 +316,13 @@ static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
     if (K <= 128) { GK(128); return; }
     if (K <= 160) { GK(160); return; }
 #undef GK
-    hipLaunchKernelGGL((k_gemm_mfma64<32, 32, 16>), grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL((k_gemm_mfma<T, 32, 32, 16, R>), grid, dim3(256), 0, st, g);
     return;
   }
-  const int nt = (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0]));
-  // the largest tile with enough workgroups and at most 25% padding; else
-  // the tile with the most workgroups
-  int pick = -1;
-  for (int t = 0; t < nt && pick < 0; ++t) {
-    const int tm = kGemmTiles[t].tm, tn = kGemmTiles[t].tn;
-    const long wg = (long)cdiv(M, tm) * cdiv(N, tn) * nz;
-    const long padded = (long)cdiv(M, tm) * tm * cdiv(N, tn) * tn;
-    if (wg >= kGemmWantWG && padded * 4 <= (long)M * N * 5) pick = t;
-  }
-  if (pick < 0) pick = nt - 1;
 #define GL(TM_, TN_)                                                                      \
-  hipLaunchKernelGGL((k_gemm<T, TM_, TN_>), dim3(cdiv(N, TN_), cdiv(M, TM_), nz),          \
+  hipLaunchKernelGGL((k_gemm<T, TM_, TN_, R>), dim3(cdiv(N, TN_), cdiv(M, TM_), nz),       \
                      dim3(256), 0, st, g)
-  switch (pick) {
+  switch (code) {
     case 0: GL(128, 96); break;
     case 1: GL(128, 64); break;
     case 2: GL(64, 96); break;
@@ -301,6 +332,38 @@ static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
     default: GL(32, 32); break;
   }
 #undef GL
+}
+
+// the default choice when the autotuner has not set one for this shape
+static int gemm_default_code(const GemmArgs &g, int nz) {
+  const int M = g.M, N = g.N;
+  if (g_gemm_mfma > 0) {
+    // 64 x 64 tiles when they give every CU work and pad little, else 32 x 32
+    const long wg64 = (long)cdiv(M, 64) * cdiv(N, 64) * nz;
+    const long pad64 = (long)cdiv(M, 64) * 64 * cdiv(N, 64) * 64;
+    if (wg64 >= 512 && pad64 * 4 <= (long)M * N * 5) return 10;
+    // small problems: 32 x 32 tiles, with the whole K extent staged in one
+    // chunk (family 2: one memory latency per launch instead of one per
+    // 16-wide chunk)
+    return (g_gemm_mfma == 2 && g.K <= 160) ? 12 : 11;
+  }
+  const int nt = (int)(sizeof(kGemmTiles) / sizeof(kGemmTiles[0]));
+  // the largest tile with enough workgroups and at most 25% padding; else
+  // the tile with the most workgroups
+  for (int t = 0; t < nt; ++t) {
+    const int tm = kGemmTiles[t].tm, tn = kGemmTiles[t].tn;
+    const long wg = (long)cdiv(M, tm) * cdiv(N, tn) * nz;
+    const long padded = (long)cdiv(M, tm) * tm * cdiv(N, tn) * tn;
+    if (wg >= kGemmWantWG && padded * 4 <= (long)M * N * 5) return t;
+  }
+  return nt - 1;
+}
+
+template <typename T, int R>
+static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
+  int code = tune_lookup(sizeof(T) == 8 ? 0 : 1, g, nz);
+  if (code < 0) code = gemm_default_code(g, nz);
+  launch_gemm_code<T, R>(g, nz, st, code);
 }
 
 // ---------------------------------------------------------------------------
@@ -392,6 +455,38 @@ extern "C" int smx_set_gemm_mfma(int mode) {
 }
 extern "C" const char *smx_last_error(void) { return g_err; }
 
+template <typename T>
+static void launch_gemm_role(int role, const GemmArgs &g, int nz, hipStream_t st) {
+  switch (role) {
+    case 1: launch_gemm<T, 1>(g, nz, st); break;
+    case 2: launch_gemm<T, 2>(g, nz, st); break;
+    case 3: launch_gemm<T, 3>(g, nz, st); break;
+    default: launch_gemm<T, 0>(g, nz, st); break;
+  }
+}
+
+extern "C" int smx_gemm_ex(int role, int prec, int M, int N, int K, double alpha, double beta,
+                           const void *A, long ars, long acs, long sa1, long sa2,
+                           const void *B, long brs, long bcs, long sb1, long sb2,
+                           void *C, long crs, long ccs, long sc1, long sc2,
+                           int nz1, int nz2, const void *rscale, int rperiod,
+                           const int *active, int mask_mode, int rows_per_run,
+                           void *stream) {
+  if (M <= 0 || N <= 0 || nz1 <= 0 || nz2 <= 0) return 0;
+  if (role < 0 || role > 3) return fail_msg("smx_gemm_ex: role must be 0..3");
+  if (K < 0) return fail_msg("smx_gemm: negative K");
+  if (rscale && rperiod <= 0) return fail_msg("smx_gemm: rperiod must be > 0");
+  if (mask_mode == 1 && rows_per_run <= 0) return fail_msg("smx_gemm: rows_per_run");
+  if ((long)nz1 * nz2 > 65535) return fail_msg("smx_gemm: batch > 65535");
+  GemmArgs g{M, N, K, alpha, beta, A, ars, acs, sa1, sa2, B, brs, bcs, sb1, sb2,
+             C, crs, ccs, sc1, sc2, nz2, rscale, rperiod, active, mask_mode,
+             rows_per_run};
+  if (prec == 0) launch_gemm_role<double>(role, g, nz1 * nz2, STREAM(stream));
+  else launch_gemm_role<float>(role, g, nz1 * nz2, STREAM(stream));
+  CHECK_LAUNCH("smx_gemm");
+  return 0;
+}
+
 extern "C" int smx_gemm(int prec, int M, int N, int K, double alpha, double beta,
                         const void *A, long ars, long acs, long sa1, long sa2,
                         const void *B, long brs, long bcs, long sb1, long sb2,
@@ -399,18 +494,36 @@ extern "C" int smx_gemm(int prec, int M, int N, int K, double alpha, double beta
                         int nz1, int nz2, const void *rscale, int rperiod,
                         const int *active, int mask_mode, int rows_per_run,
                         void *stream) {
-  if (M <= 0 || N <= 0 || nz1 <= 0 || nz2 <= 0) return 0;
-  if (K < 0) return fail_msg("smx_gemm: negative K");
-  if (rscale && rperiod <= 0) return fail_msg("smx_gemm: rperiod must be > 0");
-  if (mask_mode == 1 && rows_per_run <= 0) return fail_msg("smx_gemm: rows_per_run");
-  if ((long)nz1 * nz2 > 65535L * 65535L) return fail_msg("smx_gemm: batch too large");
-  GemmArgs g{M, N, K, alpha, beta, A, ars, acs, sa1, sa2, B, brs, bcs, sb1, sb2,
-             C, crs, ccs, sc1, sc2, nz2, rscale, rperiod, active, mask_mode,
-             rows_per_run};
-  if ((long)nz1 * nz2 > 65535) return fail_msg("smx_gemm: batch > 65535");
-  if (prec == 0) launch_gemm<double>(g, nz1 * nz2, STREAM(stream));
-  else launch_gemm<float>(g, nz1 * nz2, STREAM(stream));
-  CHECK_LAUNCH("smx_gemm");
+  return smx_gemm_ex(0, prec, M, N, K, alpha, beta, A, ars, acs, sa1, sa2, B, brs, bcs, sb1,
+                     sb2, C, crs, ccs, sc1, sc2, nz1, nz2, rscale, rperiod, active, mask_mode,
+                     rows_per_run, stream);
+}
+
+extern "C" int smx_gemm_tune(int prec, int M, int N, int K, int nz, int a_kfast, int b_jfast,
+                             int code) {
+  if (prec != 0 && prec != 1) return fail_msg("smx_gemm_tune: prec must be 0 or 1");
+  if (code >= 0 && !gemm_code_ok(code, K)) return fail_msg("smx_gemm_tune: bad kernel code");
+  const int akf = a_kfast != 0, bjf = b_jfast != 0;
+  for (int i = 0; i < g_ntune; ++i) {
+    GemmTune &t = g_tune[i];
+    if (t.prec == prec && t.M == M && t.N == N && t.K == K && t.nz == nz && t.akf == akf &&
+        t.bjf == bjf) {
+      if (code >= 0) {
+        t.code = code;
+      } else {
+        t = g_tune[--g_ntune];
+      }
+      return 0;
+    }
+  }
+  if (code < 0) return 0;
+  if (g_ntune >= GEMM_TUNE_MAX) return fail_msg("smx_gemm_tune: table full");
+  g_tune[g_ntune++] = GemmTune{prec, M, N, K, nz, akf, bjf, code};
+  return 0;
+}
+
+extern "C" int smx_gemm_tune_clear(void) {
+  g_ntune = 0;
   return 0;
 }
 

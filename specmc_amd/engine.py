@@ -25,7 +25,7 @@ Layouts (row-major, run-major)
 import numpy as np
 import torch
 
-from . import runtime
+from . import kclock, runtime
 from .basis import shape_ops
 
 KIND_CONST, KIND_DEPTH, KIND_COUPLED = "const", "eta_z", "coupled"
@@ -33,8 +33,27 @@ PROFILES = {"harm": 0, "diag": 1, "geom": 2}
 
 MEMO_RTOL = 1e-12          # coupled_model.py:327, 749-765
 PC_APPLY = "inv"           # default application of the per-mode preconditioner blocks
+# "fp64": every kernel in FP64.  "mixed": the dense rescue factorisation in
+# FP32 (FP32 LU, FP32 triangular solves) under the FP64 defect correction
+# against the FP64 matrix-free operator, which refines its result to the
+# reference's acceptance test; runs whose FP32 attempt misses it are redone
+# with the FP64 factor.  The Krylov operators, preconditioner blocks,
+# Gram-Schmidt, least squares and Helmholtz solves stay FP64.
+PRECISIONS = ("fp64", "mixed")
+RESCUE_BYTES = 24 << 30    # HBM for dense rescue matrices factored together
+# FGMRES of the mixed-precision rescue (FP32 dense LU as the preconditioner)
+RESCUE_RESTART, RESCUE_MAXITER, RESCUE_RTOL = 20, 60, 1e-12
 ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
+
+
+def _law_row(p):
+    """Per-run viscosity-law parameters as the eta kernels read them:
+    (ln deta_T, Ts, eta_min, eta_max, eta_smooth, arrhenius flag)."""
+    Ts = float(p.get("Ts", 0.0))
+    return np.array([np.log(float(p.get("deta_T", 1000.0))), Ts,
+                     float(p.get("eta_min", 1e-3)), float(p.get("eta_max", 1e3)),
+                     float(p.get("eta_smooth", 0.0)), 1.0 if Ts > 0.0 else 0.0])
 
 
 class Solver:
@@ -60,6 +79,7 @@ class Solver:
         self.cnt_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
         self.cnt_ev = [torch.cuda.Event(), torch.cuda.Event()]
         self.rounds = 0
+        self._graphs = {}
 
     def act_pre(self):
         return self.ctl[5]
@@ -67,40 +87,81 @@ class Solver:
     def act_mv(self):
         return self.ctl[6]
 
-    def solve(self, precond, op, active=None, maxiter=None):
-        """Run FGMRES on every active run; ``precond(vin, zout, mask)`` and
-        ``op(zin, wout, mask)`` apply M^{-1} and A to the masked runs."""
+    def _round(self, precond, op):
+        """Enqueue one FGMRES round: cycle end, M^{-1}, A, Arnoldi step."""
         e, L = self.e, self.e.L
         B, n, m, P = e.B, self.n, self.m, e.prec
         s = e.stream()
+        L.fg_cycle_end(P, B, n, m, e.p(self.x), e.p(self.Z), e.p(self.V),
+                       e.p(self.H), e.p(self.g), e.p(self.y), e.p(self.vin),
+                       e.p(self.zbuf), e.p(self.ctl), s)
+        precond(self.vin, self.zbuf, self.act_pre())
+        op(self.zbuf, self.w, self.act_mv())
+        L.fg_post(P, B, n, m, e.p(self.b), e.p(self.w), e.p(self.zbuf),
+                  e.p(self.Z), e.p(self.V), e.p(self.H), e.p(self.cs),
+                  e.p(self.sn), e.p(self.g), e.p(self.ctl), e.p(self.sc), self.ortho, s)
+
+    def _poll(self, slot):
+        """Count the unfinished runs into device slot ``slot`` and start its
+        copy to the pinned host slot."""
+        e = self.e
+        e.L.fg_count(e.B, e.p(self.ctl), e.p(self.cnt, slot), e.stream())
+        self.cnt_host[slot:slot + 1].copy_(self.cnt[slot:slot + 1], non_blocking=True)
+        self.cnt_ev[slot].record()
+
+    def solve(self, precond, op, active=None, maxiter=None, key=None):
+        """Run FGMRES on every active run; ``precond(vin, zout, mask)`` and
+        ``op(zin, wout, mask)`` apply M^{-1} and A to the masked runs.
+
+        With ``key`` set and the engine's ``graph_rounds`` G > 0, the rounds
+        run as replays of a HIP graph of G rounds captured once per ``key``
+        (the launches of a round have fixed arguments: buffers, sizes and the
+        device-side masks), and the host checks completion once per replay.
+        Rounds after a run's completion skip every kernel for it, so the
+        result is the eager one."""
+        e, L = self.e, self.e.L
+        B, n, m, P = e.B, self.n, self.m, e.prec
         mi = self.maxiter if maxiter is None else int(maxiter)
         L.fg_init(P, B, n, m, e.p(self.b), e.p(self.x), e.p(self.V), e.p(self.g),
-                  e.p(self.ctl), e.p(self.sc), self.rtol, 0.0, mi, e.p(active), s)
-        # Round r is enqueued before the host waits for the count of round
-        # r-1, so the device never idles on the host's check; a run that is
-        # done skips every kernel of a round (the masks), so the one round
-        # enqueued after the last run finished is a no-op.
+                  e.p(self.ctl), e.p(self.sc), self.rtol, 0.0, mi, e.p(active), e.stream())
+        G = int(getattr(e, "graph_rounds", 0)) if key is not None else 0
+        if G > 0 and kclock.armed():
+            G = 0             # clocked launches stay eager
+        graph = None
         rounds = 0
+        if G > 0:
+            graph = self._graphs.get(key)
+            if graph is None:
+                # one eager round first: kernel attributes and lazily queried
+                # device limits are set outside the capture
+                self._round(precond, op)
+                rounds += 1
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    for _ in range(G):
+                        self._round(precond, op)
+                self._graphs[key] = graph
+        # Round (or replay) r is enqueued before the host waits for the
+        # count of r-1, so the device never idles on the host's check; a run
+        # that is done skips every kernel of a round (the masks), so the work
+        # enqueued after the last run finished is a no-op.
+        polls = 0
         while True:
-            slot = rounds & 1
-            L.fg_cycle_end(P, B, n, m, e.p(self.x), e.p(self.Z), e.p(self.V),
-                           e.p(self.H), e.p(self.g), e.p(self.y), e.p(self.vin),
-                           e.p(self.zbuf), e.p(self.ctl), s)
-            precond(self.vin, self.zbuf, self.act_pre())
-            op(self.zbuf, self.w, self.act_mv())
-            L.fg_post(P, B, n, m, e.p(self.b), e.p(self.w), e.p(self.zbuf),
-                      e.p(self.Z), e.p(self.V), e.p(self.H), e.p(self.cs),
-                      e.p(self.sn), e.p(self.g), e.p(self.ctl), e.p(self.sc), self.ortho, s)
-            L.fg_count(B, e.p(self.ctl), e.p(self.cnt, slot), s)
-            self.cnt_host[slot:slot + 1].copy_(self.cnt[slot:slot + 1], non_blocking=True)
-            self.cnt_ev[slot].record()
-            rounds += 1
-            if rounds >= 2:
+            slot = polls & 1
+            if graph is not None:
+                graph.replay()
+                rounds += G
+            else:
+                self._round(precond, op)
+                rounds += 1
+            self._poll(slot)
+            polls += 1
+            if polls >= 2:
                 prev = slot ^ 1
                 self.cnt_ev[prev].synchronize()
                 if int(self.cnt_host[prev]) == 0:
                     break
-        self.cnt_ev[(rounds - 1) & 1].synchronize()
+        self.cnt_ev[(polls - 1) & 1].synchronize()
         self.rounds += rounds
         return rounds
 
@@ -132,9 +193,17 @@ class Engine:
                  profile="harm", stokes_rtol=1e-8, stokes_restart=80,
                  stokes_maxiter=800, theta_rtol=1e-10, theta_restart=40,
                  theta_maxiter=200, defect_tol=1e-8, max_K=8, K=3, pc_apply=None,
-                 ortho=None):
+                 ortho=None, precision="fp64", graph_rounds=0):
         self.dev = runtime.require_device() if device is None else device
+        # > 0: FGMRES rounds replayed from HIP graphs of this many rounds
+        self.graph_rounds = int(graph_rounds)
         self.L = runtime.load()
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be one of %s, got %r" % (PRECISIONS, precision))
+        if precision == "mixed" and int(prec) != 0:
+            raise ValueError("precision='mixed' runs on the FP64 engine (prec=0)")
+        self.precision = precision
+        self.rescue_prec = 1 if precision == "mixed" else 0
         pc_apply = PC_APPLY if pc_apply is None else pc_apply
         if pc_apply not in ("lu", "inv"):
             raise ValueError("pc_apply must be 'lu' or 'inv', got %r" % (pc_apply,))
@@ -168,12 +237,7 @@ class Engine:
         self.H = torch.tensor(self.Hh, dtype=dt, device=dev)
         self.diss_scale = torch.tensor(self.Dih / np.array([float(p.get("Ra", 1e4)) for p in P]),
                                        dtype=dt, device=dev)
-        law = np.zeros((B, 6))
-        for i, p in enumerate(P):
-            Ts = float(p.get("Ts", 0.0))
-            law[i] = (np.log(float(p.get("deta_T", 1000.0))), Ts,
-                      float(p.get("eta_min", 1e-3)), float(p.get("eta_max", 1e3)),
-                      float(p.get("eta_smooth", 0.0)), 1.0 if Ts > 0.0 else 0.0)
+        law = np.stack([_law_row(p) for p in P])
         self.lawh = law
         self.law = torch.tensor(law, dtype=torch.float64, device=dev)
         self.counters = torch.zeros(B, 8, dtype=torch.float64, device=dev)
@@ -233,11 +297,14 @@ class Engine:
     def stream(self):
         return torch.cuda.current_stream(self.dev).cuda_stream
 
+    # GEMM roles (they name the kernel symbols in traces)
+    Z_ROLE, X_ROLE, MODE_ROLE, DENSE_ROLE = 0, 1, 2, 3
+
     def gemm(self, M, N, K, A, ars, acs, Bm, brs, bcs, C, crs, ccs,
              alpha=1.0, beta=0.0, nz1=1, nz2=1, sa=(0, 0), sb=(0, 0), sc=(0, 0),
              aoff=0, boff=0, coff=0, rscale=None, rperiod=1, active=None,
-             mask_mode=0, rpr=1):
-        self.L.gemm(self.prec, M, N, K, alpha, beta,
+             mask_mode=0, rpr=1, role=0):
+        self.L.gemm_ex(role, self.prec, M, N, K, alpha, beta,
                     self.p(A, aoff), ars, acs, sa[0], sa[1],
                     self.p(Bm, boff), brs, bcs, sb[0], sb[1],
                     self.p(C, coff), crs, ccs, sc[0], sc[1],
@@ -258,7 +325,7 @@ class Engine:
         """per run b: dst[b][r][c] = sum_m S[r][m] src[b][m][c] (x-DFT)."""
         self.gemm(R, C, Kin, S, Kin, 1, src, src_ld, 1, dst, dst_ld, 1, beta=beta,
                   nz1=self.B, sb=(src_bs, 0), sc=(dst_bs, 0), boff=soff, coff=doff,
-                  active=active, mask_mode=2)
+                  active=active, mask_mode=2, role=self.X_ROLE)
 
     def green(self, src, dst, active=None, nb=None):
         """dst[b][k] = G_k src[b][k] (stokes_coupled.py:318-325)."""
@@ -266,7 +333,7 @@ class Engine:
         nb = self.B if nb is None else nb
         self.gemm(nb, N, N, src, n, 1, self.t["G"], 1, N, dst, n, 1,
                   nz1=1, nz2=self.Nk, sa=(0, N), sb=(0, N * N), sc=(0, N),
-                  active=active, mask_mode=1, rpr=1)
+                  active=active, mask_mode=1, rpr=1, role=self.MODE_ROLE)
 
     # ------------------------------------------------------------ state I/O
     def set_theta(self, th, idx=None):
@@ -286,6 +353,56 @@ class Engine:
             self.has_hist[:] = False
         else:
             self.has_hist[idx] = False
+
+    def admit(self, slot, params, theta):
+        """Load a new run into ``slot``: its physics, its initial real
+        coefficients ``theta`` (Nk, Nz), cleared history, memo, cached
+        factors and counters.  The other slots are untouched, so runs can
+        join a bucket while the rest keep stepping (continuous batching)."""
+        i = int(slot)
+        if not 0 <= i < self.B:
+            raise IndexError("slot %d outside 0..%d" % (i, self.B - 1))
+        p = dict(params)
+        self.params[i] = p
+        Ra = float(p.get("Ra", 1e4))
+        self.Hh[i] = float(p.get("internal_heating", 0.0))
+        self.Dih[i] = float(p.get("dissipation_number", 0.0))
+        self.lawh[i] = _law_row(p)
+        vals = torch.tensor([Ra, self.Hh[i], self.Dih[i] / Ra], dtype=self.dtype)
+        self.Ra[i], self.H[i], self.diss_scale[i] = vals.to(self.dev)
+        self.law[i] = torch.from_numpy(self.lawh[i]).to(self.dev)
+        self.counters[i].zero_()
+        th = torch.from_numpy(np.ascontiguousarray(np.real(theta), dtype=np.float64))
+        self.theta[i] = th.reshape(self.n).to(device=self.dev, dtype=self.dtype)
+        for t in (self.theta_prev, self.psi, self.psi_prev, self.vsol, self.omega):
+            t[i].zero_()
+        self.has_hist[i] = False
+        self.dt_prev[i] = 0.0
+        self._a0_fact[i] = np.nan
+        self.n_steps[i] = 0
+        self.n_stokes[i] = 0
+        if self.kind == KIND_CONST:
+            self._lam_fact[i] = np.nan
+        elif self.kind == KIND_DEPTH:
+            eg = self._depth_profile(i)
+            self.eta_grid_h[i] = eg
+            prof = self.ops.Vi @ eg
+            self.eta_prof_c[i] = torch.from_numpy(prof).to(self.dev, self.dtype)
+            self.eta_c_depth[i].zero_()
+            self.eta_c_depth[i, :self.Nz] = self.eta_prof_c[i]
+            mask = torch.zeros(self.B, dtype=torch.int32, device=self.dev)
+            mask[i] = 1
+            self._reduced_op_factor(self.eta_prof_c, self.vv_lu, self.vv_piv, mask)
+        else:
+            self.memo_valid[i] = False
+            for a in (self.memo_hits, self.bfbt_iters, self.last_bfbt_iters,
+                      self.last_bfbt_info, self.n_rescue, self.n_rescue_direct,
+                      self.n_rescue_weak, self.n_defect_failed, self.n_defect_double_fail,
+                      self.n_rescue_fp32, self.n_rescue_fp32_redo, self.rescue_iters):
+                a[i] = 0
+            self.last_bfbt_res[i] = 0.0
+            self.last_defect_res[i] = 0.0
+            self.last_route[i] = "bfbt"
 
     # ------------------------------------------------------- shared pieces
     def project(self, src, dst, active=None):
@@ -449,6 +566,15 @@ class Engine:
         L.lu_solve(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv), self.p(self.s1),
                    self.p(self.theta), 1, -1.0, None, None, Nk, 0, self.p(active), s)
 
+    def set_pc_apply(self, mode):
+        """Switch the per-mode preconditioner blocks between 'lu' and 'inv';
+        cached Helmholtz factors are rebuilt at the next step."""
+        if mode not in ("lu", "inv"):
+            raise ValueError("pc_apply must be 'lu' or 'inv', got %r" % (mode,))
+        if mode != self.pc_apply:
+            self.pc_apply = mode
+            self._a0_fact[:] = np.nan
+
     def _pc_factor(self, mats, piv, mask):
         """Factor the assembled per-mode blocks ``mats`` [B][Nk][N*N] in place
         (LU with pivots, or explicit inverse)."""
@@ -482,7 +608,8 @@ class Engine:
         L.cheb_mult(self.prec, B, N, self.p(prof_c), self.p(M0), self.p(active), s)
         mm = lambda A, sa, Bm, sb, C: self.gemm(N, N, N, A, N, 1, Bm, N, 1, C, N, 1, nz1=B,
                                                 sa=(sa, 0), sb=(sb, 0), sc=(NN, 0),
-                                                active=active, mask_mode=2)
+                                                active=active, mask_mode=2,
+                                                role=self.MODE_ROLE)
         mm(t["D2"], 0, M0, NN, T1)           # D2 M
         mm(t["D"], 0, M0, NN, T2)            # D M
         mm(T2, NN, t["D"], 0, T3)            # (D M) D
@@ -492,7 +619,8 @@ class Engine:
         L.axpby(self.prec, NN, B, 1.0, None, self.p(T3), 1.0, None, self.p(CM), self.p(CM),
                 self.p(active), s)           # Comm = (D2 M - 2 D M D) + M D2
         self.gemm(N, N, N, CM, N, 1, t["G"], N, 1, self.rb_CG, N, 1, nz1=B, nz2=Nk,
-                  sa=(NN, 0), sb=(0, NN), sc=(Nk * NN, NN), active=active, mask_mode=2)
+                  sa=(NN, 0), sb=(0, NN), sc=(Nk * NN, NN), active=active, mask_mode=2,
+                  role=self.MODE_ROLE)
         L.tau_assemble(self.prec, 1, B, Nk, N, None, self.p(t["kx"]), None, self.p(M0),
                        self.p(T1), self.p(self.rb_CG), self.p(t["V0"]), self.p(t["V1"]),
                        self.p(lu), self.p(active), s)
@@ -508,28 +636,33 @@ class Engine:
         self.rb_T3, self.rb_C = z(B, N * N), z(B, N * N)
         self.rb_CG = z(B, Nk, N * N)
 
+    def _depth_profile(self, i):
+        """Window-clipped eta(z) profile of run ``i`` on the CGL nodes
+        (convection_modes.py:86-110)."""
+        o = self.ops
+        lo, hi, w = self.lawh[i, 2], self.lawh[i, 3], self.lawh[i, 4]
+        eg = np.exp(-self.lawh[i, 0] * (1.0 + (0.0 - 1.0) * o.z))
+        elo, ehi = lo * (1.0 - 1e-12), hi * (1.0 + 1e-12)
+        if w > 0.0:
+            ylo, yhi = np.log(lo), np.log(hi)
+            y = np.log(eg)
+            sm = lambda q: (lambda c: c * c * c * (c * (6.0 * c - 15.0) + 10.0))(np.clip(q, 0.0, 1.0))
+            Y = ylo + (y - ylo) * sm((y - ylo) / w)
+            eg = np.exp(yhi - (yhi - Y) * sm((yhi - Y) / w))
+        elif np.any((eg < elo) | (eg > ehi)):
+            eg = np.clip(eg, lo, hi)
+        return eg
+
     def _init_depth(self):
         B, Nk, N, dev = self.B, self.Nk, self.Nz, self.dev
         z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=dev)
         self._alloc_reduced()
         o = self.ops
-        # eta(z) profile, window-clipped, per run (convection_modes.py:86-110)
         prof = np.zeros((B, N))
         self.eta_grid_h = np.zeros((B, N))
         for i in range(B):
-            lo, hi, w = self.lawh[i, 2], self.lawh[i, 3], self.lawh[i, 4]
-            eg = np.exp(-self.lawh[i, 0] * (1.0 + (0.0 - 1.0) * o.z))
-            elo, ehi = lo * (1.0 - 1e-12), hi * (1.0 + 1e-12)
-            if w > 0.0:
-                ylo, yhi = np.log(lo), np.log(hi)
-                y = np.log(eg)
-                sm = lambda q: (lambda c: c * c * c * (c * (6.0 * c - 15.0) + 10.0))(np.clip(q, 0.0, 1.0))
-                Y = ylo + (y - ylo) * sm((y - ylo) / w)
-                eg = np.exp(yhi - (yhi - Y) * sm((yhi - Y) / w))
-            elif np.any((eg < elo) | (eg > ehi)):
-                eg = np.clip(eg, lo, hi)
-            self.eta_grid_h[i] = eg
-            prof[i] = o.Vi @ eg
+            self.eta_grid_h[i] = self._depth_profile(i)
+            prof[i] = o.Vi @ self.eta_grid_h[i]
         self.eta_prof_c = torch.tensor(prof, dtype=self.dtype, device=dev)
         ec = np.zeros((B, Nk, N))
         ec[:, 0, :] = prof
@@ -582,6 +715,9 @@ class Engine:
         self.n_rescue_weak = np.zeros(B, dtype=np.int64)
         self.n_defect_failed = np.zeros(B, dtype=np.int64)
         self.n_defect_double_fail = np.zeros(B, dtype=np.int64)
+        self.n_rescue_fp32 = np.zeros(B, dtype=np.int64)
+        self.n_rescue_fp32_redo = np.zeros(B, dtype=np.int64)
+        self.rescue_iters = np.zeros(B, dtype=np.int64)
         self.last_defect_res = np.zeros(B)
         self.last_route = ["bfbt"] * B
         self.defect_tol = defect_tol
@@ -656,7 +792,8 @@ class Engine:
         A = torch.empty(n, n, dtype=self.dtype, device=self.dev) if out is None else out
         L.cheb_mult(self.prec, Nk, N, self.p(self.eta_c[r]), self.p(w["M"]), None, st)
         mm = lambda X, sx, Y, sy, C: self.gemm(N, N, N, X, N, 1, Y, N, 1, C, N, 1, nz1=Nk,
-                                               sa=(sx, 0), sb=(sy, 0), sc=(NN, 0))
+                                               sa=(sx, 0), sb=(sy, 0), sc=(NN, 0),
+                                               role=self.DENSE_ROLE)
         mm(t["D2"], 0, w["M"], NN, w["P"])        # D2 M_d
         mm(w["M"], NN, t["D2"], 0, w["Q"])        # M_d D2
         mm(t["D"], 0, w["M"], NN, w["DM"])        # D M_d
@@ -665,7 +802,7 @@ class Engine:
                        self.p(w["R"]), self.p(t["kx"]), -1.0, self.p(A), self.p(w["U"]), st)
         # A_ij += U_ij G_j
         self.gemm(N, N, N, w["U"], N, 1, t["G"], N, 1, A, n, 1, beta=1.0, nz1=Nk, nz2=Nk,
-                  sa=(Nk * NN, NN), sb=(0, NN), sc=(N * n, N))
+                  sa=(Nk * NN, NN), sb=(0, NN), sc=(N * n, N), role=self.DENSE_ROLE)
         L.schur_tau(self.prec, Nk, N, self.p(t["V0"]), self.p(t["V1"]), self.p(A), st)
         return A
 
@@ -688,7 +825,7 @@ class Engine:
         def op(zin, wout, mask):
             self.schur_matvec(zin, wout, mask)
 
-        sv.solve(precond, op, active)
+        sv.solve(precond, op, active, key="stokes")
         it, info, res = sv.results()
         act = np.ones(B, bool) if active is None else active.cpu().numpy().astype(bool)
         self.bfbt_iters[act] += it[act]
@@ -708,32 +845,52 @@ class Engine:
             self.vsol[m] = sv.x[m]
         self.green(self.vsol, self.psi, active)
 
-    def _rescue(self, theta, idx):
+    def _rescue(self, theta, idx, rprec=None):
         """Recompute rejected BFBT solves with the robust route
         (coupled_model.py:613-654) for every listed run at once: dense Schur LU
         at this eta + Picard defect correction against the matrix-free
         operator, accepted when the true relative residual meets the
         tolerance, else the direct dense solve (Nk*Nz <= 4096) or a counted
-        weak acceptance."""
+        weak acceptance.  ``rprec`` 1 factors in FP32 (mixed precision, see
+        ``_rescue_mixed``)."""
         sv, n, B = self.ssol, self.n, self.B
+        rprec = self.rescue_prec if rprec is None else int(rprec)
         tol = self.defect_tol
         bad = [int(r) for r in idx]
+        # dense matrices of at most RESCUE_BYTES at a time
+        cap = max(1, int(RESCUE_BYTES // (n * n * (4 if rprec else 8))))
+        if len(bad) > cap:
+            for c0 in range(0, len(bad), cap):
+                self._rescue(theta, bad[c0:c0 + cap], rprec)
+            return
         nb = len(bad)
         bt = torch.as_tensor(bad, device=self.dev)
         L, st = self.L, self.stream()
-        A = torch.empty(nb, n, n, dtype=self.dtype, device=self.dev)
-        for q, r in enumerate(bad):
-            self.assemble_schur(r, out=A[q])
+        if rprec == 0:
+            A = torch.empty(nb, n, n, dtype=self.dtype, device=self.dev)
+            for q, r in enumerate(bad):
+                self.assemble_schur(r, out=A[q])
+        else:
+            # assemble each matrix in FP64, keep its FP32 rounding
+            A = torch.empty(nb, n, n, dtype=torch.float32, device=self.dev)
+            A64 = torch.empty(n, n, dtype=self.dtype, device=self.dev)
+            for q, r in enumerate(bad):
+                self.assemble_schur(r, out=A64)
+                A[q].copy_(A64)
+            del A64
         piv = torch.empty(nb, n, dtype=torch.int32, device=self.dev)
         info = torch.empty(nb, dtype=torch.int32, device=self.dev)
-        L.dense_lu(self.prec, nb, n, self.p(A), n * n, self.p(piv), self.p(info), st)
+        L.dense_lu(rprec, nb, n, self.p(A), n * n, self.p(piv), self.p(info), st)
         mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
         mask[bt] = 1
+        if rprec != 0:
+            self._rescue_mixed(theta, bad, bt, A, piv, mask)
+            return
 
         def solve(R):
             R = R.contiguous()
             X = torch.empty_like(R)
-            L.dense_lu_solve(self.prec, nb, n, self.p(A), n * n, self.p(piv), self.p(R),
+            L.dense_lu_solve(0, nb, n, self.p(A), n * n, self.p(piv), self.p(R),
                              self.p(X), st)
             return X
 
@@ -780,6 +937,51 @@ class Engine:
                 self.last_defect_res[r] = res_direct[q]
                 self.last_route[r] = "bfbt->direct"
         sv.x[bt] = x
+
+    def _rescue_mixed(self, theta, bad, bt, A32, piv, mask):
+        """Mixed-precision rescue: FGMRES in FP64 on the matrix-free Schur
+        operator, right-preconditioned by the FP32 dense LU of the same
+        operator (its solves take and return FP64 vectors).  The outer FP64
+        residual refines the FP32 factor's answer (GMRES-based iterative
+        refinement), and the Krylov acceleration removes the few directions
+        in which the FP32 factor is poor, which a plain defect iteration
+        would leave at the FP32 level.  Runs whose true residual misses the
+        acceptance test are redone with the FP64 factor."""
+        sv, n, nb, L, st = self.ssol, self.n, len(bad), self.L, self.stream()
+        rs = getattr(self, "rsol", None)
+        if rs is None:
+            rs = self.rsol = Solver(self, n, RESCUE_RESTART, RESCUE_MAXITER, RESCUE_RTOL)
+        rs.b.copy_(sv.b)
+        R = torch.empty(nb, n, dtype=self.dtype, device=self.dev)
+        X = torch.empty_like(R)
+
+        def precond(vin, zout, m):
+            R.copy_(vin[bt])
+            L.dense_lu_solve(2, nb, n, self.p(A32), n * n, self.p(piv), self.p(R),
+                             self.p(X), st)
+            zout[bt] = X
+
+        def op(zin, wout, m):
+            self.schur_matvec(zin, wout, m)
+
+        rs.solve(precond, op, mask)
+        it, info, res = rs.results()
+        good, redo = [], []
+        for r in bad:
+            if np.isfinite(res[r]) and res[r] <= sv.rtol:
+                good.append(r)
+                self.n_rescue_fp32[r] += 1
+                self.rescue_iters[r] += int(it[r])
+                self.last_defect_res[r] = res[r]
+                self.last_route[r] = "bfbt->legacy"
+            else:
+                redo.append(r)
+        if good:
+            g = torch.as_tensor(good, device=self.dev)
+            sv.x[g] = rs.x[g]
+        if redo:
+            self.n_rescue_fp32_redo[redo] += 1
+            self._rescue(theta, redo, rprec=0)
 
     def stokes(self, theta, active=None):
         """(psi, v = -omega) of ``theta`` for the active runs into self.psi /
@@ -905,7 +1107,7 @@ class Engine:
         def op(zin, wout, mask):
             self.theta_matvec(zin, wout, mask)
 
-        sv.solve(precond, op, active)
+        sv.solve(precond, op, active, key="theta")
         self.last_theta = sv.results()
         self.project(sv.x, self.theta, active)           # output projected
         # history (theta^{n-1} = th, dt_prev, psi^{n-1} = psi_n)

@@ -14,6 +14,13 @@ import torch
 from . import runtime
 
 TAGS = {"fg_post": 1, "pc_apply": 2}
+_armed = [False]
+
+
+def armed():
+    """True while a clock is running: launches then record events from the C
+    side, which must not be captured into HIP graphs."""
+    return _armed[0]
 
 
 class KernelClock:
@@ -25,11 +32,13 @@ class KernelClock:
 
     def start(self):
         runtime.load().clock_start(TAGS[self.name], self.capacity)
+        _armed[0] = True
 
     def stop(self):
         torch.cuda.synchronize()
         ms, n, u = ctypes.c_double(0.0), ctypes.c_int(0), ctypes.c_longlong(0)
         runtime.load().clock_stop(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(u))
+        _armed[0] = False
         if n.value >= self.capacity:
             raise RuntimeError("launch clock full: raise its capacity above %d" % self.capacity)
         self.launches += n.value

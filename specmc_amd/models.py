@@ -51,7 +51,8 @@ class _DeviceModel:
                  internal_heating=0.0, dissipation_number=0.0,
                  eta_min=1e-3, eta_max=1e3, eta_smooth=0.0, arrhenius_Ts=0.0,
                  precond="bfbt", symmetrize=True, stokes_maxiter=None,
-                 defect_tol=1e-8, max_K=8, K=3, pc_apply=None, ortho=None, **extra):
+                 defect_tol=1e-8, max_K=8, K=3, pc_apply=None, ortho=None,
+                 precision="fp64", graph_rounds=0, **extra):
         self.Ra, self.Nx, self.Nz, self.Lx = float(Ra), int(Nx), int(Nz), float(Lx)
         self.internal_heating = float(internal_heating)
         self.dissipation_number = float(dissipation_number)
@@ -63,7 +64,8 @@ class _DeviceModel:
                       eta_smooth=float(eta_smooth),
                       internal_heating=self.internal_heating,
                       dissipation_number=self.dissipation_number)
-        kw = dict(defect_tol=defect_tol, max_K=int(max_K), K=int(K))
+        kw = dict(defect_tol=defect_tol, max_K=int(max_K), K=int(K), precision=precision,
+                  graph_rounds=int(graph_rounds))
         if stokes_maxiter is not None:
             kw["stokes_maxiter"] = int(stokes_maxiter)
         if pc_apply is not None:
@@ -200,7 +202,7 @@ class _DeviceModel:
         d = dict(device="hip", kind=self.kind, Nx=self.Nx, Nz=self.Nz, Lx=self.Lx,
                  Ra=self.Ra, deta_T=self.deta_T, internal_heating=self.internal_heating,
                  dissipation_number=self.dissipation_number, n_steps=self.n_steps,
-                 n_stokes=self.n_stokes, precision="fp64" if self.prec == 0 else "fp32")
+                 n_stokes=self.n_stokes, precision=e.precision if self.prec == 0 else "fp32")
         if self.kind != KIND_CONST:
             th = self.theta
             d.update(theta_scheme=NUM.THETA_SCHEME, theta_rtol=NUM.THETA_RTOL,
@@ -217,6 +219,9 @@ class _DeviceModel:
                      n_bfbt_rescue_direct=int(e.n_rescue_direct[0]),
                      n_bfbt_rescue_weak=int(e.n_rescue_weak[0]),
                      n_defect_failed=int(e.n_defect_failed[0]),
+                     n_rescue_fp32=int(e.n_rescue_fp32[0]),
+                     n_rescue_fp32_redo=int(e.n_rescue_fp32_redo[0]),
+                     n_rescue_fp32_iters=int(e.rescue_iters[0]),
                      last_defect_res=float(e.last_defect_res[0]),
                      last_stokes_route=e.last_route[0])
         return d

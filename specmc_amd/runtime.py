@@ -20,6 +20,10 @@ SIGNATURES = {
     "smx_last_error": [],
     "smx_gemm": [I, I, I, I, D, D, P, L, L, L, L, P, L, L, L, L, P, L, L, L, L,
                  I, I, P, I, P, I, I, P],
+    "smx_gemm_ex": [I, I, I, I, I, D, D, P, L, L, L, L, P, L, L, L, L, P, L, L, L, L,
+                    I, I, P, I, P, I, I, P],
+    "smx_gemm_tune": [I, I, I, I, I, I, I, I],
+    "smx_gemm_tune_clear": [],
     "smx_lu_factor": [I, I, I, P, P, I, P, P],
     "smx_lu_solve": [I, I, I, P, P, P, P, I, D, P, P, I, I, P, P],
     "smx_invert": [I, I, I, P, I, P, P],
@@ -35,7 +39,7 @@ SIGNATURES = {
     "smx_schur_tau": [I, I, I, P, P, P, P],
     "smx_eta_law": [I, I, I, I, P, P, P, P, P, P, P],
     "smx_eta_guard_profile": [I, I, I, I, P, P, P, I, P, P, P],
-    "smx_xprod": [I, I, I, I, I, I, I, P, P, P, L, P, P, P, P, P, P],
+    "smx_xprod": [I, I, I, I, I, I, I, P, P, P, L, P, P, P, P, P, P, P],
     "smx_advect_prod": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P],
     "smx_axpby": [I, L, I, D, P, P, D, P, P, P, P, P],
     "smx_rowscale": [I, I, I, I, P, P, D, P, I, P, P, P],
@@ -62,9 +66,14 @@ class _Checked:
     def __init__(self, lib, name):
         self.fn = getattr(lib, name)
         self.name = name
+        self.nargs = len(SIGNATURES[name])
         self.err = lib.smx_last_error
 
     def __call__(self, *args):
+        # ctypes passes surplus arguments unconverted (as C int), which
+        # would truncate pointers: insist on the declared count
+        if len(args) != self.nargs:
+            raise TypeError("%s takes %d arguments, got %d" % (self.name, self.nargs, len(args)))
         rc = self.fn(*args)
         if rc != 0:
             raise RuntimeError("%s failed (%d): %s"

@@ -1,0 +1,272 @@
+"""Continuous batching of independent runs on one GPU.
+
+A ``Server`` takes requests (a ``ConvectionConfig``, an initial state, a
+``tstop`` and an arrival time), groups them by shape into buckets and steps
+every bucket's active runs together.  Runs join a bucket in a free slot while
+its other runs keep stepping, and leave it when their own ``t >= tstop``.  A
+bucket is one ``Engine`` whose batch dimension is its slot count.  Admission
+is by HBM footprint: a bucket is created only when its resident buffers fit
+the memory budget, and idle buckets are released when a waiting request of
+another shape needs the room.
+
+Each run follows the march of api.py:169-190 with the production CFL policy
+(cfl_policy.py:64-74), exactly as it would alone: every kernel is masked per
+run and no arithmetic couples two runs, so a result does not depend on what
+else was in the bucket (``tests/test_gpu_serve.py`` checks this against solo
+marches).
+
+Time-to-solution of a request is its finish time minus its arrival time
+(queueing included); ``Server.stats()`` reports its p50 / p99 and the
+aggregate run-steps/s.
+"""
+import time
+from collections import OrderedDict, deque
+
+import numpy as np
+import torch
+
+from . import _numerics as NUM
+from .api import model_spec
+from .batch import run_params, _KIND
+from .core.cfl_policy import clamp_safety
+from .engine import Engine, KIND_CONST, KIND_COUPLED
+
+
+def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=40):
+    """Resident HBM bytes of one slot of an FP64 ``Engine`` of this shape
+    (Krylov bases, per-mode blocks, product-grid fields, state), not counting
+    the transient dense rescue matrices (bounded by ``engine.RESCUE_BYTES``)."""
+    Nk, N = Nx // 2 + 1, Nz
+    n, nrp, nru = Nk * N, Nx + 1, Nx // 2 + 1
+    Q = (3 * N - 2 + 1) // 2 + 1
+    w = 8
+    b = (12 * n + 2 * nrp * N * 6 + 3 * nru * N + 8 * n) * w      # state, grids, scratch
+    if kind == KIND_CONST:
+        return b + (Nk * N * N + n + nrp * N) * w
+    b += (2 * theta_restart + 6) * n * w + Nk * N * N * w          # theta FGMRES, Helmholtz
+    b += (6 * N * N + Nk * N * N) * w                               # reduced-operator scratch
+    if kind == KIND_COUPLED:
+        b += (2 * stokes_restart + 6) * n * w                       # Stokes FGMRES
+        b += Nk * N * N * w                                         # BFBT blocks
+        b += (2 * Nk * 4 * Q + Nk * 7 * N + nrp * Q + 2 * nrp * N + 4 * n) * w
+    else:
+        b += Nk * N * N * w                                         # eta(z) operator LU
+    return int(b)
+
+
+class Request:
+    """One run to serve."""
+
+    __slots__ = ("rid", "cfg", "theta0", "tstop", "arrival", "admitted", "finished",
+                 "steps", "t", "theta", "diag", "bucket", "slot")
+
+    def __init__(self, rid, cfg, theta0, tstop, arrival=0.0):
+        self.rid, self.cfg = rid, cfg
+        self.theta0 = np.asarray(theta0)
+        self.tstop, self.arrival = float(tstop), float(arrival)
+        self.admitted = self.finished = None
+        self.steps, self.t = 0, 0.0
+        self.theta = self.diag = None
+        self.bucket = self.slot = None
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+class Bucket:
+    """Slots of one (kind, Nx, Nz, Lx) sharing an Engine."""
+
+    def __init__(self, key, cfg, slots, precision="fp64", **engine_kw):
+        kind, Nx, Nz, Lx = key
+        self.key = key
+        self.S = int(slots)
+        params = [run_params(cfg)] * self.S
+        self.eng = Engine(kind, Nx, Nz, Lx, params, precision=precision, **engine_kw)
+        self.req = [None] * self.S
+        self.t = np.zeros(self.S)
+        self.tstop = np.zeros(self.S)
+        self.live = np.zeros(self.S, dtype=bool)
+        self.steps = 0
+        self.last_used = 0.0
+
+    def free_slots(self):
+        return [i for i in range(self.S) if self.req[i] is None]
+
+    def admit(self, slot, r, now):
+        self.eng.admit(slot, run_params(r.cfg), r.theta0)
+        self.req[slot] = r
+        self.t[slot] = 0.0
+        self.tstop[slot] = r.tstop
+        self.live[slot] = r.tstop > 0.0
+        r.admitted, r.bucket, r.slot = now, self.key, slot
+
+    def step(self, safety, dtmax):
+        """One step of every live slot; -> run-steps taken."""
+        live = self.live
+        if not np.any(live):
+            return 0
+        act = None if np.all(live) else torch.as_tensor(live.astype(np.int32),
+                                                        device=self.eng.dev)
+        dt = self.eng.cfl(safety, dtmax, NUM.DTMIN, act)
+        dt = np.minimum(dtmax, dt)
+        self.eng.step(dt, act)
+        self.t[live] += dt[live]
+        for i in np.where(live)[0]:
+            self.req[i].steps += 1
+        self.steps += 1
+        self.live = live & (self.t < self.tstop)
+        return int(live.sum())
+
+    def retire(self, now, diagnostics=True):
+        """Release the slots whose run reached its tstop; -> their requests."""
+        done = [i for i in range(self.S) if self.req[i] is not None and not self.live[i]]
+        if not done:
+            return []
+        if diagnostics:
+            mask = np.zeros(self.S, dtype=np.int32)
+            mask[done] = 1
+            d = self.eng.diagnostics(torch.as_tensor(mask, device=self.eng.dev))
+        th = self.eng.get(self.eng.theta)
+        out = []
+        names = ("Nu_top", "Nu_bot", "Nu_energy", "urms", "umax", "amp")
+        for i in done:
+            r = self.req[i]
+            r.t = float(self.t[i])
+            r.theta = th[i].copy()
+            if diagnostics:
+                r.diag = dict(zip(names, (float(v) for v in d[i])))
+            r.finished = now
+            self.req[i] = None
+            out.append(r)
+        return out
+
+    def busy(self):
+        return any(r is not None for r in self.req)
+
+
+class Server:
+    """Admission, stepping and retirement of requests on one GPU.
+
+    ``slots``     slots per bucket (at most; fewer when memory is short);
+    ``mem_budget`` HBM bytes the buckets may hold (default: 80% of the free
+                  device memory at construction, less the rescue headroom);
+    ``precision`` "fp64" or "mixed" (see ``engine.PRECISIONS``)."""
+
+    def __init__(self, slots=64, mem_budget=None, precision="fp64", safety=None, dtmax=None,
+                 diagnostics=True, **engine_kw):
+        from .engine import RESCUE_BYTES
+        self.slots = int(slots)
+        if mem_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            mem_budget = max(0, int(0.8 * free) - RESCUE_BYTES)
+        self.mem_budget = int(mem_budget)
+        self.precision = precision
+        self.engine_kw = dict(engine_kw)
+        self.safety = clamp_safety(NUM.CFL_SAFETY if safety is None else safety)
+        self.dtmax = NUM.DTMAX if dtmax is None else float(dtmax)
+        self.diagnostics = bool(diagnostics)
+        self.buckets = OrderedDict()
+        self.bucket_bytes = {}
+        self.pending = deque()
+        self.done = []
+        self.run_steps = 0
+        self.t0 = None
+        self.t_end = None
+
+    # ---------------------------------------------------------------- input
+    def submit(self, requests):
+        for r in sorted(requests, key=lambda q: q.arrival):
+            self.pending.append(r)
+
+    @staticmethod
+    def key_of(cfg):
+        spec = model_spec(cfg)
+        return (_KIND[spec["model"]], int(cfg.Nx), int(cfg.Nz), float(cfg.Lx))
+
+    def used_bytes(self):
+        return sum(self.bucket_bytes.values())
+
+    # ------------------------------------------------------------ admission
+    def _bucket_for(self, r, now):
+        key = self.key_of(r.cfg)
+        b = self.buckets.get(key)
+        if b is not None:
+            return b if b.free_slots() else None
+        per = run_bytes(key[0], key[1], key[2], self.precision)
+        room = self.mem_budget - self.used_bytes()
+        if room < per:
+            # release idle buckets of other shapes, oldest first
+            for k in [k for k, bb in self.buckets.items() if not bb.busy()]:
+                room += self.bucket_bytes.pop(k)
+                del self.buckets[k]
+                torch.cuda.empty_cache()
+                if room >= self.slots * per:
+                    break
+        S = min(self.slots, room // per)
+        if S < 1:
+            return None
+        b = Bucket(key, r.cfg, S, precision=self.precision, **self.engine_kw)
+        self.buckets[key] = b
+        self.bucket_bytes[key] = S * per
+        return b
+
+    def _admit(self, now):
+        """FIFO over the arrived requests; a request that cannot be placed
+        does not block later arrivals of other shapes."""
+        waiting = deque()
+        while self.pending and self.pending[0].arrival <= now:
+            r = self.pending.popleft()
+            b = self._bucket_for(r, now)
+            if b is None:
+                waiting.append(r)
+                continue
+            b.admit(b.free_slots()[0], r, now)
+        while waiting:
+            self.pending.appendleft(waiting.pop())
+
+    # ----------------------------------------------------------------- loop
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    def run(self, max_wall=None, on_retire=None):
+        """Serve until every submitted request is finished (or ``max_wall``
+        seconds have passed)."""
+        if self.t0 is None:
+            self.t0 = time.perf_counter()
+        while self.pending or any(b.busy() for b in self.buckets.values()):
+            now = self.now()
+            if max_wall is not None and now > max_wall:
+                break
+            self._admit(now)
+            stepped = 0
+            for b in list(self.buckets.values()):
+                stepped += b.step(self.safety, self.dtmax)
+                fin = b.retire(self.now(), self.diagnostics)
+                if fin:
+                    b.last_used = self.now()
+                    self.done.extend(fin)
+                    if on_retire is not None:
+                        for r in fin:
+                            on_retire(r)
+            self.run_steps += stepped
+            if stepped == 0 and self.pending:
+                arrived = [r for r in self.pending if r.arrival <= self.now()]
+                if not arrived:       # idle until the next arrival
+                    time.sleep(max(0.0, min(0.05, self.pending[0].arrival - self.now())))
+                elif not any(b.busy() for b in self.buckets.values()):
+                    raise RuntimeError("request %r does not fit the memory budget (%d bytes)"
+                                       % (arrived[0].rid, self.mem_budget))
+        torch.cuda.synchronize()
+        self.t_end = self.now()
+        return self.done
+
+    def stats(self):
+        lat = np.array([r.latency for r in self.done if r.latency is not None])
+        wall = self.t_end if self.t_end is not None else self.now()
+        return dict(requests=len(self.done), run_steps=int(self.run_steps), wall_s=wall,
+                    steps_per_s=self.run_steps / wall if wall > 0 else None,
+                    p50_s=float(np.percentile(lat, 50)) if lat.size else None,
+                    p99_s=float(np.percentile(lat, 99)) if lat.size else None,
+                    buckets=[dict(key=list(k), slots=b.S, steps=b.steps)
+                             for k, b in self.buckets.items()])

@@ -1,0 +1,67 @@
+"""Host-side logic of serving and of the benchmark workloads (no device)."""
+import numpy as np
+
+from specmc_amd import workloads as W
+from specmc_amd.engine import KIND_CONST, KIND_COUPLED, KIND_DEPTH
+from specmc_amd.serve import Request, Server, run_bytes
+
+
+def test_run_bytes_grows_with_shape_and_solver():
+    for kind in (KIND_CONST, KIND_DEPTH, KIND_COUPLED):
+        sizes = [run_bytes(kind, Nx, Nz) for Nx, Nz in ((16, 24), (64, 96), (256, 192))]
+        assert sizes == sorted(sizes) and sizes[0] > 0
+    for Nx, Nz in ((32, 24), (128, 96)):
+        assert run_bytes(KIND_CONST, Nx, Nz) < run_bytes(KIND_DEPTH, Nx, Nz) \
+            < run_bytes(KIND_COUPLED, Nx, Nz)
+    # the Stokes Krylov bases dominate: (2 * 80 + 6) vectors of n doubles
+    n = 33 * 96
+    assert run_bytes(KIND_COUPLED, 64, 96) > 166 * n * 8
+
+
+def test_bucket_keys():
+    reqs = W.ensemble_requests(n=40)
+    keys = {Server.key_of(r.cfg) for r in reqs}
+    for k, Nx, Nz, Lx in keys:
+        assert k in (KIND_DEPTH, KIND_COUPLED)
+        assert (Nx, Nz) in ((32, 24), (64, 48), (64, 96), (128, 96)) and Lx in (2.0, 4.0)
+    # eta_T and arrhenius share the coupled engine kind
+    kinds = {(r.cfg.viscosity_model, Server.key_of(r.cfg)[0]) for r in reqs}
+    assert ("arrhenius", KIND_COUPLED) in kinds and ("eta_T", KIND_COUPLED) in kinds
+
+
+def test_workloads_are_seeded_and_well_formed():
+    a, b = W.ensemble_requests(n=16), W.ensemble_requests(n=16)
+    for p, q in zip(a, b):
+        assert p.cfg == q.cfg and np.array_equal(p.theta0, q.theta0)
+        assert p.theta0.shape == (p.cfg.Nx // 2 + 1, p.cfg.Nz)
+        assert np.isrealobj(p.theta0) and p.arrival == 0.0
+        if p.cfg.viscosity_model == "arrhenius":
+            assert p.cfg.arrhenius_Ts == 1.0
+            assert abs(p.cfg.arrhenius_E - 2.0 * np.log(p.cfg.deta_T)) < 1e-12
+    s = W.stream_requests(n=32, rate=2.0)
+    arr = np.array([r.arrival for r in s])
+    assert np.all(np.diff(arr) > 0) and arr[0] > 0
+    # mean inter-arrival near 1/rate
+    assert 0.25 < np.mean(np.diff(np.r_[0.0, arr])) < 1.0
+    for r in s:
+        assert 1e-4 <= r.tstop <= 1e-2
+        assert r.cfg.eta_min == 1e-5 and r.cfg.eta_max == 1.0
+        assert 0.0 <= r.cfg.dissipation_number <= 0.5
+    c2 = W.c2_requests()[0]
+    assert c2.cfg.Lx == 4.0 and c2.cfg.internal_heating == 1.5 and c2.tstop == 0.03
+
+
+def test_theta0_vanishes_at_the_walls():
+    """The seeded perturbations are sin(n pi z) in z and the heated profile is
+    z(1-z)/2: theta0 is zero on both walls."""
+    r = W.ensemble_requests(n=3)[0]
+    g = W._Grid(r.cfg.Nx, r.cfg.Nz, r.cfg.Lx)
+    th = np.fft.irfft((g.V @ r.theta0.T).T * r.cfg.Nx, n=r.cfg.Nx, axis=0)
+    assert np.max(np.abs(th[:, [0, -1]])) < 1e-12
+
+
+def test_request_latency():
+    r = Request("x", W.c1_config(), W.c1_theta0().real, 1e-3, arrival=2.0)
+    assert r.latency is None
+    r.finished = 3.5
+    assert r.latency == 1.5
