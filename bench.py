@@ -187,6 +187,61 @@ def served_leg(z, log, w, B, value, horizon, load, nreq, precision, graph_rounds
     return st
 
 
+def run_c2(precisions=("fp64", "mixed")):
+    """The heated 2:1 case marched to t = 0.03 with the single-run model,
+    diagnostics at the first steps with t >= 0.01, 0.02, 0.03."""
+    from specmc_amd import api
+    from specmc_amd.workloads import c2_requests, C2_LOG_TIMES
+    r = c2_requests()[0]
+    out = {}
+    for prec in precisions:
+        m = api.build_model(r.cfg, precision=prec)
+        logs, marks = [], list(C2_LOG_TIMES)
+        t0 = time.perf_counter()
+
+        def cb(n, t, s):
+            if n % 25 == 0:
+                print("c2 %s: step %d t=%.5f wall %.1fs" % (prec, n, t, time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+            if marks and t >= marks[0] - 1e-15:
+                d = m.diagnostics(s)
+                logs.append(dict(t_mark=marks.pop(0), step=n, t=t,
+                                 Nu_top=d["Nu_top"], Nu_energy=d["Nu_energy"], urms=d["urms"],
+                                 wall_s=time.perf_counter() - t0))
+                print("c2 %s: %s" % (prec, json.dumps(logs[-1])), file=sys.stderr, flush=True)
+
+        s, info = api.march_adaptive(m, r.theta0, r.tstop, on_step=cb)
+        wall = time.perf_counter() - t0
+        dsc = m.describe()
+        out[prec] = dict(steps=info["steps"], wall_s=wall, steps_per_s=info["steps"] / wall,
+                         log=logs, rescues=dsc.get("n_bfbt_rescue"),
+                         rescues_fp32=dsc.get("n_rescue_fp32"), theta=np.real(s))
+    res = dict(metric="heated 2:1 run (64x96, Ra=1.25e4, H=1.5, eta_T 1e3) to t=0.03",
+               unit="steps/s", config=dict(workload="c2", Nx=64, Nz=96, Lx=4.0))
+    for prec in precisions:
+        o = dict(out[prec])
+        o.pop("theta")
+        res[prec] = o
+    if "fp64" in out and "mixed" in out:
+        a, b = out["fp64"], out["mixed"]
+        res["mixed_vs_fp64"] = dict(
+            theta_l2=float(np.linalg.norm(b["theta"] - a["theta"]) / np.linalg.norm(a["theta"])),
+            **{k: max(abs(lb[k] - la[k]) / abs(la[k]) for la, lb in zip(a["log"], b["log"]))
+               for k in ("Nu_top", "Nu_energy", "urms")})
+    return res
+
+
+def run_served(reqs, slots, precision, graph_rounds, max_wall, label):
+    from specmc_amd.serve import Server
+    srv = Server(slots=slots, precision=precision, graph_rounds=graph_rounds)
+    srv.submit(reqs)
+    srv.run(max_wall=max_wall)
+    st = srv.stats()
+    st.update(workload=label, precision=precision, submitted=len(reqs),
+              unfinished=len(reqs) - st["requests"])
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +267,17 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device once first (a known-size "
                          "reference for hardware byte counters)")
+    ap.add_argument("--workload", default="c1", choices=("c1", "c2", "c3", "c4"),
+                    help="c1: the 64x96 headline (default); c2: heated 2:1 run to t=0.03; "
+                         "c3: 256-run uneven-grid ensemble; c4: Poisson stream")
+    ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"),
+                    help="precision of the c3 / c4 server")
+    ap.add_argument("--slots", type=int, default=64, help="slots per bucket (c3, c4)")
+    ap.add_argument("--requests", type=int, default=0,
+                    help="c3: runs (default 256); c4: requests (default 48)")
+    ap.add_argument("--rate", type=float, default=0.5, help="c4 arrivals per second")
+    ap.add_argument("--tstop", type=float, default=1e-3, help="c3 run length")
+    ap.add_argument("--max-wall", type=float, default=600.0, help="c3 / c4 wall limit (s)")
     ap.add_argument("--make-states", action="store_true")
     ap.add_argument("--states-out", default=STATES)
     a = ap.parse_args()
@@ -225,6 +291,20 @@ def main():
     runtime.require_device()
     if a.make_states:
         make_states(a.states_out)
+        return
+    if a.workload != "c1":
+        from specmc_amd import workloads as W
+        if a.workload == "c2":
+            out = run_c2()
+        elif a.workload == "c3":
+            out = run_served(W.ensemble_requests(n=a.requests or 256, tstop=a.tstop),
+                             a.slots, a.precision, a.graph_rounds, a.max_wall,
+                             "c3: 256-run uneven-grid ensemble, tstop=%g" % a.tstop)
+        else:
+            out = run_served(W.stream_requests(n=a.requests or 48, rate=a.rate),
+                             a.slots, a.precision, a.graph_rounds, a.max_wall,
+                             "c4: Poisson stream at %g/s" % a.rate)
+        print(json.dumps(out), flush=True)
         return
     if not os.path.exists(STATES):
         raise SystemExit("missing %s: run `python bench.py --make-states` first" % STATES)

@@ -135,8 +135,9 @@ int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const void *rhs,
                   const void *bscale, int per_run, int shared,
                   const int *active, void *stream);
 
-/* Batched dense LU with partial pivoting, FP64 (prec must be 0), row-major
- * n x n matrices (matrix q at A + q*mstride), n <= 8192.  piv int[nmat][n]:
+/* Batched dense LU with partial pivoting, FP64 (prec 0, n <= 8192) or FP32
+ * (prec 1, n <= 32768), row-major n x n matrices (matrix q at
+ * A + q*mstride).  piv int[nmat][n]:
  * at step i rows i and piv[i] were exchanged (0-based, LAPACK order);
  * info int[nmat]: 0, or 1 + the first column with a zero pivot.  Replaces
  * sla.lu_factor of the assembled Schur matrix in LaggedStokes.refresh
@@ -146,7 +147,9 @@ int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const void *rhs,
 int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, int *piv,
                  int *info, void *stream);
 /* x_q = A_q^{-1} b_q from smx_dense_lu factors; b and x are [nmat][n]
- * (sla.lu_solve of the defect correction, coupled_model.py:544-554). */
+ * (sla.lu_solve of the defect correction, coupled_model.py:544-554).
+ * prec 0: FP64 factor and vectors; 1: FP32 factor and vectors; 2: FP32
+ * factor, FP64 b and x (the preconditioner of the mixed-precision rescue). */
 int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
                        const int *piv, const void *b, void *x, void *stream);
 

@@ -24,7 +24,8 @@
 #include "specmc_hip.h"
 
 #define LUD_NB 64
-#define LUD_MAXN 8192
+#define LUD_MAXN 8192         // FP64 (sub-panel rows held in registers)
+#define LUD_MAXN_F32 32768    // FP32 (narrower sub-panels; solve vector in LDS)
 
 // ---------------------------------------------------------------------------
 // sub-panel factorisation: rows s0 + tid + 1024 k (k < R), columns s0..s0+PB-1
@@ -311,8 +312,9 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
                             int *info, void *stream) {
   if (nmat <= 0) return 0;
   if (prec != 0 && prec != 1) return fail_msg("smx_dense_lu: prec must be 0 (FP64) or 1 (FP32)");
-  if (n <= 0 || n > LUD_MAXN || mstride < (long)n * n)
-    return fail_msg("smx_dense_lu: bad sizes (1 <= n <= 8192, mstride >= n*n)");
+  if (n <= 0 || n > (prec ? LUD_MAXN_F32 : LUD_MAXN) || mstride < (long)n * n)
+    return fail_msg("smx_dense_lu: bad sizes (1 <= n <= 8192 FP64 / 32768 FP32, "
+                    "mstride >= n*n)");
   hipStream_t s = STREAM(stream);
   hipLaunchKernelGGL(k_lud_zero, dim3(cdiv(nmat, 256)), dim3(256), 0, s, info, nmat);
   if (prec == 1) {
@@ -320,7 +322,9 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
     if (n <= 1024) return lud_run<float, 1, 8>(nmat, n, a, mstride, piv, info, s);
     if (n <= 2048) return lud_run<float, 2, 8>(nmat, n, a, mstride, piv, info, s);
     if (n <= 4096) return lud_run<float, 4, 8>(nmat, n, a, mstride, piv, info, s);
-    return lud_run<float, 8, 4>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 8192) return lud_run<float, 8, 4>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 16384) return lud_run<float, 16, 4>(nmat, n, a, mstride, piv, info, s);
+    return lud_run<float, 32, 2>(nmat, n, a, mstride, piv, info, s);
   }
   double *a = (double *)A;
   if (n <= 1024) return lud_run<double, 1, 8>(nmat, n, a, mstride, piv, info, s);
@@ -436,8 +440,9 @@ extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, lon
   if (prec < 0 || prec > 2)
     return fail_msg("smx_dense_lu_solve: prec must be 0 (FP64), 1 (FP32) or 2 (FP32 factor, "
                     "FP64 vectors)");
-  if (n <= 0 || n > LUD_MAXN || mstride < (long)n * n)
-    return fail_msg("smx_dense_lu_solve: bad sizes (1 <= n <= 8192, mstride >= n*n)");
+  if (n <= 0 || n > (prec ? LUD_MAXN_F32 : LUD_MAXN) || mstride < (long)n * n)
+    return fail_msg("smx_dense_lu_solve: bad sizes (1 <= n <= 8192 FP64 / 32768 FP32, "
+                    "mstride >= n*n)");
   const size_t es = prec == 0 ? 8 : 4;
   const size_t sh = (size_t)n * es + 64 * 65 * es;
   hipStream_t st = STREAM(stream);

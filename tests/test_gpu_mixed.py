@@ -116,7 +116,16 @@ def test_mixed_forced_rescue_matches_oracle(eng):
     assert m.n_bfbt_rescue == 1 and ref.n_bfbt_rescue == 1
     assert d["n_rescue_fp32"] == 1 and d["n_rescue_fp32_redo"] == 0
     assert d["last_defect_res"] <= 1e-8
-    assert rel(psi, psi_r) < 1e-8
+    # modes k >= 1 to the FP64 bar; the k = 0 row (the x-mean stream function,
+    # zero by the mirror symmetry) lies along a near-null direction of the
+    # Schur operator, which the residual test cannot see: the FP32 factor
+    # leaves it at the FP32 level there (it enters theta, Nu and Vrms only at
+    # second order)
+    e1 = rel(psi[1:], psi_r[1:])
+    e0 = float(np.abs(psi[0]).max() / np.abs(psi).max())
+    print("  modes k>=1 rel %.2e, k=0 row %.2e of max |psi|" % (e1, e0), flush=True)
+    assert e1 < 1e-10
+    assert e0 < 1e-6
 
 
 @pytest.mark.parametrize("shape", [(16, 24), (32, 48)])
@@ -147,3 +156,34 @@ def test_mixed_trajectory_through_rescues(eng, shape):
     assert dsc["n_rescue_fp32"] >= 1
     assert eth < 1e-8
     assert enu < 1e-6 and evr < 1e-6
+
+
+@pytest.mark.parametrize("n", [12480, 24768])
+def test_dense_lu_fp32_large(eng, n):
+    """FP32 factor beyond the FP64 limit (the 128x192 and 256x192 Schur
+    sizes): FP64 refinement with it reaches a 1e-12 residual."""
+    lib = eng.L
+    g = torch.Generator(device="cpu").manual_seed(n)
+    print("fp32 dense lu n=%d" % n, flush=True)
+    M = torch.randn(n, n, generator=g, dtype=torch.float64).to(eng.dev)
+    M.diagonal().add_(4.0 * np.sqrt(n))
+    A = M.to(torch.float32)
+    piv = torch.zeros(1, n, dtype=torch.int32, device=eng.dev)
+    info = torch.ones(1, dtype=torch.int32, device=eng.dev)
+    lib.dense_lu(1, 1, n, A.data_ptr(), n * n, piv.data_ptr(), info.data_ptr(), eng.stream())
+    b = torch.randn(1, n, generator=g, dtype=torch.float64).to(eng.dev)
+    x = torch.zeros_like(b)
+    r = b.clone()
+    dx = torch.zeros_like(b)
+    hist = []
+    for _ in range(4):
+        lib.dense_lu_solve(2, 1, n, A.data_ptr(), n * n, piv.data_ptr(), r.data_ptr(),
+                           dx.data_ptr(), eng.stream())
+        x += dx
+        r = b - (M @ x[0])[None, :]
+        hist.append(float(torch.linalg.norm(r) / torch.linalg.norm(b)))
+    torch.cuda.synchronize()
+    print("  residuals %s" % ["%.1e" % h for h in hist], flush=True)
+    assert int(info.cpu()[0]) == 0
+    assert hist[0] < 1e-4
+    assert hist[-1] < 1e-12
