@@ -235,7 +235,9 @@ def run_served(reqs, slots, precision, graph_rounds, max_wall, label):
     from specmc_amd.serve import Server
     srv = Server(slots=slots, precision=precision, graph_rounds=graph_rounds)
     srv.submit(reqs)
-    srv.run(max_wall=max_wall)
+    srv.run(max_wall=max_wall,
+            progress=lambda p: print("%s: %s" % (label, json.dumps(p)), file=sys.stderr,
+                                     flush=True))
     st = srv.stats()
     st.update(workload=label, precision=precision, submitted=len(reqs),
               unfinished=len(reqs) - st["requests"])

@@ -86,6 +86,16 @@ int smx_gemm_ex(int role, int prec, int M, int N, int K, double alpha, double be
  * give the same products to roundoff. */
 int smx_set_gemm_mfma(int mode);
 
+/* Kernel variants, process-wide (no counterpart in the reference; both forms
+ * give the same results to roundoff): smx_set_xprod_mfma(1) (default) runs
+ * the FP64 smx_xprod on the matrix cores, 0 on the vector-FMA kernel;
+ * smx_set_gj_rows(1) inverts with the row-owner Gauss-Jordan kernel, 0
+ * (default) with the first one; smx_set_inv_pairs(1) (default) applies FP64
+ * inverses of even order with the row-pair kernel, 0 with the first one. */
+int smx_set_xprod_mfma(int on);
+int smx_set_gj_rows(int on);
+int smx_set_inv_pairs(int on);
+
 /* Autotuner table (no counterpart in the reference; the choices are measured
  * by specmc_amd.autotune): the kernel used for GEMMs of precision `prec`,
  * shape M x N x K, nz batch entries and operand layout (a_kfast: acs == 1,

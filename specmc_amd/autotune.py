@@ -122,8 +122,10 @@ def tune_pc(eng, iters):
     zout = torch.zeros_like(vin)
     ev = lambda: torch.cuda.Event(enable_timing=True)
     res = {}
+    from .engine import inverse_fits_lds
     keep = eng.pc_apply
-    for mode in ("inv", "lu"):
+    modes = ("inv", "lu") if inverse_fits_lds(N, eng.esize) else ("lu",)
+    for mode in modes:
         eng.pc_apply = mode
         work = mats.clone()
         eng._pc_factor(work, piv, None)

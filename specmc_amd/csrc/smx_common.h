@@ -108,11 +108,31 @@ static inline size_t lds_limit() {
   return lim;
 }
 
+// the attribute is set once per kernel for the largest size requested so
+// far (the call is not a stream operation, and repeating it per launch would
+// also repeat it inside graph captures)
 template <typename K>
 static inline bool lds_optin(K kern, size_t bytes) {
   if (bytes <= 64 * 1024) return true;
   if (bytes > lds_limit()) return false;
-  return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bytes) == hipSuccess;
+  static thread_local const void *seen[64];
+  static thread_local size_t seen_bytes[64];
+  static thread_local int nseen = 0;
+  const void *k = (const void *)kern;
+  for (int i = 0; i < nseen; ++i)
+    if (seen[i] == k && seen_bytes[i] >= bytes) return true;
+  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+      hipSuccess)
+    return false;
+  for (int i = 0; i < nseen; ++i)
+    if (seen[i] == k) {
+      seen_bytes[i] = bytes;
+      return true;
+    }
+  if (nseen < 64) {
+    seen[nseen] = k;
+    seen_bytes[nseen++] = bytes;
+  }
+  return true;
 }
 

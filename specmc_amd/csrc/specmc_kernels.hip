@@ -187,42 +187,46 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
 #pragma unroll
     for (int b = 0; b < BN; ++b) acc[a][b] = VT{T(0), T(0), T(0), T(0)};
   const bool a_kfast = (g.acs == 1), b_jfast = (g.bcs == 1);
+  // register double buffering: the global loads of chunk k0 + KC are in
+  // flight while the matrix cores work on chunk k0 from LDS
+  constexpr int NA = KC * TM / 256, NB = KC * TN / 256;
+  T ra[NA], rb[NB];
+  auto load_chunk = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      int e = u * 256 + tid, ii, kk;
+      if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
+      const int i = i0 + ii, k = k0 + kk;
+      const bool ok = (i < g.M && k < g.K);
+      ra[u] = A[ok ? (long)i * g.ars + (long)k * g.acs : 0];
+      ra[u] = ok ? ra[u] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      int e = u * 256 + tid, jj, kb;
+      if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
+      const int j = j0 + jj, kq = k0 + kb;
+      const bool ok = (j < g.N && kq < g.K);
+      rb[u] = Bm[ok ? (long)kq * g.brs + (long)j * g.bcs : 0];
+      rb[u] = ok ? rb[u] : T(0);
+    }
+  };
+  load_chunk(0);
   for (int k0 = 0; k0 < g.K; k0 += KC) {
-    {
-      constexpr int NA = KC * TM / 256, NB = KC * TN / 256;
-      T ra[NA], rb[NB];
 #pragma unroll
-      for (int u = 0; u < NA; ++u) {
-        int e = u * 256 + tid, ii, kk;
-        if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
-        const int i = i0 + ii, k = k0 + kk;
-        const bool ok = (i < g.M && k < g.K);
-        ra[u] = A[ok ? (long)i * g.ars + (long)k * g.acs : 0];
-        ra[u] = ok ? ra[u] : T(0);
-      }
+    for (int u = 0; u < NA; ++u) {
+      int e = u * 256 + tid, ii, kk;
+      if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
+      As[kk][ii] = ra[u];
+    }
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        int e = u * 256 + tid, jj, kb;
-        if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
-        const int j = j0 + jj, kq = k0 + kb;
-        const bool ok = (j < g.N && kq < g.K);
-        rb[u] = Bm[ok ? (long)kq * g.brs + (long)j * g.bcs : 0];
-        rb[u] = ok ? rb[u] : T(0);
-      }
-#pragma unroll
-      for (int u = 0; u < NA; ++u) {
-        int e = u * 256 + tid, ii, kk;
-        if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
-        As[kk][ii] = ra[u];
-      }
-#pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        int e = u * 256 + tid, jj, kb;
-        if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
-        Bs[kb][jj] = rb[u];
-      }
+    for (int u = 0; u < NB; ++u) {
+      int e = u * 256 + tid, jj, kb;
+      if (b_jfast) { kb = e / TN; jj = e % TN; } else { jj = e / KC; kb = e % KC; }
+      Bs[kb][jj] = rb[u];
     }
     __syncthreads();
+    if (k0 + KC < g.K) load_chunk(k0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KC; ks += 4) {
       T a[BM], b[BN];
@@ -259,6 +263,28 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
       }
     }
   }
+}
+
+// Gauss-Jordan variant: 1 row-owner kernel, 0 the first kernel (default: at
+// 96 x 96 the row-owner kernel measured 40 ms for 8448 inverses against 11.7)
+static int g_gj_rows = 0;
+extern "C" int smx_set_gj_rows(int on) {
+  g_gj_rows = on ? 1 : 0;
+  return 0;
+}
+
+// inverse-apply variant for FP64: 1 row-pair kernel (default for even N), 0 the first
+static int g_inv_pairs = 1;
+extern "C" int smx_set_inv_pairs(int on) {
+  g_inv_pairs = on ? 1 : 0;
+  return 0;
+}
+
+// k_xprod variant for FP64: 1 matrix cores (default), 0 the vector kernel
+static int g_xprod_mfma = 1;
+extern "C" int smx_set_xprod_mfma(int on) {
+  g_xprod_mfma = on ? 1 : 0;
+  return 0;
 }
 
 // GEMM kernel family: 0 vector FMA, 1 matrix cores with 16-wide K chunks
@@ -790,6 +816,86 @@ __global__ void __launch_bounds__(256) k_gj_invert(T *Am, int N, int per_run, co
   for (int e = tid; e < NN; e += nt) G[e] = A[e];
 }
 
+// Gauss-Jordan inversion, row-owner form: thread (r = tid % N, g = tid / N)
+// owns row r of the columns q = g, g + G, ... (G = blockDim / N), so the
+// elimination needs no index arithmetic per entry.  Four barriers per
+// column: pivot search (wavefront 0), row swap, pivot column / scaled pivot
+// row copies, update.  Same pivot choice (first largest |a| at or below the
+// diagonal) and the same arithmetic per entry as k_gj_invert.
+template <typename T>
+__global__ void __launch_bounds__(512) k_gj_invert_rows(T *Am, int N, int per_run,
+                                                        const int *active) {
+  extern __shared__ unsigned char smem_raw[];
+  T *A = (T *)smem_raw;                 // [col][row]
+  T *fcol = A + N * N;                  // pivot column after the swap
+  T *prow = fcol + N;                   // scaled pivot row
+  int *sw = (int *)(prow + N);          // row swap of each step
+  __shared__ int s_piv;
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  T *Gm = Am + (long)m * N * N;
+  const int tid = threadIdx.x, nt = blockDim.x, G = nt / N;
+  const int r = tid % N, g = tid / N;   // every thread of the block has g < G
+  for (int e = tid; e < N * N; e += nt) A[e] = Gm[e];
+  __syncthreads();
+  for (int c = 0; c < N; ++c) {
+    if (tid < 64) {
+      T best = T(-1);
+      int bi = N;
+      for (int rr = c + tid; rr < N; rr += 64) {
+        const T v = fabs(A[c * N + rr]);
+        if (v > best) { best = v; bi = rr; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const T ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (tid == 0) {
+        if (bi >= N) bi = c;
+        s_piv = bi;
+        sw[c] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != c)
+      for (int q = tid; q < N; q += nt) {
+        const T t = A[q * N + c];
+        A[q * N + c] = A[q * N + p];
+        A[q * N + p] = t;
+      }
+    __syncthreads();
+    const T inv = T(1) / A[c * N + c];
+    for (int e = tid; e < N; e += nt) {
+      fcol[e] = A[c * N + e];
+      prow[e] = A[e * N + c] * inv;
+    }
+    __syncthreads();
+    const T fr = fcol[r];
+#pragma unroll 4
+    for (int q = g; q < N; q += G) {
+      T v;
+      if (q == c) v = (r == c) ? inv : -fr * inv;
+      else v = (r == c) ? prow[q] : A[q * N + r] - fr * prow[q];
+      A[q * N + r] = v;
+    }
+    __syncthreads();
+  }
+  // undo the row swaps as column swaps, last step first
+  for (int c = N - 1; c >= 0; --c) {
+    const int pr = sw[c];
+    if (pr != c)
+      for (int rr = tid; rr < N; rr += nt) {
+        const T t = A[c * N + rr];
+        A[c * N + rr] = A[pr * N + rr];
+        A[pr * N + rr] = t;
+      }
+    __syncthreads();
+  }
+  for (int e = tid; e < N * N; e += nt) Gm[e] = A[e];
+}
+
 // x[m] = inv[m] (scale * b[m]) (b[N-2] = b[N-1] = 0 when tau_zero); column-major inv.
 // One block per system: the 4 wavefronts split the columns, lanes the rows.
 template <typename T>
@@ -825,6 +931,56 @@ __global__ void __launch_bounds__(256) k_inv_apply(const T *Inv, const T *rhs, T
   __syncthreads();
   for (int r = tid; r < N; r += 256)
     x[(long)m * N + r] = (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+}
+
+// FP64 form of k_inv_apply for even N: thread t owns the row pair
+// (2 (t % RP), 2 (t % RP) + 1), RP = N / 2, for the columns c = t / RP,
+// t / RP + CG, ... (CG = 256 / RP column groups), reading each column as
+// 16-byte pieces, so a wavefront's loads cover whole column segments and
+// nearly every lane works; partial sums meet in LDS.
+__global__ void __launch_bounds__(256) k_inv_apply_pairs(const double *Inv, const double *rhs,
+                                                         double *x, int N, int tau_zero,
+                                                         double scale, const double *kscale,
+                                                         const double *bscale, int per_run,
+                                                         int shared, const int *active,
+                                                         unsigned long long *units) {
+  __shared__ double bs[LUS_MAXN];
+  __shared__ double part[2 * 256];
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  if (units && threadIdx.x == 0) atomicAdd(units, 1ull);
+  const long mi = shared ? (m % per_run) : m;
+  const double *A = Inv + mi * (long)N * N;
+  const int tid = threadIdx.x;
+  double s = kscale ? scale * kscale[m % per_run] : scale;
+  if (bscale) s = s * bscale[m / per_run];
+  for (int r = tid; r < N; r += 256) {
+    double v = rhs[(long)m * N + r] * s;
+    if (tau_zero && r >= N - 2) v = 0.0;
+    bs[r] = v;
+  }
+  __syncthreads();
+  const int RP = N >> 1, CG = 256 / RP;
+  const int rp = tid % RP, cg = tid / RP;
+  if (cg < CG) {
+    double a0 = 0.0, a1 = 0.0;
+    const double *col = A + 2 * rp;
+#pragma unroll 4
+    for (int c = cg; c < N; c += CG) {
+      const double2 v = *(const double2 *)(col + (long)c * N);
+      const double b = bs[c];
+      a0 = fma(v.x, b, a0);
+      a1 = fma(v.y, b, a1);
+    }
+    part[cg * N + 2 * rp] = a0;
+    part[cg * N + 2 * rp + 1] = a1;
+  }
+  __syncthreads();
+  for (int r = tid; r < N; r += 256) {
+    double acc = 0.0;
+    for (int g = 0; g < CG; ++g) acc += part[g * N + r];
+    x[(long)m * N + r] = acc;
+  }
 }
 
 extern "C" int smx_lu_factor(int prec, int nmat, int N, void *LU, int *piv,
@@ -946,6 +1102,25 @@ extern "C" int smx_invert(int prec, int nmat, int N, void *A, int per_run, const
                           void *stream) {
   if (nmat <= 0) return 0;
   if (N <= 0 || per_run <= 0) return fail_msg("smx_invert: bad sizes");
+  if (N >= 8 && N <= 512 && g_gj_rows) {
+    // row-owner kernel: N * G threads, G = min(N, 512 / N)
+    const int G = max(1, min(N, 512 / N));
+    const size_t es = prec == 0 ? 8 : 4;
+    const size_t sh = (size_t)N * N * es + 2 * (size_t)N * es + (size_t)N * 4;
+    if (prec == 0) {
+      if (!lds_optin(k_gj_invert_rows<double>, sh))
+        return fail_msg("smx_invert: matrix exceeds LDS");
+      hipLaunchKernelGGL(k_gj_invert_rows<double>, dim3(nmat), dim3(N * G), sh, STREAM(stream),
+                         (double *)A, N, per_run, active);
+    } else {
+      if (!lds_optin(k_gj_invert_rows<float>, sh))
+        return fail_msg("smx_invert: matrix exceeds LDS");
+      hipLaunchKernelGGL(k_gj_invert_rows<float>, dim3(nmat), dim3(N * G), sh, STREAM(stream),
+                         (float *)A, N, per_run, active);
+    }
+    CHECK_LAUNCH("smx_invert");
+    return 0;
+  }
   const size_t sh = (size_t)N * N * (prec == 0 ? 8 : 4) + (size_t)N * 4;
   if (prec == 0) {
     if (!lds_optin(k_gj_invert<double>, sh)) return fail_msg("smx_invert: matrix exceeds LDS");
@@ -968,7 +1143,12 @@ extern "C" int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const v
   if (N <= 0 || N > LUS_MAXN || per_run <= 0) return fail_msg("smx_inv_apply: bad sizes");
   const bool clk = clk_on(2);
   if (clk) clk_begin(STREAM(stream));
-  if (prec == 0)
+  if (prec == 0 && (N % 2) == 0 && N >= 4 && g_inv_pairs)
+    hipLaunchKernelGGL(k_inv_apply_pairs, dim3(nmat), dim3(256), 0, STREAM(stream),
+                       (const double *)Inv, (const double *)rhs, (double *)x, N, tau_zero,
+                       rhs_scale, (const double *)kscale, (const double *)bscale, per_run,
+                       shared, active, clk_units(2));
+  else if (prec == 0)
     hipLaunchKernelGGL(k_inv_apply<double>, dim3(nmat), dim3(256), 0, STREAM(stream),
                        (const double *)Inv, (const double *)rhs, (double *)x, N, tau_zero,
                        rhs_scale, (const double *)kscale, (const double *)bscale, per_run,
@@ -1344,6 +1524,103 @@ __global__ void __launch_bounds__(256) k_xprod(int nf, int nodd, int Nk, int nro
   }
 }
 
+// FP64 matrix-core form of k_xprod: one block per (run, field, 32-column
+// tile).  The synthesis G = S Y (rows nrow, K = Nk) and the analysis
+// out = A (eta .* G) (rows Nk, K = nrow) run on v_mfma_f64_16x16x4f64: each
+// wavefront owns 16 x 16 output tiles; A fragments come from L1/L2 (the
+// transform matrices are a few tens of KB per shape), B fragments from LDS.
+// The product with eta happens on the synthesis accumulators before they
+// are staged in LDS as the analysis operand.
+#define XT 32
+// one 16 x 16 output tile: acc += Arow[0..K) (lane row) x Bs[k][col], K in
+// steps of 4; with KTC > 0 the K loop is unrolled and every A fragment is
+// loaded before the first MFMA (the fragments come from L1/L2)
+template <int KTC>
+__device__ __forceinline__ v4f64 xp_tile(const double *Ar, bool row_ok, int Klim, int KT,
+                                         const double *Bs, int ld, int col, int lk) {
+  v4f64 acc = v4f64{0.0, 0.0, 0.0, 0.0};
+  if (KTC > 0) {
+    double af[KTC > 0 ? KTC : 1];
+#pragma unroll
+    for (int ks = 0; ks < KTC; ++ks) {
+      const int k = ks * 4 + lk;
+      af[ks] = (row_ok && k < Klim) ? Ar[k] : 0.0;
+    }
+#pragma unroll
+    for (int ks = 0; ks < KTC; ++ks)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ks], Bs[(ks * 4 + lk) * ld + col], acc, 0,
+                                                 0, 0);
+  } else {
+    for (int ks = 0; ks < KT; ++ks) {
+      const int k = ks * 4 + lk;
+      const double a = (row_ok && k < Klim) ? Ar[k] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[k * ld + col], acc, 0, 0, 0);
+    }
+  }
+  return acc;
+}
+
+template <int KT1C, int KT2C>
+__global__ void __launch_bounds__(256) k_xprod_mfma(int nf, int nodd, int Nk, int nrow, int Q,
+                                                    const double *y, const double *fscale,
+                                                    const double *eta, long eta_bs, double *out,
+                                                    const double *So, const double *Ce,
+                                                    const double *Ao, const double *Ae,
+                                                    const int *active) {
+  extern __shared__ unsigned char smem_raw[];
+  const int KP1 = (Nk + 3) & ~3;          // synthesis K, padded to the MFMA K step
+  const int MP1 = (nrow + 15) & ~15;      // grid rows, padded to the MFMA tile
+  double *Ys = (double *)smem_raw;        // [KP1][XT + 1]
+  double *Hs = Ys + KP1 * (XT + 1);       // [MP1][XT + 1]
+  const int b = blockIdx.z, f = blockIdx.y, q0 = blockIdx.x * XT;
+  if (active && !active[b]) return;
+  const bool odd = f < nodd;
+  const double *S = odd ? So : Ce;        // [nrow][Nk]
+  const double *An = odd ? Ao : Ae;       // [Nk][nrow]
+  const long rs = (long)nf * Q;
+  const double *yb = y + (long)b * Nk * rs + (long)f * Q;
+  const double *eb = eta + (long)b * eta_bs;
+  double *ob = out + (long)b * Nk * rs + (long)f * Q;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int e = tid; e < KP1 * XT; e += 256) {
+    const int m = e / XT, c = e % XT, q = q0 + c;
+    double v = 0.0;
+    if (m < Nk && q < Q) {
+      v = yb[(long)m * rs + q];
+      if (fscale) v *= fscale[(long)f * Nk + m];
+    }
+    Ys[m * (XT + 1) + c] = v;
+  }
+  __syncthreads();
+  const int MT1 = MP1 / 16;
+  for (int t = w; t < MT1 * 2; t += 4) {
+    const int ti = t >> 1, tj = t & 1;
+    const int row = ti * 16 + lr;
+    const v4f64 acc = xp_tile<KT1C>(S + (long)(row < nrow ? row : 0) * Nk, row < nrow, Nk,
+                                    KP1 / 4, Ys, XT + 1, tj * 16 + lr, lk);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = ti * 16 + lk + 4 * r, c = tj * 16 + lr, q = q0 + c;
+      const double ev = (j < nrow && q < Q) ? eb[(long)j * Q + q] : 0.0;
+      Hs[j * (XT + 1) + c] = acc[r] * ev;
+    }
+  }
+  __syncthreads();
+  const int MT2 = (Nk + 15) / 16;
+  for (int t = w; t < MT2 * 2; t += 4) {
+    const int ti = t >> 1, tj = t & 1;
+    const int row = ti * 16 + lr;
+    const v4f64 acc = xp_tile<KT2C>(An + (long)(row < Nk ? row : 0) * nrow, row < Nk, nrow,
+                                    MP1 / 4, Hs, XT + 1, tj * 16 + lr, lk);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = ti * 16 + lk + 4 * r, q = q0 + tj * 16 + lr;
+      if (m < Nk && q < Q) ob[(long)m * rs + q] = acc[r];
+    }
+  }
+}
+
 extern "C" int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
                          const void *y, const void *fscale, const void *eta, long eta_bs,
                          void *out,
@@ -1351,9 +1628,35 @@ extern "C" int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, in
                          const void *Ce, const void *Ao, const void *Ae, const int *active,
                          void *stream) {
   if (B <= 0) return 0;
+  if (prec == 0 && g_xprod_mfma) {
+    const int KP1 = (Nk + 3) & ~3, MP1 = (nrow + 15) & ~15;
+    const size_t sh = (size_t)(KP1 + MP1) * (XT + 1) * 8;
+    const dim3 grid(cdiv(Q, XT), nf, B);
+    hipStream_t st = STREAM(stream);
+#define XPM(A_, B_)                                                                           \
+  do {                                                                                        \
+    if (!lds_optin(k_xprod_mfma<A_, B_>, sh))                                                 \
+      return fail_msg("smx_xprod: grid too large for LDS");                                  \
+    hipLaunchKernelGGL((k_xprod_mfma<A_, B_>), grid, dim3(256), sh, st, nf, nodd, Nk, nrow,   \
+                       Q, (const double *)y, (const double *)fscale, (const double *)eta,     \
+                       eta_bs, (double *)out, (const double *)So, (const double *)Ce,         \
+                       (const double *)Ao, (const double *)Ae, active);                       \
+  } while (0)
+    // unrolled K loops for the grid widths Nx = 16, 32, 64, 128
+    const int KT1 = KP1 / 4, KT2 = MP1 / 4;
+    if (KT1 == 3 && KT2 == 8) XPM(3, 8);
+    else if (KT1 == 5 && KT2 == 12) XPM(5, 12);
+    else if (KT1 == 9 && KT2 == 20) XPM(9, 20);
+    else if (KT1 == 17 && KT2 == 36) XPM(17, 36);
+    else XPM(0, 0);
+#undef XPM
+    CHECK_LAUNCH("smx_xprod");
+    return 0;
+  }
   dim3 grid(cdiv(Q, XC), nf, B);
   size_t sh = (size_t)(Nk + nrow) * XC * (prec == 0 ? 8 : 4);
-  if (sh > 64 * 1024) return fail_msg("smx_xprod: grid too large for LDS");
+  if (!(prec == 0 ? lds_optin(k_xprod<double>, sh) : lds_optin(k_xprod<float>, sh)))
+    return fail_msg("smx_xprod: grid too large for LDS");
   if (prec == 0)
     hipLaunchKernelGGL(k_xprod<double>, grid, dim3(256), sh, STREAM(stream), nf, nodd, Nk,
                        nrow, Q, (const double *)y, (const double *)fscale,
@@ -1430,7 +1733,8 @@ extern "C" int smx_advect_prod(int prec, int B, int Nk, int nrow, int N, const v
   if (B <= 0) return 0;
   dim3 grid(cdiv(N, XC), B);
   size_t sh = (size_t)(2 * Nk + nrow) * XC * (prec == 0 ? 8 : 4);
-  if (sh > 64 * 1024) return fail_msg("smx_advect_prod: grid too large for LDS");
+  if (!(prec == 0 ? lds_optin(k_advect<double>, sh) : lds_optin(k_advect<float>, sh)))
+    return fail_msg("smx_advect_prod: grid too large for LDS");
   if (prec == 0)
     hipLaunchKernelGGL(k_advect<double>, grid, dim3(256), sh, STREAM(stream), Nk, nrow, N,
                        (const double *)y, (const double *)ks0, (const double *)ux,
@@ -2086,40 +2390,46 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
         }
       }
     } else if constexpr (E > 0) {
-      // modified Gram-Schmidt with v_{i+1} loaded while h_i is reduced
+      // modified Gram-Schmidt in the reference's order, with the basis
+      // vectors v_{i+1}, v_{i+2} already loading while h_i is reduced
       __shared__ double red2[32];
       __syncthreads();
-      T vc[E], vn[E];
+      T v0[E], v1[E], v2[E];
+      auto ld = [&](T *v, int i) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) {
-        long e = tid + (long)q * nt;
-        vc[q] = (q < per && e < n) ? Vr[e] : T(0);
-      }
-      for (int i = 0; i <= j; ++i) {
-        if (i < j) {
-          const T *vnx = Vr + (long)(i + 1) * n;
-#pragma unroll
-          for (int q = 0; q < E; ++q) {
-            long e = tid + (long)q * nt;
-            vn[q] = (q < per && e < n) ? vnx[e] : T(0);
-          }
+        for (int q = 0; q < E; ++q) {
+          long e = tid + (long)q * nt;
+          v[q] = (i <= j && q < per && e < n) ? Vr[(long)i * n + e] : T(0);
         }
+      };
+      auto mgs = [&](const T *v, int i) {
         double s = 0;
 #pragma unroll
         for (int q = 0; q < E; ++q) {
           long e = tid + (long)q * nt;
-          if (q < per && e < n) s += (double)vc[q] * (double)W.v[q];
+          if (q < per && e < n) s += (double)v[q] * (double)W.v[q];
         }
         s = block_sum_db(s, red2, i);
         const T h = (T)s;
 #pragma unroll
         for (int q = 0; q < E; ++q) {
           long e = tid + (long)q * nt;
-          if (q < per && e < n) W.v[q] = W.v[q] - h * vc[q];
+          if (q < per && e < n) W.v[q] = W.v[q] - h * v[q];
         }
         if (tid == 0) hcol[i] = h;
-#pragma unroll
-        for (int q = 0; q < E; ++q) vc[q] = vn[q];
+      };
+      ld(v0, 0);
+      ld(v1, 1);
+      ld(v2, 2);
+      for (int i = 0; i <= j; i += 3) {
+        mgs(v0, i);
+        ld(v0, i + 3);
+        if (i + 1 > j) break;
+        mgs(v1, i + 1);
+        ld(v1, i + 4);
+        if (i + 2 > j) break;
+        mgs(v2, i + 2);
+        ld(v2, i + 5);
       }
     } else {
       for (int i = 0; i <= j; ++i) {

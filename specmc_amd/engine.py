@@ -57,6 +57,14 @@ def _law_row(p):
                      float(p.get("eta_smooth", 0.0)), 1.0 if Ts > 0.0 else 0.0])
 
 
+LDS_BYTES = 160 * 1024      # per workgroup on gfx950
+
+
+def inverse_fits_lds(N, esize):
+    """Whether the Gauss-Jordan inversion of an N x N block fits in LDS."""
+    return N * N * esize + 2 * N * esize + 4 * N <= LDS_BYTES
+
+
 class Solver:
     """Device FGMRES workspace for B independent systems of size n
     (krylov_fgmres.py:60-179 per run, driven in lockstep rounds)."""
@@ -138,7 +146,7 @@ class Solver:
                 self._round(precond, op)
                 rounds += 1
                 graph = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(graph):
+                with torch.cuda.graph(graph, capture_error_mode="thread_local"):
                     for _ in range(G):
                         self._round(precond, op)
                 self._graphs[key] = graph
@@ -222,6 +230,8 @@ class Engine:
         self.ops = shape_ops(Nx, Nz, Lx)
         o = self.ops
         self.Nx, self.Nz, self.Nk, self.Q = o.Nx, o.Nz, o.Nk, o.Q
+        if self.pc_apply == "inv" and not inverse_fits_lds(self.Nz, self.esize):
+            self.pc_apply = "lu"     # the in-LDS inversion does not take this Nz
         self.nrp, self.nru = o.nrp, o.nru
         self.n = self.Nk * self.Nz
         self.B = B = len(params)
@@ -547,7 +557,8 @@ class Engine:
                       self.p(t["Ae_p"]), self.p(active), s)
         self.zrows(self.s1, t["Vi"], self.nc, N, N, N, N, active=active)
         # rhs = theta / dt + Nc ; out = Z_lam^{-1}(-rhs), lam = 1/dt
-        inv_dt = 1.0 / np.asarray(dt, dtype=np.float64)
+        dt = np.asarray(dt, dtype=np.float64)
+        inv_dt = 1.0 / np.where(dt > 0.0, dt, 1.0)     # runs outside the mask carry dt = 0
         self.lam.copy_(torch.from_numpy(inv_dt).to(self.dtype))
         refac = np.where(self._lam_fact != inv_dt)[0]
         if active is not None:
@@ -572,6 +583,8 @@ class Engine:
         cached Helmholtz factors are rebuilt at the next step."""
         if mode not in ("lu", "inv"):
             raise ValueError("pc_apply must be 'lu' or 'inv', got %r" % (mode,))
+        if mode == "inv" and not inverse_fits_lds(self.Nz, self.esize):
+            raise ValueError("pc_apply='inv' needs the %d x %d block in LDS" % (self.Nz, self.Nz))
         if mode != self.pc_apply:
             self.pc_apply = mode
             self._a0_fact[:] = np.nan

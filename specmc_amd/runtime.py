@@ -30,6 +30,9 @@ SIGNATURES = {
     "smx_clock_start": [I, I],
     "smx_clock_stop": [P, P, P],
     "smx_set_gemm_mfma": [I],
+    "smx_set_xprod_mfma": [I],
+    "smx_set_gj_rows": [I],
+    "smx_set_inv_pairs": [I],
     "smx_dense_lu": [I, I, I, P, L, P, P, P],
     "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P],
     "smx_inv_apply": [I, I, I, P, P, P, I, D, P, P, I, I, P, P],

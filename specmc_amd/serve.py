@@ -21,6 +21,7 @@ aggregate run-steps/s.
 """
 import time
 from collections import OrderedDict, deque
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -89,6 +90,7 @@ class Bucket:
         self.live = np.zeros(self.S, dtype=bool)
         self.steps = 0
         self.last_used = 0.0
+        self.stream = None          # own HIP stream when buckets step concurrently
 
     def free_slots(self):
         return [i for i in range(self.S) if self.req[i] is None]
@@ -151,10 +153,13 @@ class Server:
     ``slots``     slots per bucket (at most; fewer when memory is short);
     ``mem_budget`` HBM bytes the buckets may hold (default: 80% of the free
                   device memory at construction, less the rescue headroom);
-    ``precision`` "fp64" or "mixed" (see ``engine.PRECISIONS``)."""
+    ``precision`` "fp64" or "mixed" (see ``engine.PRECISIONS``);
+    ``concurrent`` step the buckets of a round in worker threads, each bucket
+                  on its own HIP stream, so the device overlaps them (the
+                  round is joined before admission and retirement)."""
 
     def __init__(self, slots=64, mem_budget=None, precision="fp64", safety=None, dtmax=None,
-                 diagnostics=True, **engine_kw):
+                 diagnostics=True, concurrent=False, workers=8, **engine_kw):
         from .engine import RESCUE_BYTES
         self.slots = int(slots)
         if mem_budget is None:
@@ -173,6 +178,8 @@ class Server:
         self.run_steps = 0
         self.t0 = None
         self.t_end = None
+        self.concurrent = bool(concurrent)
+        self._pool = ThreadPoolExecutor(max_workers=int(workers)) if concurrent else None
 
     # ---------------------------------------------------------------- input
     def submit(self, requests):
@@ -229,20 +236,32 @@ class Server:
     def now(self):
         return time.perf_counter() - self.t0
 
-    def run(self, max_wall=None, on_retire=None):
+    def run(self, max_wall=None, on_retire=None, progress=None, every=15.0):
         """Serve until every submitted request is finished (or ``max_wall``
-        seconds have passed)."""
+        seconds have passed).  ``progress(stats)`` is called about every
+        ``every`` seconds of serving."""
         if self.t0 is None:
             self.t0 = time.perf_counter()
+        next_report = self.now() + every
         while self.pending or any(b.busy() for b in self.buckets.values()):
+            if progress is not None and self.now() >= next_report:
+                next_report = self.now() + every
+                progress(dict(t=self.now(), done=len(self.done), pending=len(self.pending),
+                              live=sum(int(b.live.sum()) for b in self.buckets.values()),
+                              run_steps=self.run_steps, buckets=len(self.buckets)))
             now = self.now()
             if max_wall is not None and now > max_wall:
                 break
             self._admit(now)
             stepped = 0
-            for b in list(self.buckets.values()):
-                stepped += b.step(self.safety, self.dtmax)
-                fin = b.retire(self.now(), self.diagnostics)
+            busy = [b for b in self.buckets.values() if b.busy()]
+            if self.concurrent and len(busy) > 1:
+                results = [f.result() for f in [self._pool.submit(self._round, b) for b in busy]]
+            else:
+                results = [(b, b.step(self.safety, self.dtmax),
+                            b.retire(self.now(), self.diagnostics)) for b in busy]
+            for b, n, fin in results:
+                stepped += n
                 if fin:
                     b.last_used = self.now()
                     self.done.extend(fin)
@@ -260,6 +279,19 @@ class Server:
         torch.cuda.synchronize()
         self.t_end = self.now()
         return self.done
+
+    def _round(self, b):
+        """One step and the retirements of bucket ``b`` on its own stream
+        (worker thread); the stream first waits for the default stream's
+        work (admissions), and is drained before the round joins."""
+        if b.stream is None:
+            b.stream = torch.cuda.Stream()
+        b.stream.wait_stream(torch.cuda.default_stream())
+        with torch.cuda.stream(b.stream):
+            n = b.step(self.safety, self.dtmax)
+            fin = b.retire(self.now(), self.diagnostics)
+        b.stream.synchronize()
+        return b, n, fin
 
     def stats(self):
         lat = np.array([r.latency for r in self.done if r.latency is not None])

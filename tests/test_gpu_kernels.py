@@ -1,0 +1,117 @@
+"""Kernel variants on the device (MI355X) against each other and float64 host
+arithmetic: the matrix-core pseudo-spectral product against the vector
+kernel and a numpy restatement; the row-owner Gauss-Jordan inversion against
+numpy.linalg.inv and the first kernel.
+
+Tolerance: 1e-13 relative (the variants reorder floating-point sums only).
+"""
+import numpy as np
+import pytest
+import torch
+
+from golden_io import rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a HIP device")
+    from specmc_amd import runtime
+    return runtime.load()
+
+
+@pytest.mark.parametrize("shape", [(16, 24), (64, 96), (128, 96), (256, 192)])
+def test_xprod_mfma_matches_vector_kernel_and_numpy(lib, shape):
+    from specmc_amd.basis import shape_ops
+    Nx, Nz = shape
+    o = shape_ops(Nx, Nz, 2.0)
+    dev = torch.device("cuda")
+    t = o.device_tensors(torch.float64, dev)
+    Nk, nrow, Q, nf, nodd, B = o.Nk, o.nrp, o.Q, 4, 3, 3
+    rng = np.random.default_rng(Nx + Nz)
+    y = rng.normal(size=(B, Nk, nf, Q))
+    fs = rng.normal(size=(nf, Nk))
+    eta = rng.uniform(0.1, 2.0, size=(B, nrow, Q))
+    dy, dfs, de = (torch.tensor(a, device=dev) for a in (y, fs, eta))
+    outs = []
+    s = torch.cuda.current_stream().cuda_stream
+    for on in (1, 0):
+        lib.set_xprod_mfma(on)
+        out = torch.zeros(B, Nk, nf, Q, dtype=torch.float64, device=dev)
+        lib.xprod(0, B, nf, nodd, Nk, nrow, Q, dy.data_ptr(), dfs.data_ptr(), de.data_ptr(),
+                  nrow * Q, out.data_ptr(), t["So_p"].data_ptr(), t["Ce_p"].data_ptr(),
+                  t["Ao_p"].data_ptr(), t["Ae_p"].data_ptr(), None, s)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    lib.set_xprod_mfma(1)
+    So, Ce = t["So_p"].cpu().numpy(), t["Ce_p"].cpu().numpy()
+    Ao, Ae = t["Ao_p"].cpu().numpy(), t["Ae_p"].cpu().numpy()
+    want = np.zeros_like(y)
+    for b in range(B):
+        for f in range(nf):
+            S, An = (So, Ao) if f < nodd else (Ce, Ae)
+            g = S @ (fs[f][:, None] * y[b, :, f, :])
+            want[b, :, f, :] = An @ (g * eta[b])
+    e_mfma, e_vec = rel(outs[0], want), rel(outs[1], want)
+    print("xprod %s: |want| %.3e |mfma| %.3e |vector| %.3e; mfma %.2e, vector %.2e, "
+          "max |mfma - vector| %.2e" % (shape, np.linalg.norm(want), np.linalg.norm(outs[0]),
+                                        np.linalg.norm(outs[1]), e_mfma, e_vec,
+                                        np.abs(outs[0] - outs[1]).max()), flush=True)
+    assert np.linalg.norm(want) > 0.0
+    assert e_mfma < 1e-13 and e_vec < 1e-13
+
+
+@pytest.mark.parametrize("N", [24, 48, 96])
+def test_gj_rows_inverse(lib, N):
+    rng = np.random.default_rng(N)
+    nm = 7
+    M = rng.normal(size=(nm, N, N)) + np.sqrt(N) * np.eye(N)
+    dev = torch.device("cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    got = []
+    for on in (1, 0):
+        lib.set_gj_rows(on)
+        A = torch.tensor(np.ascontiguousarray(M.transpose(0, 2, 1)), device=dev)
+        lib.invert(0, nm, N, A.data_ptr(), 1, None, s)
+        torch.cuda.synchronize()
+        got.append(A.cpu().numpy().transpose(0, 2, 1))
+    lib.set_gj_rows(0)
+    want = np.linalg.inv(M)
+    print("gj N=%d: rows %.2e, first %.2e, rows vs first %.2e" % (
+        N, rel(got[0], want), rel(got[1], want), rel(got[0], got[1])), flush=True)
+    assert rel(got[0], want) < 1e-12
+    assert rel(got[0], got[1]) < 1e-13
+
+
+@pytest.mark.parametrize("N", [24, 48, 96, 192])
+def test_inv_apply_pairs(lib, N):
+    """Row-pair inverse application against the first kernel and numpy, with
+    the tau zeroing, per-mode and per-run scales."""
+    rng = np.random.default_rng(7 + N)
+    nm, per_run = 12, 4
+    Inv = rng.normal(size=(nm, N, N))
+    b = rng.normal(size=(nm, N))
+    ks = rng.normal(size=per_run)
+    bsc = rng.normal(size=nm // per_run)
+    dev = torch.device("cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    dI = torch.tensor(np.ascontiguousarray(Inv.transpose(0, 2, 1)), device=dev)
+    db, dk, dbs = (torch.tensor(a, device=dev) for a in (b, ks, bsc))
+    got = []
+    for on in (1, 0):
+        lib.set_inv_pairs(on)
+        x = torch.zeros(nm, N, dtype=torch.float64, device=dev)
+        lib.inv_apply(0, nm, N, dI.data_ptr(), db.data_ptr(), x.data_ptr(), 1, -0.5,
+                      dk.data_ptr(), dbs.data_ptr(), per_run, 0, None, s)
+        torch.cuda.synchronize()
+        got.append(x.cpu().numpy())
+    lib.set_inv_pairs(1)
+    sc = -0.5 * ks[np.arange(nm) % per_run] * bsc[np.arange(nm) // per_run]
+    bz = b * sc[:, None]
+    bz[:, N - 2:] = 0.0
+    want = np.einsum("mij,mj->mi", Inv, bz)
+    print("inv_apply N=%d: pairs %.2e, first %.2e" % (N, rel(got[0], want), rel(got[1], want)),
+          flush=True)
+    assert rel(got[0], want) < 1e-13 and rel(got[1], want) < 1e-13

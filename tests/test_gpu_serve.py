@@ -49,7 +49,8 @@ def _requests():
     return out
 
 
-def test_served_mix_matches_solo(dev):
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_served_mix_matches_solo(dev, concurrent):
     from specmc_amd import api
     from specmc_amd.serve import Server
     reqs = _requests()
@@ -58,8 +59,10 @@ def test_served_mix_matches_solo(dev):
         m = api.build_model(r.cfg)
         s, info = api.march_adaptive(m, r.theta0, r.tstop)
         solo[r.rid] = (s, info["steps"], m.diagnostics(s))
-    # two slots per bucket: later arrivals reuse the slots of retired runs
-    srv = Server(slots=2)
+    # two slots per bucket: later arrivals reuse the slots of retired runs;
+    # concurrent: the buckets step in worker threads on their own streams,
+    # with graph-replayed FGMRES rounds
+    srv = Server(slots=2, concurrent=concurrent, graph_rounds=2 if concurrent else 0)
     srv.submit(_requests())
     done = srv.run(max_wall=300)
     st = srv.stats()
