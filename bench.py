@@ -187,9 +187,14 @@ def served_leg(z, log, w, B, value, horizon, load, nreq, precision, graph_rounds
     return st
 
 
-def run_c2(precisions=("fp64", "mixed")):
+class _WallLimit(Exception):
+    pass
+
+
+def run_c2(precisions=("fp64", "mixed"), max_wall=None):
     """The heated 2:1 case marched to t = 0.03 with the single-run model,
-    diagnostics at the first steps with t >= 0.01, 0.02, 0.03."""
+    diagnostics at the first steps with t >= 0.01, 0.02, 0.03 (stopped early,
+    and reported as such, past ``max_wall`` seconds per precision)."""
     from specmc_amd import api
     from specmc_amd.workloads import c2_requests, C2_LOG_TIMES
     r = c2_requests()[0]
@@ -198,8 +203,12 @@ def run_c2(precisions=("fp64", "mixed")):
         m = api.build_model(r.cfg, precision=prec)
         logs, marks = [], list(C2_LOG_TIMES)
         t0 = time.perf_counter()
+        last = {}
 
         def cb(n, t, s):
+            last.update(n=n, t=t, s=s)
+            if max_wall is not None and time.perf_counter() - t0 > max_wall:
+                raise _WallLimit()
             if n % 25 == 0:
                 print("c2 %s: step %d t=%.5f wall %.1fs" % (prec, n, t, time.perf_counter() - t0),
                       file=sys.stderr, flush=True)
@@ -210,10 +219,15 @@ def run_c2(precisions=("fp64", "mixed")):
                                  wall_s=time.perf_counter() - t0))
                 print("c2 %s: %s" % (prec, json.dumps(logs[-1])), file=sys.stderr, flush=True)
 
-        s, info = api.march_adaptive(m, r.theta0, r.tstop, on_step=cb)
+        try:
+            s, info = api.march_adaptive(m, r.theta0, r.tstop, on_step=cb)
+            complete = True
+        except _WallLimit:
+            s, info, complete = last["s"], dict(steps=last["n"], t=last["t"]), False
         wall = time.perf_counter() - t0
         dsc = m.describe()
-        out[prec] = dict(steps=info["steps"], wall_s=wall, steps_per_s=info["steps"] / wall,
+        out[prec] = dict(steps=info["steps"], t=info["t"], complete=complete, wall_s=wall,
+                         steps_per_s=info["steps"] / wall,
                          log=logs, rescues=dsc.get("n_bfbt_rescue"),
                          rescues_fp32=dsc.get("n_rescue_fp32"), theta=np.real(s))
     res = dict(metric="heated 2:1 run (64x96, Ra=1.25e4, H=1.5, eta_T 1e3) to t=0.03",
@@ -222,24 +236,27 @@ def run_c2(precisions=("fp64", "mixed")):
         o = dict(out[prec])
         o.pop("theta")
         res[prec] = o
-    if "fp64" in out and "mixed" in out:
+    if "fp64" in out and "mixed" in out and out["fp64"]["log"] and out["mixed"]["log"]:
         a, b = out["fp64"], out["mixed"]
         res["mixed_vs_fp64"] = dict(
-            theta_l2=float(np.linalg.norm(b["theta"] - a["theta"]) / np.linalg.norm(a["theta"])),
             **{k: max(abs(lb[k] - la[k]) / abs(la[k]) for la, lb in zip(a["log"], b["log"]))
                for k in ("Nu_top", "Nu_energy", "urms")})
+        if a["complete"] and b["complete"]:
+            res["mixed_vs_fp64"]["theta_l2"] = float(np.linalg.norm(b["theta"] - a["theta"])
+                                                     / np.linalg.norm(a["theta"]))
     return res
 
 
-def run_served(reqs, slots, precision, graph_rounds, max_wall, label):
+def run_served(reqs, slots, precision, graph_rounds, max_wall, label, concurrent=True):
     from specmc_amd.serve import Server
-    srv = Server(slots=slots, precision=precision, graph_rounds=graph_rounds)
+    srv = Server(slots=slots, precision=precision, graph_rounds=graph_rounds,
+                 concurrent=concurrent)
     srv.submit(reqs)
     srv.run(max_wall=max_wall,
             progress=lambda p: print("%s: %s" % (label, json.dumps(p)), file=sys.stderr,
                                      flush=True))
     st = srv.stats()
-    st.update(workload=label, precision=precision, submitted=len(reqs),
+    st.update(workload=label, precision=precision, concurrent=concurrent, submitted=len(reqs),
               unfinished=len(reqs) - st["requests"])
     return st
 
@@ -280,6 +297,9 @@ def main():
     ap.add_argument("--rate", type=float, default=0.5, help="c4 arrivals per second")
     ap.add_argument("--tstop", type=float, default=1e-3, help="c3 run length")
     ap.add_argument("--max-wall", type=float, default=600.0, help="c3 / c4 wall limit (s)")
+    ap.add_argument("--sequential", action="store_true",
+                    help="c3 / c4: step the buckets one after another on one stream")
+    ap.add_argument("--c2-precision", default="fp64,mixed")
     ap.add_argument("--make-states", action="store_true")
     ap.add_argument("--states-out", default=STATES)
     a = ap.parse_args()
@@ -297,15 +317,16 @@ def main():
     if a.workload != "c1":
         from specmc_amd import workloads as W
         if a.workload == "c2":
-            out = run_c2()
+            out = run_c2(tuple(a.c2_precision.split(",")), a.max_wall)
         elif a.workload == "c3":
             out = run_served(W.ensemble_requests(n=a.requests or 256, tstop=a.tstop),
                              a.slots, a.precision, a.graph_rounds, a.max_wall,
-                             "c3: 256-run uneven-grid ensemble, tstop=%g" % a.tstop)
+                             "c3: 256-run uneven-grid ensemble, tstop=%g" % a.tstop,
+                             not a.sequential)
         else:
             out = run_served(W.stream_requests(n=a.requests or 48, rate=a.rate),
                              a.slots, a.precision, a.graph_rounds, a.max_wall,
-                             "c4: Poisson stream at %g/s" % a.rate)
+                             "c4: Poisson stream at %g/s" % a.rate, not a.sequential)
         print(json.dumps(out), flush=True)
         return
     if not os.path.exists(STATES):

@@ -94,6 +94,9 @@ int smx_set_gemm_mfma(int mode);
  * inverses of even order with the row-pair kernel, 0 with the first one. */
 int smx_set_xprod_mfma(int on);
 int smx_set_gj_rows(int on);
+/* smx_set_gj_reg(1) (default) inverts FP64 blocks of order 16..128 with the
+ * matrix held in registers (4N threads), 0 with the kernels above. */
+int smx_set_gj_reg(int on);
 int smx_set_inv_pairs(int on);
 
 /* Autotuner table (no counterpart in the reference; the choices are measured

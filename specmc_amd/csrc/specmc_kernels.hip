@@ -265,6 +265,13 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   }
 }
 
+// FP64 inversion in registers (default for 16 <= N <= 128)
+static int g_gj_reg = 1;
+extern "C" int smx_set_gj_reg(int on) {
+  g_gj_reg = on ? 1 : 0;
+  return 0;
+}
+
 // Gauss-Jordan variant: 1 row-owner kernel, 0 the first kernel (default: at
 // 96 x 96 the row-owner kernel measured 40 ms for 8448 inverses against 11.7)
 static int g_gj_rows = 0;
@@ -789,13 +796,18 @@ __global__ void __launch_bounds__(256) k_gj_invert(T *Am, int N, int per_run, co
     // pivot row: a_cq /= a_cc, the pivot entry becomes 1/a_cc
     for (int q = tid; q < N; q += nt) A[q * N + c] = (q == c) ? inv : A[q * N + c] * inv;
     __syncthreads();
-    // eliminate column c from the other rows; column c takes -a_rc / a_cc
-    for (int e = tid; e < NN; e += nt) {
-      int q = e / N, r = e % N;
-      if (r == c) continue;
-      T f = A[c * N + r];
-      if (q == c) continue;
-      A[q * N + r] -= f * A[q * N + c];
+    // eliminate column c from the other rows; column c takes -a_rc / a_cc.
+    // Entry e = tid + i nt is (column q, row r); both advance by the fixed
+    // (dq, dr) = (nt / N, nt % N) per i, so no division per entry.
+    {
+      int q = tid / N, r = tid % N;
+      const int dq = nt / N, dr = nt % N;
+      for (int e = tid; e < NN; e += nt) {
+        if (r != c && q != c) A[q * N + r] -= A[c * N + r] * A[q * N + c];
+        q += dq;
+        r += dr;
+        if (r >= N) { r -= N; ++q; }
+      }
     }
     __syncthreads();
     for (int r = tid; r < N; r += nt)
@@ -814,6 +826,119 @@ __global__ void __launch_bounds__(256) k_gj_invert(T *Am, int N, int per_run, co
     __syncthreads();
   }
   for (int e = tid; e < NN; e += nt) G[e] = A[e];
+}
+
+// FP64 Gauss-Jordan inversion with the matrix in registers: 4N threads;
+// thread (r = tid % N, g = tid / N) holds row r of the columns q = g + 4 i
+// (i < CPT).  Per column step only the pivot column and the two swapped rows
+// pass through LDS.  Same pivot choice and the same arithmetic per entry as
+// k_gj_invert: the pivot row is scaled by 1/a_cc first, every other row
+// subtracts (its column-c entry) x (the scaled pivot row), column c becomes
+// -a_rc / a_cc with 1/a_cc on the diagonal.  The row swaps are undone at the
+// end as a permutation of the columns on the way out.
+#define GJR_MAXN 128
+template <int CPT>
+__global__ void __launch_bounds__(512) k_gj_invert_reg(double *Am, int N, int per_run,
+                                                       const int *active) {
+  __shared__ double colv[GJR_MAXN], rowa[GJR_MAXN], rowb[GJR_MAXN];
+  __shared__ int sw[GJR_MAXN], dest[GJR_MAXN];
+  __shared__ int s_piv;
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  double *Gm = Am + (long)m * N * N;
+  const int tid = threadIdx.x, r = tid % N, g = tid / N;
+  double a[CPT];
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int q = g + 4 * i;
+    a[i] = (q < N) ? Gm[(long)q * N + r] : 0.0;
+  }
+  for (int c = 0; c < N; ++c) {
+    const int gc = c & 3, ic = c >> 2;
+    if (g == gc) {
+#pragma unroll
+      for (int i = 0; i < CPT; ++i)
+        if (i == ic) colv[r] = a[i];
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = N;
+      for (int rr = c + tid; rr < N; rr += 64) {
+        const double v = fabs(colv[rr]);
+        if (v > best) { best = v; bi = rr; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (tid == 0) {
+        if (bi >= N) bi = c;
+        s_piv = bi;
+        sw[c] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (r == c) {
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const int q = g + 4 * i;
+        if (q < N) rowa[q] = a[i];
+      }
+    }
+    if (r == p) {           // also when p == c: rowb is the pivot row
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const int q = g + 4 * i;
+        if (q < N) rowb[q] = a[i];
+      }
+    }
+    __syncthreads();
+    // after the swap, row c holds the old row p (rowb) and row p the old row c
+    if (p != c && (r == c || r == p)) {
+      const double *src = (r == c) ? rowb : rowa;
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const int q = g + 4 * i;
+        if (q < N) a[i] = src[q];
+      }
+    }
+    const double inv = 1.0 / rowb[c];
+    const double fr = colv[(r == c) ? p : (r == p ? c : r)];
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int q = g + 4 * i;
+      if (q >= N) continue;
+      if (q == c) {
+        a[i] = (r == c) ? inv : -fr * inv;
+      } else {
+        const double pr = rowb[q] * inv;
+        a[i] = (r == c) ? pr : a[i] - fr * pr;
+      }
+    }
+    __syncthreads();
+  }
+  // undo the row swaps as column swaps, last step first: position j ends up
+  // holding column colat[j]; column q goes to dest[q]
+  if (tid == 0) {
+    int *colat = dest;        // reused: built in place, then inverted below
+    for (int j = 0; j < N; ++j) colat[j] = j;
+    for (int c = N - 1; c >= 0; --c) {
+      const int s2 = sw[c];
+      const int t = colat[c];
+      colat[c] = colat[s2];
+      colat[s2] = t;
+    }
+    for (int j = 0; j < N; ++j) sw[colat[j]] = j;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int q = g + 4 * i;
+    if (q < N) Gm[(long)sw[q] * N + r] = a[i];
+  }
 }
 
 // Gauss-Jordan inversion, row-owner form: thread (r = tid % N, g = tid / N)
@@ -1102,6 +1227,22 @@ extern "C" int smx_invert(int prec, int nmat, int N, void *A, int per_run, const
                           void *stream) {
   if (nmat <= 0) return 0;
   if (N <= 0 || per_run <= 0) return fail_msg("smx_invert: bad sizes");
+  if (prec == 0 && N >= 16 && N <= GJR_MAXN && g_gj_reg) {
+    // register-resident kernel: 4N threads, ceil(N / 4) columns per thread
+    const int cpt = (N + 3) / 4;
+    const dim3 grid(nmat), blk(4 * N);
+    hipStream_t st = STREAM(stream);
+    if (cpt <= 6)
+      hipLaunchKernelGGL(k_gj_invert_reg<6>, grid, blk, 0, st, (double *)A, N, per_run, active);
+    else if (cpt <= 12)
+      hipLaunchKernelGGL(k_gj_invert_reg<12>, grid, blk, 0, st, (double *)A, N, per_run, active);
+    else if (cpt <= 24)
+      hipLaunchKernelGGL(k_gj_invert_reg<24>, grid, blk, 0, st, (double *)A, N, per_run, active);
+    else
+      hipLaunchKernelGGL(k_gj_invert_reg<32>, grid, blk, 0, st, (double *)A, N, per_run, active);
+    CHECK_LAUNCH("smx_invert");
+    return 0;
+  }
   if (N >= 8 && N <= 512 && g_gj_rows) {
     // row-owner kernel: N * G threads, G = min(N, 512 / N)
     const int G = max(1, min(N, 512 / N));

@@ -22,6 +22,8 @@ Layouts (row-major, run-major)
     Krylov bases         [B][m+1][n], [B][m][n]
     LU factors           [B][Nk][Nz*Nz] column-major, pivots [B][Nk][Nz]
 """
+import threading
+
 import numpy as np
 import torch
 
@@ -63,6 +65,12 @@ LDS_BYTES = 160 * 1024      # per workgroup on gfx950
 def inverse_fits_lds(N, esize):
     """Whether the Gauss-Jordan inversion of an N x N block fits in LDS."""
     return N * N * esize + 2 * N * esize + 4 * N <= LDS_BYTES
+
+
+# HIP-graph captures are taken one at a time in the process: torch registers
+# generator state when a capture begins, which it refuses while another
+# thread's capture is open.  Launches and replays from other threads go on.
+_CAPTURE_LOCK = threading.Lock()
 
 
 class Solver:
@@ -146,9 +154,10 @@ class Solver:
                 self._round(precond, op)
                 rounds += 1
                 graph = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(graph, capture_error_mode="thread_local"):
-                    for _ in range(G):
-                        self._round(precond, op)
+                with _CAPTURE_LOCK:
+                    with torch.cuda.graph(graph, capture_error_mode="thread_local"):
+                        for _ in range(G):
+                            self._round(precond, op)
                 self._graphs[key] = graph
         # Round (or replay) r is enqueued before the host waits for the
         # count of r-1, so the device never idles on the host's check; a run
@@ -781,10 +790,10 @@ class Engine:
         # truncation to Nz Chebyshev modes, then D2 t0, D2 t1, D t3
         self.gemm(B * Nk * 4, N, Q, W.tz, Q, 1, t["Tp"], 1, Q, W.tt, N, 1,
                   active=active, mask_mode=1, rpr=Nk * 4)
+        # one GEMM over all B*Nk mode rows per field (rows stride 4N in, 3N out)
         for f_in, f_out, Mat in ((0, 0, "D2"), (1, 1, "D2"), (3, 2, "D")):
-            self.gemm(Nk, N, N, W.tt, 4 * N, 1, t[Mat], 1, N, W.dtt, 3 * N, 1,
-                      nz1=B, sa=(Nk * 4 * N, 0), sc=(Nk * 3 * N, 0), aoff=f_in * N,
-                      coff=f_out * N, active=active, mask_mode=2)
+            self.gemm(B * Nk, N, N, W.tt, 4 * N, 1, t[Mat], 1, N, W.dtt, 3 * N, 1,
+                      aoff=f_in * N, coff=f_out * N, active=active, mask_mode=1, rpr=Nk)
         L.stokes_combine(self.prec, B, Nk, N, self.p(W.tt), self.p(W.dtt), self.p(t["kx"]),
                          self.p(v), self.p(t["V0"]), self.p(t["V1"]), self.p(out),
                          self.p(active), s)

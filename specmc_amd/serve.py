@@ -19,9 +19,10 @@ Time-to-solution of a request is its finish time minus its arrival time
 (queueing included); ``Server.stats()`` reports its p50 / p99 and the
 aggregate run-steps/s.
 """
+import queue
+import threading
 import time
 from collections import OrderedDict, deque
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -90,7 +91,13 @@ class Bucket:
         self.live = np.zeros(self.S, dtype=bool)
         self.steps = 0
         self.last_used = 0.0
-        self.stream = None          # own HIP stream when buckets step concurrently
+        # concurrent serving: the bucket's own stream, worker thread and inbox
+        self.stream = None
+        self.thread = None
+        self.inbox = deque()
+        self.stop = False
+        self.error = None
+        self.assigned = 0           # routed here and not yet collected (main thread only)
 
     def free_slots(self):
         return [i for i in range(self.S) if self.req[i] is None]
@@ -146,6 +153,10 @@ class Bucket:
     def busy(self):
         return any(r is not None for r in self.req)
 
+    def idle(self):
+        """Nothing routed here is still unfinished."""
+        return self.assigned == 0 and not self.busy()
+
 
 class Server:
     """Admission, stepping and retirement of requests on one GPU.
@@ -154,12 +165,15 @@ class Server:
     ``mem_budget`` HBM bytes the buckets may hold (default: 80% of the free
                   device memory at construction, less the rescue headroom);
     ``precision`` "fp64" or "mixed" (see ``engine.PRECISIONS``);
-    ``concurrent`` step the buckets of a round in worker threads, each bucket
-                  on its own HIP stream, so the device overlaps them (the
-                  round is joined before admission and retirement)."""
+    ``concurrent`` every bucket steps in a worker thread of its own, on its
+                  own HIP stream, with no barrier between buckets: a bucket
+                  of small runs is not held back by one of large runs.  The
+                  worker admits requests from its inbox into free slots
+                  between its steps; the main thread routes arrivals,
+                  creates and releases buckets and collects retirements."""
 
     def __init__(self, slots=64, mem_budget=None, precision="fp64", safety=None, dtmax=None,
-                 diagnostics=True, concurrent=False, workers=8, **engine_kw):
+                 diagnostics=True, concurrent=False, **engine_kw):
         from .engine import RESCUE_BYTES
         self.slots = int(slots)
         if mem_budget is None:
@@ -179,7 +193,8 @@ class Server:
         self.t0 = None
         self.t_end = None
         self.concurrent = bool(concurrent)
-        self._pool = ThreadPoolExecutor(max_workers=int(workers)) if concurrent else None
+        self._finished = queue.Queue()
+        self._steps_lock = threading.Lock()
 
     # ---------------------------------------------------------------- input
     def submit(self, requests):
@@ -199,15 +214,16 @@ class Server:
         key = self.key_of(r.cfg)
         b = self.buckets.get(key)
         if b is not None:
+            if self.concurrent:
+                return b            # its worker admits from the inbox as slots free
             return b if b.free_slots() else None
         per = run_bytes(key[0], key[1], key[2], self.precision)
         room = self.mem_budget - self.used_bytes()
         if room < per:
             # release idle buckets of other shapes, oldest first
-            for k in [k for k, bb in self.buckets.items() if not bb.busy()]:
-                room += self.bucket_bytes.pop(k)
-                del self.buckets[k]
-                torch.cuda.empty_cache()
+            for k in [k for k, bb in self.buckets.items() if bb.idle()]:
+                self._release(k)
+                room = self.mem_budget - self.used_bytes()
                 if room >= self.slots * per:
                     break
         S = min(self.slots, room // per)
@@ -216,7 +232,41 @@ class Server:
         b = Bucket(key, r.cfg, S, precision=self.precision, **self.engine_kw)
         self.buckets[key] = b
         self.bucket_bytes[key] = S * per
+        if self.concurrent:
+            b.stream = torch.cuda.Stream()
+            b.stream.wait_stream(torch.cuda.default_stream())
+            b.thread = threading.Thread(target=self._worker, args=(b,), daemon=True)
+            b.thread.start()
         return b
+
+    def _release(self, key):
+        b = self.buckets.pop(key)
+        self.bucket_bytes.pop(key)
+        if b.thread is not None:
+            b.stop = True
+            b.thread.join()
+        del b
+        torch.cuda.empty_cache()
+
+    def _worker(self, b):
+        """Bucket loop of concurrent serving: admit from the inbox, step the
+        live slots, retire, until told to stop."""
+        try:
+            with torch.cuda.stream(b.stream):
+                while not b.stop:
+                    while b.inbox and b.free_slots():
+                        b.admit(b.free_slots()[0], b.inbox.popleft(), self.now())
+                    if not np.any(b.live) and not b.busy():
+                        time.sleep(0.001)
+                        continue
+                    n = b.step(self.safety, self.dtmax)
+                    fin = b.retire(self.now(), self.diagnostics)
+                    with self._steps_lock:
+                        self.run_steps += n
+                    for r in fin:
+                        self._finished.put((b, r))
+        except BaseException as exc:          # surfaced by the main loop
+            b.error = exc
 
     def _admit(self, now):
         """FIFO over the arrived requests; a request that cannot be placed
@@ -228,7 +278,11 @@ class Server:
             if b is None:
                 waiting.append(r)
                 continue
-            b.admit(b.free_slots()[0], r, now)
+            b.assigned += 1
+            if self.concurrent:
+                b.inbox.append(r)
+            else:
+                b.admit(b.free_slots()[0], r, now)
         while waiting:
             self.pending.appendleft(waiting.pop())
 
@@ -243,7 +297,7 @@ class Server:
         if self.t0 is None:
             self.t0 = time.perf_counter()
         next_report = self.now() + every
-        while self.pending or any(b.busy() for b in self.buckets.values()):
+        while self.pending or any(b.assigned > 0 for b in self.buckets.values()):
             if progress is not None and self.now() >= next_report:
                 next_report = self.now() + every
                 progress(dict(t=self.now(), done=len(self.done), pending=len(self.pending),
@@ -253,15 +307,16 @@ class Server:
             if max_wall is not None and now > max_wall:
                 break
             self._admit(now)
+            if self.concurrent:
+                self._collect(on_retire, now)
+                continue
             stepped = 0
             busy = [b for b in self.buckets.values() if b.busy()]
-            if self.concurrent and len(busy) > 1:
-                results = [f.result() for f in [self._pool.submit(self._round, b) for b in busy]]
-            else:
-                results = [(b, b.step(self.safety, self.dtmax),
-                            b.retire(self.now(), self.diagnostics)) for b in busy]
+            results = [(b, b.step(self.safety, self.dtmax),
+                        b.retire(self.now(), self.diagnostics)) for b in busy]
             for b, n, fin in results:
                 stepped += n
+                b.assigned -= len(fin)
                 if fin:
                     b.last_used = self.now()
                     self.done.extend(fin)
@@ -270,28 +325,48 @@ class Server:
                             on_retire(r)
             self.run_steps += stepped
             if stepped == 0 and self.pending:
-                arrived = [r for r in self.pending if r.arrival <= self.now()]
+                arrived = [r for r in self.pending if r.arrival <= now]
                 if not arrived:       # idle until the next arrival
                     time.sleep(max(0.0, min(0.05, self.pending[0].arrival - self.now())))
                 elif not any(b.busy() for b in self.buckets.values()):
                     raise RuntimeError("request %r does not fit the memory budget (%d bytes)"
                                        % (arrived[0].rid, self.mem_budget))
+        if self.concurrent:
+            for b in self.buckets.values():
+                b.stop = True
+            for b in self.buckets.values():
+                if b.thread is not None:
+                    b.thread.join()
+            self._collect(on_retire, None, wait=False)
         torch.cuda.synchronize()
         self.t_end = self.now()
         return self.done
 
-    def _round(self, b):
-        """One step and the retirements of bucket ``b`` on its own stream
-        (worker thread); the stream first waits for the default stream's
-        work (admissions), and is drained before the round joins."""
-        if b.stream is None:
-            b.stream = torch.cuda.Stream()
-        b.stream.wait_stream(torch.cuda.default_stream())
-        with torch.cuda.stream(b.stream):
-            n = b.step(self.safety, self.dtmax)
-            fin = b.retire(self.now(), self.diagnostics)
-        b.stream.synchronize()
-        return b, n, fin
+    def _collect(self, on_retire, now, wait=True):
+        """Concurrent serving: gather retirements, surface worker errors,
+        and wait briefly when nothing happened.  ``now`` is the time of the
+        admission pass just made (a request arrived later is not late)."""
+        got = False
+        while True:
+            try:
+                b, r = self._finished.get(timeout=0.005 if (wait and not got) else 0.0)
+            except queue.Empty:
+                break
+            got = True
+            b.assigned -= 1
+            b.last_used = self.now()
+            self.done.append(r)
+            if on_retire is not None:
+                on_retire(r)
+        for b in self.buckets.values():
+            if b.error is not None:
+                raise RuntimeError("bucket %r failed" % (b.key,)) from b.error
+        if now is not None and not got and self.pending and not any(
+                b.assigned > 0 for b in self.buckets.values()):
+            arrived = [r for r in self.pending if r.arrival <= now]
+            if arrived:
+                raise RuntimeError("request %r does not fit the memory budget (%d bytes)"
+                                   % (arrived[0].rid, self.mem_budget))
 
     def stats(self):
         lat = np.array([r.latency for r in self.done if r.latency is not None])

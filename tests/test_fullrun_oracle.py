@@ -1,0 +1,89 @@
+"""The whole 64x96 eta_T run (552 steps to t = 5e-3, through the stagnating
+Stokes solves and their dense rescue) against the oracle's march of the same
+run (tests/golden/oracle_c1_full_64x96.npz, written by
+tests/golden/make_oracle_c1.py).
+
+* CPU: the device trajectory recorded on MI355X by ``bench.py --make-states``
+  (bench_states_c1_64x96.npz) -- step count, per-step Stokes iteration counts
+  and routes, final theta and diagnostics.
+* GPU: the whole run marched again, FP64 (1e-10 relative on theta, Nu, Vrms)
+  and mixed precision (1e-6 on Nu and Vrms, the mixed bar).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from golden_io import GOLDEN, rel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(GOLDEN, "oracle_c1_full_64x96.npz")
+STATES = os.path.join(ROOT, "bench_states_c1_64x96.npz")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(ORACLE),
+                                reason="oracle full-run fixture not generated")
+
+
+def _oracle():
+    with np.load(ORACLE, allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    d["diag"] = json.loads(str(d["diag"]))
+    d["info"] = json.loads(str(d["info"]))
+    return d
+
+
+def test_recorded_device_trajectory_matches_oracle():
+    o = _oracle()
+    with np.load(STATES, allow_pickle=False) as z:
+        dev_log, dev_final = z["log"], z["final"]
+        dev_diag = json.loads(str(z["diag"]))
+    olog = o["log"]
+    assert len(dev_log) == len(olog) == o["info"]["steps"]
+    # per-step Stokes iterations and rescue routes
+    assert np.array_equal(dev_log[:, 2].astype(int), olog[:, 2].astype(int))
+    assert np.array_equal(dev_log[:, 4] > 0, olog[:, 3] > 0)
+    np.testing.assert_allclose(dev_log[:, 1], olog[:, 1], rtol=1e-10, atol=0)
+    err = rel(np.real(dev_final), np.real(o["final"]))
+    print("recorded device run vs oracle: theta rel %.2e" % err)
+    assert err < 1e-10
+    for k in ("Nu_top", "Nu_energy", "urms"):
+        assert abs(dev_diag[k] - o["diag"][k]) <= 1e-10 * abs(o["diag"][k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_device_full_run_matches_oracle(precision, tol):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("needs a HIP device")
+    from specmc_amd import ConvectionConfig, api
+    from specmc_amd.core import bench_ic
+    o = _oracle()
+    cfg = ConvectionConfig(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64, Nz=96,
+                           Lx=2.0, precond="bfbt")
+    m = api.build_model(cfg, precision=precision)
+    th0 = bench_ic.conductive_theta(m, amp=3e-2)
+    its = []
+
+    def cb(n, t, s):
+        its.append(m.last_bfbt_iters)
+        if n % 100 == 0:
+            print("%s step %d t=%.5f" % (precision, n, t), flush=True)
+
+    s, info = api.march_adaptive(m, th0, 5e-3, on_step=cb)
+    d = m.diagnostics(s)
+    eth = rel(s, np.real(o["final"]))
+    errs = {k: abs(d[k] - o["diag"][k]) / abs(o["diag"][k]) for k in ("Nu_top", "Nu_energy", "urms")}
+    print("%s full run: %d steps, theta rel %.2e, %s" % (precision, info["steps"], eth, errs),
+          flush=True)
+    if precision == "fp64":
+        assert info["steps"] == o["info"]["steps"]
+        assert its == [int(v) for v in o["log"][:, 2]]
+        assert eth < tol
+    else:
+        # dt follows the velocity, which differs at roundoff; the last step can
+        # land on either side of tstop
+        assert abs(info["steps"] - o["info"]["steps"]) <= 1
+    for k, e in errs.items():
+        assert e < tol, (k, e)

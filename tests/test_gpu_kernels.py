@@ -63,26 +63,36 @@ def test_xprod_mfma_matches_vector_kernel_and_numpy(lib, shape):
     assert e_mfma < 1e-13 and e_vec < 1e-13
 
 
-@pytest.mark.parametrize("N", [24, 48, 96])
-def test_gj_rows_inverse(lib, N):
+@pytest.mark.parametrize("N", [16, 24, 48, 96, 128])
+def test_gj_inverse_variants(lib, N):
+    """The register-resident, row-owner and first Gauss-Jordan kernels give
+    the same inverse (the same arithmetic per entry) and match numpy."""
     rng = np.random.default_rng(N)
     nm = 7
     M = rng.normal(size=(nm, N, N)) + np.sqrt(N) * np.eye(N)
+    M[0] = rng.normal(size=(N, N))           # pivoting on every column
     dev = torch.device("cuda")
     s = torch.cuda.current_stream().cuda_stream
-    got = []
-    for on in (1, 0):
-        lib.set_gj_rows(on)
-        A = torch.tensor(np.ascontiguousarray(M.transpose(0, 2, 1)), device=dev)
-        lib.invert(0, nm, N, A.data_ptr(), 1, None, s)
-        torch.cuda.synchronize()
-        got.append(A.cpu().numpy().transpose(0, 2, 1))
-    lib.set_gj_rows(0)
+    got = {}
+    try:
+        for name, reg, rows in (("reg", 1, 0), ("rows", 0, 1), ("first", 0, 0)):
+            lib.set_gj_reg(reg)
+            lib.set_gj_rows(rows)
+            A = torch.tensor(np.ascontiguousarray(M.transpose(0, 2, 1)), device=dev)
+            lib.invert(0, nm, N, A.data_ptr(), 1, None, s)
+            torch.cuda.synchronize()
+            got[name] = A.cpu().numpy().transpose(0, 2, 1)
+    finally:
+        lib.set_gj_reg(1)
+        lib.set_gj_rows(0)
     want = np.linalg.inv(M)
-    print("gj N=%d: rows %.2e, first %.2e, rows vs first %.2e" % (
-        N, rel(got[0], want), rel(got[1], want), rel(got[0], got[1])), flush=True)
-    assert rel(got[0], want) < 1e-12
-    assert rel(got[0], got[1]) < 1e-13
+    print("gj N=%d: reg %.2e rows %.2e first %.2e; reg vs first %.2e" % (
+        N, rel(got["reg"], want), rel(got["rows"], want), rel(got["first"], want),
+        rel(got["reg"], got["first"])), flush=True)
+    for k in got:
+        assert rel(got[k], want) < 1e-11, k
+    assert rel(got["reg"], got["first"]) < 1e-14
+    assert rel(got["rows"], got["first"]) < 1e-14
 
 
 @pytest.mark.parametrize("N", [24, 48, 96, 192])
