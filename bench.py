@@ -1,13 +1,14 @@
 """Benchmark: batched steps/s at 64x96 eta_T (deta_T = 1e3) vs the plain FP64
-single-run device stepper.
+single-run device stepper, with p50/p99 run latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-    python bench.py --make-states        # record the config-1 trajectory once
+    python bench.py --workload c2|c3|c4      # the other scenarios
+    python bench.py --make-states            # record the 64x96 trajectory once
 
-Workload (config 1): eta_T viscosity, deta_T = 1e3, Ra = 1e4, Nx = 64, Nz = 96,
-Lx = 2 (the unit box on its mirror-doubled carrier), BFBT Stokes route, theta0
-= conductive + 3e-2 cos(pi x) sin(pi z), marched to tstop = 5e-3.  A "step" is
-one run-step (CFL dt + coupled Stokes solve + BDF2 theta solve).
+Workload (the headline): eta_T viscosity, deta_T = 1e3, Ra = 1e4, Nx = 64,
+Nz = 96, Lx = 2 (the unit box on its mirror-doubled carrier), BFBT Stokes
+route, theta0 = conductive + 3e-2 cos(pi x) sin(pi z), marched to tstop = 5e-3.
+A "step" is one run-step (CFL dt + coupled Stokes solve + BDF2 theta solve).
 
 The cost of a step changes by ~50x along that run (the Stokes FGMRES needs ~10
 iterations at first, then hits its 800-iteration budget and goes through the
@@ -16,12 +17,17 @@ plain stepper through the whole run and records phase checkpoints and the
 per-step log in ``bench_states_c1_64x96.npz``.  The default run times, from
 each phase checkpoint, ``--steps`` steps of
 
-* plain   -- one run on the FP64 device stepper, unbatched;
+* plain   -- one run on the FP64 device stepper, unbatched, eager launches;
 * batched -- B runs (the checkpoint with seeded 1e-3 relative perturbations)
-  advancing together, each with its own dt,
+  advancing together in FP64, each with its own dt;
+* mixed   -- the same B runs in mixed precision (FP32 dense rescue factor
+  under FP64 FGMRES refinement) with graph-replayed FGMRES rounds: ``value``;
+  its errors against the batched FP64 leg (same inputs) are reported;
+* served  -- a Poisson stream of B requests resumed from the checkpoints,
+  served with continuous admission and retirement: p50/p99.
 
 and weights the phases by their share of the recorded run, giving the
-whole-run aggregate rates.  ``value`` is the batched aggregate run-steps/s.
+whole-run aggregate rates.
 """
 import argparse
 import json
@@ -197,9 +203,20 @@ def run_c2(precisions=("fp64", "mixed"), max_wall=None):
     and reported as such, past ``max_wall`` seconds per precision)."""
     from specmc_amd import api
     from specmc_amd.workloads import c2_requests, C2_LOG_TIMES
+    import threading
+    import torch
     r = c2_requests()[0]
-    out = {}
-    for prec in precisions:
+    out, errors = {}, []
+
+    def one(prec):
+        try:
+            with torch.cuda.stream(torch.cuda.Stream()):
+                march_one(prec)
+                torch.cuda.current_stream().synchronize()
+        except BaseException as exc:
+            errors.append(exc)
+
+    def march_one(prec):
         m = api.build_model(r.cfg, precision=prec)
         logs, marks = [], list(C2_LOG_TIMES)
         t0 = time.perf_counter()
@@ -229,7 +246,16 @@ def run_c2(precisions=("fp64", "mixed"), max_wall=None):
         out[prec] = dict(steps=info["steps"], t=info["t"], complete=complete, wall_s=wall,
                          steps_per_s=info["steps"] / wall,
                          log=logs, rescues=dsc.get("n_bfbt_rescue"),
-                         rescues_fp32=dsc.get("n_rescue_fp32"), theta=np.real(s))
+                         rescues_fp32=dsc.get("n_rescue_fp32"),
+                         rescues_fp32_redone=dsc.get("n_rescue_fp32_redo"), theta=np.real(s))
+    # the runs of the two precisions march concurrently, each on its own stream
+    threads = [threading.Thread(target=one, args=(p,)) for p in precisions]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
     res = dict(metric="heated 2:1 run (64x96, Ra=1.25e4, H=1.5, eta_T 1e3) to t=0.03",
                unit="steps/s", config=dict(workload="c2", Nx=64, Nz=96, Lx=4.0))
     for prec in precisions:
@@ -378,7 +404,8 @@ def main():
             n3, t3 = _timed(ens, a.steps, a.warmup)
             r.update(mixed=n3 / t3, ms_mixed_step=1e3 * t3 / a.steps,
                      rescues_fp32=int(ens.eng.n_rescue_fp32.sum()),
-                     rescues_fp32_redone=int(ens.eng.n_rescue_fp32_redo.sum()))
+                     rescues_fp32_redone=int(ens.eng.n_rescue_fp32_redo.sum()),
+                     rescue_fgmres_iters=int(ens.eng.rescue_iters.sum()))
             if ref is not None:
                 r["mixed_vs_fp64"] = _diag_errors(ens, *ref)
             del ens

@@ -45,6 +45,9 @@ PRECISIONS = ("fp64", "mixed")
 RESCUE_BYTES = 24 << 30    # HBM for dense rescue matrices factored together
 # FGMRES of the mixed-precision rescue (FP32 dense LU as the preconditioner)
 RESCUE_RESTART, RESCUE_MAXITER, RESCUE_RTOL = 20, 60, 1e-12
+# after an FP32-factored rescue of a run misses the acceptance test, that
+# run's next rescues use the FP64 factor directly this many times
+RESCUE_FP32_BACKOFF = 8
 DENSE_FP64_MAXN = 8192     # largest n of the FP64 dense factor (FP32: 32768)
 ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
@@ -418,7 +421,8 @@ class Engine:
             for a in (self.memo_hits, self.bfbt_iters, self.last_bfbt_iters,
                       self.last_bfbt_info, self.n_rescue, self.n_rescue_direct,
                       self.n_rescue_weak, self.n_defect_failed, self.n_defect_double_fail,
-                      self.n_rescue_fp32, self.n_rescue_fp32_redo, self.rescue_iters):
+                      self.n_rescue_fp32, self.n_rescue_fp32_redo, self.rescue_iters,
+                      self.fp32_backoff):
                 a[i] = 0
             self.last_bfbt_res[i] = 0.0
             self.last_defect_res[i] = 0.0
@@ -741,6 +745,7 @@ class Engine:
         self.n_rescue_fp32 = np.zeros(B, dtype=np.int64)
         self.n_rescue_fp32_redo = np.zeros(B, dtype=np.int64)
         self.rescue_iters = np.zeros(B, dtype=np.int64)
+        self.fp32_backoff = np.zeros(B, dtype=np.int64)
         self.last_defect_res = np.zeros(B)
         self.last_route = ["bfbt"] * B
         self.defect_tol = defect_tol
@@ -877,10 +882,19 @@ class Engine:
         weak acceptance.  ``rprec`` 1 factors in FP32 (mixed precision, see
         ``_rescue_mixed``)."""
         sv, n, B = self.ssol, self.n, self.B
+        bad = [int(r) for r in idx]
+        if rprec is None and self.rescue_prec == 1 and n <= DENSE_FP64_MAXN:
+            # runs whose last FP32 attempt failed take the FP64 factor for a while
+            back = [r for r in bad if self.fp32_backoff[r] > 0]
+            if back:
+                self.fp32_backoff[back] -= 1
+                self._rescue(theta, back, rprec=0)
+                bad = [r for r in bad if r not in set(back)]
+                if not bad:
+                    return
         rprec = self.rescue_prec if rprec is None else int(rprec)
         if n > DENSE_FP64_MAXN:
             rprec = 1          # the FP64 factor does not go this large
-        bad = [int(r) for r in idx]
         # dense matrices of at most RESCUE_BYTES at a time
         cap = max(1, int(RESCUE_BYTES // (n * n * (4 if rprec else 8))))
         if len(bad) > cap:
@@ -1018,6 +1032,7 @@ class Engine:
             sv.x[g] = rs.x[g]
         elif redo:
             self.n_rescue_fp32_redo[redo] += 1
+            self.fp32_backoff[redo] = RESCUE_FP32_BACKOFF
             self._rescue(theta, redo, rprec=0)
 
     def stokes(self, theta, active=None):
