@@ -162,9 +162,11 @@ int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, int *piv,
 /* x_q = A_q^{-1} b_q from smx_dense_lu factors; b and x are [nmat][n]
  * (sla.lu_solve of the defect correction, coupled_model.py:544-554).
  * prec 0: FP64 factor and vectors; 1: FP32 factor and vectors; 2: FP32
- * factor, FP64 b and x (the preconditioner of the mixed-precision rescue). */
+ * factor, FP64 b and x (the preconditioner of the mixed-precision rescue).
+ * active: optional int[nmat]; matrices whose flag is 0 leave x untouched. */
 int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
-                       const int *piv, const void *b, void *x, void *stream);
+                       const int *piv, const void *b, void *x, const int *active,
+                       void *stream);
 
 /* Chebyshev-tau operators, column-major, per (run, mode):
  *   kind 0: D2 - (k^2 + lam_b) I, rows N-2, N-1 = (-1)^n, 1
@@ -249,6 +251,16 @@ int smx_advect_prod(int prec, int B, int Nk, int nrow, int N,
                     void *out, const void *So, const void *Ce,
                     const void *Ae, const int *active, void *stream);
 
+/* out[b] = max over the run's n_per_run grid values of
+ * |eta - eta_lag| / max(|eta_lag|, 1e-300): the pointwise relative drift of
+ * the viscosity since the frozen factorisation (RBCCoupled._eta_drift,
+ * coupled_model.py:509-522). */
+int smx_eta_drift(int prec, int B, long n_per_run, const void *eta, const void *eta_lag,
+                  double *out, const int *active, void *stream);
+/* out[b] = ||x[b, 0:n]||_2 for rows of stride ld (the np.linalg.norm of the
+ * Picard loop in RBCCoupled._defect_solve, coupled_model.py:543-566). */
+int smx_rownorm(int prec, int B, long n, long ld, const void *x, double *out,
+                const int *active, void *stream);
 /* Elementwise / small kernels of the step.
  * axpby: out = (a*ab[b]) x + (b*bb[b]) y per run (ab, bb, y may be NULL).
  * rowscale: out[b][k][j] = (alpha*ab[b]) ks[k] x[b][k][j], tau entries zeroed

@@ -3,15 +3,13 @@
 ``build_model(cfg)`` returns the device-backed class of ``cfg.viscosity_model``
 and checks it, ``march`` / ``march_adaptive`` run the reference's loops
 unchanged (api.py:147-190), and the introspection helpers keep their
-meaning.  ``precond='legacy'`` (the lagged-LU route) is served by the BFBT
-FGMRES route on the device: both solve the same coupled Schur system to the
-same relative tolerance, only the Krylov path differs; a ``UserWarning`` says
-so once per process.
+meaning.  Both Stokes routes of the coupled models run on the device:
+``precond='bfbt'`` and ``precond='legacy'`` (the lagged-LU route, the
+ConvectionConfig default).
 """
 from __future__ import annotations
 
 import os
-import warnings
 
 import numpy as np
 
@@ -26,7 +24,6 @@ __all__ = ["MODEL_CLASSES", "numerics", "model_spec", "model_kwargs",
 
 _BUILDERS = {"const": "RBC", "eta_z": "RBCImplicitTheta", "eta_T": "RBCCoupled",
              "arrhenius": "RBCCoupled"}
-_warned_legacy = [False]
 
 
 def _classes():
@@ -96,11 +93,6 @@ def build_model(cfg: ConvectionConfig, precision="fp64"):
     kw = dict(spec["kwargs"])
     if precision != "fp64":
         kw["precision"] = precision
-    if kw.get("precond") == "legacy" and not _warned_legacy[0]:
-        _warned_legacy[0] = True
-        warnings.warn("precond='legacy' runs the BFBT-preconditioned FGMRES route on "
-                      "the device (same coupled Schur system and tolerance; the "
-                      "lagged-LU Krylov path is not carried)", UserWarning, stacklevel=2)
     model = spec["cls"](**kw)
     if type(model) is not spec["cls"]:
         raise AssertionError("internal mapping error: viscosity_model=%r built %s, "

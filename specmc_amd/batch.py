@@ -18,12 +18,14 @@ _KIND = {"const": KIND_CONST, "eta_z": KIND_DEPTH, "eta_T": KIND_COUPLED,
 
 
 def run_params(cfg):
-    """Per-run physics of a ConvectionConfig as the engine's parameter dict."""
+    """Per-run physics and Stokes route of a ConvectionConfig as the engine's
+    parameter dict."""
     return dict(Ra=float(cfg.Ra), deta_T=float(cfg.deta_T), Ts=float(cfg.arrhenius_Ts),
                 eta_min=float(cfg.eta_min), eta_max=float(cfg.eta_max),
                 eta_smooth=float(cfg.eta_smooth),
                 internal_heating=float(cfg.internal_heating),
-                dissipation_number=float(cfg.dissipation_number))
+                dissipation_number=float(cfg.dissipation_number),
+                precond=str(cfg.precond).strip().lower(), lag=int(cfg.lag))
 
 
 class Ensemble:

@@ -342,11 +342,13 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
 // ---------------------------------------------------------------------------
 template <typename T, typename TV>
 __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
-                                                    const int *piv, const TV *b, TV *x) {
+                                                    const int *piv, const TV *b, TV *x,
+                                                    const int *active) {
   extern __shared__ unsigned char smem_raw[];
   T *y = (T *)smem_raw;
   T *Db = y + n;                            // [64][65]
   const int q = blockIdx.x;
+  if (active && !active[q]) return;
   const T *M = A + (long)q * ms;
   const int *P = piv + (long)q * n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -435,7 +437,8 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
 }
 
 extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
-                                  const int *piv, const void *b, void *x, void *stream) {
+                                  const int *piv, const void *b, void *x, const int *active,
+                                  void *stream) {
   if (nmat <= 0) return 0;
   if (prec < 0 || prec > 2)
     return fail_msg("smx_dense_lu_solve: prec must be 0 (FP64), 1 (FP32) or 2 (FP32 factor, "
@@ -450,17 +453,20 @@ extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, lon
     if (!lds_optin(k_lud_solve<double, double>, sh))
       return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
     hipLaunchKernelGGL((k_lud_solve<double, double>), dim3(nmat), dim3(1024), sh, st,
-                       (const double *)LU, mstride, n, piv, (const double *)b, (double *)x);
+                       (const double *)LU, mstride, n, piv, (const double *)b, (double *)x,
+                       active);
   } else if (prec == 1) {
     if (!lds_optin(k_lud_solve<float, float>, sh))
       return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
     hipLaunchKernelGGL((k_lud_solve<float, float>), dim3(nmat), dim3(1024), sh, st,
-                       (const float *)LU, mstride, n, piv, (const float *)b, (float *)x);
+                       (const float *)LU, mstride, n, piv, (const float *)b, (float *)x,
+                       active);
   } else {
     if (!lds_optin(k_lud_solve<float, double>, sh))
       return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
     hipLaunchKernelGGL((k_lud_solve<float, double>), dim3(nmat), dim3(1024), sh, st,
-                       (const float *)LU, mstride, n, piv, (const double *)b, (double *)x);
+                       (const float *)LU, mstride, n, piv, (const double *)b, (double *)x,
+                       active);
   }
   CHECK_LAUNCH("smx_dense_lu_solve");
   return 0;

@@ -2244,6 +2244,68 @@ extern "C" int smx_maxabs(int prec, int B, long n_per_run, const void *x, double
   return 0;
 }
 
+// per-run max over the field of |eta - eta_lag| / max(|eta_lag|, 1e-300): the
+// pointwise relative drift of the viscosity since a frozen factorisation
+// (coupled_model.py:509-522)
+template <typename T>
+__global__ void __launch_bounds__(1024) k_eta_drift(long n, const T *eta, const T *lag,
+                                                    double *out, const int *active) {
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  __shared__ double red[16];
+  double m = 0.0;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+    const double l = (double)lag[(long)b * n + e];
+    const double d = fabs((double)eta[(long)b * n + e] - l) / fmax(fabs(l), 1e-300);
+    m = fmax(m, d);
+  }
+  m = block_max(m, red);
+  if (threadIdx.x == 0) out[b] = m;
+}
+extern "C" int smx_eta_drift(int prec, int B, long n_per_run, const void *eta,
+                             const void *eta_lag, double *out, const int *active,
+                             void *stream) {
+  if (B <= 0) return 0;
+  if (n_per_run <= 0) return fail_msg("smx_eta_drift: n_per_run must be positive");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_eta_drift<double>, dim3(B), dim3(1024), 0, STREAM(stream), n_per_run,
+                       (const double *)eta, (const double *)eta_lag, out, active);
+  else
+    hipLaunchKernelGGL(k_eta_drift<float>, dim3(B), dim3(1024), 0, STREAM(stream), n_per_run,
+                       (const float *)eta, (const float *)eta_lag, out, active);
+  CHECK_LAUNCH("smx_eta_drift");
+  return 0;
+}
+
+// per-run Euclidean norm of row b of x (row stride ld), accumulated in FP64
+template <typename T>
+__global__ void __launch_bounds__(1024) k_rownorm(long n, long ld, const T *x, double *out,
+                                                  const int *active) {
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  __shared__ double red[16];
+  double s = 0.0;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+    const double v = (double)x[(long)b * ld + e];
+    s = fma(v, v, s);
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[b] = sqrt(s);
+}
+extern "C" int smx_rownorm(int prec, int B, long n, long ld, const void *x, double *out,
+                           const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if (n <= 0 || ld < n) return fail_msg("smx_rownorm: need 0 < n <= ld");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_rownorm<double>, dim3(B), dim3(1024), 0, STREAM(stream), n, ld,
+                       (const double *)x, out, active);
+  else
+    hipLaunchKernelGGL(k_rownorm<float>, dim3(B), dim3(1024), 0, STREAM(stream), n, ld,
+                       (const float *)x, out, active);
+  CHECK_LAUNCH("smx_rownorm");
+  return 0;
+}
+
 // dt = min(dtmax, clip(safety * h / max(umax, 1e-12), dtmin, dtmax));
 // params per run: safety, h, dtmin, dtmax
 __global__ void k_cfl(int B, const double *umax, const double *params, double *dt,

@@ -27,6 +27,7 @@ import threading
 import numpy as np
 import torch
 
+from . import _numerics as NUM
 from . import kclock, runtime
 from .basis import shape_ops
 
@@ -48,6 +49,10 @@ RESCUE_RESTART, RESCUE_MAXITER, RESCUE_RTOL = 20, 60, 1e-12
 # after an FP32-factored rescue of a run misses the acceptance test, that
 # run's next rescues use the FP64 factor directly this many times
 RESCUE_FP32_BACKOFF = 8
+# HBM for the lagged dense factors of one engine (one n x n factor per slot):
+# the legacy route's frozen factorisation and the mixed-precision rescue's
+# reused FP32 factor
+LAG_STORE_BYTES = 96 << 30
 DENSE_FP64_MAXN = 8192     # largest n of the FP64 dense factor (FP32: 32768)
 ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
@@ -60,6 +65,12 @@ def _law_row(p):
     return np.array([np.log(float(p.get("deta_T", 1000.0))), Ts,
                      float(p.get("eta_min", 1e-3)), float(p.get("eta_max", 1e3)),
                      float(p.get("eta_smooth", 0.0)), 1.0 if Ts > 0.0 else 0.0])
+
+
+def _is_legacy(p):
+    """Whether a run's parameters select the lagged-LU route
+    (precond='legacy', the ConvectionConfig default)."""
+    return str(p.get("precond", "bfbt")).strip().lower() == "legacy"
 
 
 LDS_BYTES = 160 * 1024      # per workgroup on gfx950
@@ -422,11 +433,20 @@ class Engine:
                       self.last_bfbt_info, self.n_rescue, self.n_rescue_direct,
                       self.n_rescue_weak, self.n_defect_failed, self.n_defect_double_fail,
                       self.n_rescue_fp32, self.n_rescue_fp32_redo, self.rescue_iters,
-                      self.fp32_backoff):
+                      self.fp32_backoff, self.n_since_refresh, self.n_refresh,
+                      self.n_defect, self.n_lag_krylov, self.last_K):
                 a[i] = 0
             self.last_bfbt_res[i] = 0.0
             self.last_defect_res[i] = 0.0
-            self.last_route[i] = "bfbt"
+            self.last_corr[i] = 0.0
+            self.lag_have[i] = False
+            self.force_refresh[i] = False
+            self.legacy[i] = _is_legacy(p)
+            self.lag[i] = max(1, int(p.get("lag", 16)))
+            if self.legacy[i] and self._lag_bytes() > LAG_STORE_BYTES:
+                raise ValueError("precond='legacy' does not fit the lagged-factor store "
+                                 "at this shape and slot count")
+            self.last_route[i] = "legacy" if self.legacy[i] else "bfbt"
 
     # ------------------------------------------------------- shared pieces
     def project(self, src, dst, active=None):
@@ -750,6 +770,28 @@ class Engine:
         self.last_route = ["bfbt"] * B
         self.defect_tol = defect_tol
         self.max_K, self.K = int(max_K), int(K)
+        # legacy route (lagged factor + Picard defect correction) and the
+        # lagged FP32 factor of the mixed-precision rescue
+        # (coupled_model.py:509-581, 693-716; lagged_stokes.py:274-343)
+        self.legacy = np.array([_is_legacy(p) for p in self.params])
+        self.lag = np.array([max(1, int(p.get("lag", 16))) for p in self.params])
+        self.eta_drift_tol = float(NUM.ETA_DRIFT_TOL)
+        self.lag_prec = 1 if (self.rescue_prec == 1 or self.n > DENSE_FP64_MAXN) else 0
+        self.lag_lu = self.lag_piv = self.lag_info = self.eta_lag = None
+        self.lag_have = np.zeros(B, dtype=bool)
+        self.force_refresh = np.zeros(B, dtype=bool)
+        self.n_since_refresh = np.zeros(B, dtype=np.int64)
+        self.n_refresh = np.zeros(B, dtype=np.int64)
+        self.n_defect = np.zeros(B, dtype=np.int64)
+        self.n_lag_krylov = np.zeros(B, dtype=np.int64)
+        self.last_K = np.zeros(B, dtype=np.int64)
+        self.last_corr = np.zeros(B)
+        self.lag_red = torch.zeros(4, B, dtype=torch.float64, device=dev)
+        if self.legacy.any() and self._lag_bytes() > LAG_STORE_BYTES:
+            raise ValueError("precond='legacy' keeps one %d x %d factor per slot: %d slots "
+                             "need %.1f GB, above the %.0f GB lagged-factor store"
+                             % (self.n, self.n, B, self._lag_bytes() / 1e9,
+                                LAG_STORE_BYTES / 1e9))
 
     def _eta_of(self, theta, active):
         """eta from theta: padded synthesis, law, window, truncation to Nk
@@ -834,18 +876,40 @@ class Engine:
         L.schur_tau(self.prec, Nk, N, self.p(t["V0"]), self.p(t["V1"]), self.p(A), st)
         return A
 
+    def _mask(self, sel):
+        """Device int32 run mask of the host bool array ``sel``."""
+        return torch.as_tensor(np.asarray(sel).astype(np.int32), device=self.dev)
+
     def _stokes_coupled(self, theta, active):
-        """eta_of + BFBT preconditioner + FGMRES on the Schur system + unpack
-        (coupled_model.py:588-723, precond='bfbt')."""
+        """eta_of, the Schur right-hand side, then per run the BFBT route
+        (preconditioner + FGMRES + rescue) or the legacy route (lagged factor +
+        Picard defect correction), and the unpack (coupled_model.py:656-723)."""
         L, t, B, Nk, N, n = self.L, self.t, self.B, self.Nk, self.Nz, self.n
         s = self.stream()
         self._eta_of(theta, active)
+        sv = self.ssol
+        L.rowscale(self.prec, B, Nk, N, self.p(theta), self.p(t["kx"]), 1.0, self.p(self.Ra), 1,
+                   self.p(sv.b), self.p(active), s)
+        act = np.ones(B, bool) if active is None else active.cpu().numpy().astype(bool)
+        leg, bf = act & self.legacy, act & ~self.legacy
+        if leg.any():
+            self._stokes_legacy(leg)
+            m = torch.as_tensor(leg, device=self.dev)
+            self.vsol[m] = sv.x[m]
+        if bf.any():
+            self._stokes_bfbt(theta, self._mask(bf) if leg.any() else active)
+        self.green(self.vsol, self.psi, active)
+
+    def _stokes_bfbt(self, theta, active):
+        """BFBT preconditioner + FGMRES on the Schur system, with the rescue
+        of rejected solves (coupled_model.py:588-654, precond='bfbt'); the
+        solution goes to self.vsol.  sv.b holds the right-hand side."""
+        L, t, B, Nk, N, n = self.L, self.t, self.B, self.Nk, self.Nz, self.n
+        s = self.stream()
         self.gemm(B, N, N, self.prof, N, 1, t["Qprof"], 1, N, self.prof_c, N, 1,
                   active=active, mask_mode=1, rpr=1)
         self._reduced_op_factor(self.prof_c, self.bf_lu, self.bf_piv, active, as_pc=True)
         sv = self.ssol
-        L.rowscale(self.prec, B, Nk, N, self.p(theta), self.p(t["kx"]), 1.0, self.p(self.Ra), 1,
-                   self.p(sv.b), self.p(active), s)
 
         def precond(vin, zout, mask):
             self._pc_apply(self.bf_lu, self.bf_piv, vin, zout, 1.0, mask)
@@ -871,7 +935,6 @@ class Engine:
         else:
             m = active.bool()
             self.vsol[m] = sv.x[m]
-        self.green(self.vsol, self.psi, active)
 
     def _rescue(self, theta, idx, rprec=None):
         """Recompute rejected BFBT solves with the robust route
@@ -892,9 +955,13 @@ class Engine:
                 bad = [r for r in bad if r not in set(back)]
                 if not bad:
                     return
+        lagged = rprec is None and self.rescue_prec == 1
         rprec = self.rescue_prec if rprec is None else int(rprec)
         if n > DENSE_FP64_MAXN:
             rprec = 1          # the FP64 factor does not go this large
+        if lagged and self._lag_store() is not None:
+            self._rescue_lagged(theta, bad)
+            return
         # dense matrices of at most RESCUE_BYTES at a time
         cap = max(1, int(RESCUE_BYTES // (n * n * (4 if rprec else 8))))
         if len(bad) > cap:
@@ -930,7 +997,7 @@ class Engine:
             R = R.contiguous()
             X = torch.empty_like(R)
             L.dense_lu_solve(0, nb, n, self.p(A), n * n, self.p(piv), self.p(R),
-                             self.p(X), st)
+                             self.p(X), None, st)
             return X
 
         def afree(X):
@@ -999,7 +1066,7 @@ class Engine:
             # round it holds the iterate the operator is applied to
             R.copy_(vin[bt])
             L.dense_lu_solve(2, nb, n, self.p(A32), n * n, self.p(piv), self.p(R),
-                             self.p(X), st)
+                             self.p(X), None, st)
             on = m[bt].bool()[:, None]
             zout[bt] = torch.where(on, X, zout[bt])
 
@@ -1034,6 +1101,233 @@ class Engine:
             self.n_rescue_fp32_redo[redo] += 1
             self.fp32_backoff[redo] = RESCUE_FP32_BACKOFF
             self._rescue(theta, redo, rprec=0)
+
+    def _rescue_lagged(self, theta, bad):
+        """Mixed-precision rescue on the reused FP32 factor: each listed run
+        keeps the factor of an earlier rescue while it stays valid by the
+        legacy route's refresh rules (lag, eta drift), FGMRES on that factor
+        (``_lag_krylov``) solves at the current eta, a miss with an older
+        factor refreshes it here and solves again, and a miss with a fresh
+        one is redone with the FP64 factor (the weak acceptance above the
+        FP64 factor's size)."""
+        B, n = self.B, self.n
+        sel = np.zeros(B, dtype=bool)
+        sel[bad] = True
+        drift = self._lag_drift(sel)
+        ref = sel & (~self.lag_have | self.force_refresh | (self.n_since_refresh >= self.lag)
+                     | (drift > self.eta_drift_tol))
+        self._lag_refresh(ref)
+        self.n_since_refresh[sel] += 1
+        ok, _, res = self._lag_krylov(sel)
+        retry = sel & ~ok & ~ref
+        if retry.any():
+            self._lag_refresh(retry)
+            self.n_since_refresh[retry] += 1
+            ok2, _, res2 = self._lag_krylov(retry)
+            ok = ok | (retry & ok2)
+            res = np.where(retry, res2, res)
+        good = sel & ok
+        self.n_rescue_fp32[good] += 1
+        for r in np.where(good)[0]:
+            self.last_route[r] = "bfbt->legacy"
+        miss = sel & ~ok
+        if not miss.any():
+            return
+        if n > DENSE_FP64_MAXN:
+            self.n_rescue_weak[miss] += 1
+            for r in np.where(miss)[0]:
+                self.last_route[r] = "bfbt->legacy(weak)"
+            return
+        self.n_rescue_fp32_redo[miss] += 1
+        self.fp32_backoff[miss] = RESCUE_FP32_BACKOFF
+        self._rescue(theta, np.where(miss)[0], rprec=0)
+
+    # ------------------------------------------ lagged factors, legacy route
+    def _lag_bytes(self):
+        return self.B * self.n * self.n * (4 if self.lag_prec else 8)
+
+    def _lag_store(self):
+        """The per-slot lagged factors (allocated on first use), or None when
+        they do not fit LAG_STORE_BYTES."""
+        if self.lag_lu is None:
+            if self._lag_bytes() > LAG_STORE_BYTES:
+                return None
+            B, n = self.B, self.n
+            dt = torch.float32 if self.lag_prec else torch.float64
+            self.lag_lu = torch.empty(B, n, n, dtype=dt, device=self.dev)
+            self.lag_piv = torch.zeros(B, n, dtype=torch.int32, device=self.dev)
+            self.lag_info = torch.zeros(B, dtype=torch.int32, device=self.dev)
+            self.eta_lag = torch.zeros_like(self.eta_u)
+        return self.lag_lu
+
+    def _lag_refresh(self, sel):
+        """Assemble and factorise the Schur matrix at the current eta for the
+        selected runs and freeze that eta (LaggedStokes.refresh,
+        lagged_stokes.py:274-300; RBCCoupled._refresh, coupled_model.py:575-581).
+        Runs of consecutive slots factor as one batch, in place."""
+        runs = np.where(sel)[0]
+        if runs.size == 0:
+            return
+        self._lag_store()
+        L, st, n = self.L, self.stream(), self.n
+        A64 = None
+        groups, r0 = [], int(runs[0])
+        for a, b in zip(runs[:-1], runs[1:]):
+            if b != a + 1:
+                groups.append((r0, int(a) + 1))
+                r0 = int(b)
+        groups.append((r0, int(runs[-1]) + 1))
+        for g0, g1 in groups:
+            A = self.lag_lu[g0:g1]
+            for r in range(g0, g1):
+                if self.lag_prec == 0:
+                    self.assemble_schur(r, out=A[r - g0])
+                else:
+                    # assemble in FP64, keep the FP32 rounding
+                    if A64 is None:
+                        A64 = torch.empty(n, n, dtype=self.dtype, device=self.dev)
+                    self.assemble_schur(r, out=A64)
+                    A[r - g0].copy_(A64)
+            L.dense_lu(self.lag_prec, g1 - g0, n, self.p(A), n * n, self.p(self.lag_piv[g0]),
+                       self.p(self.lag_info[g0:g1]), st)
+            self.eta_lag[g0:g1] = self.eta_u[g0:g1]
+        self.lag_have[runs] = True
+        self.n_since_refresh[runs] = 0
+        self.n_refresh[runs] += 1
+        self.force_refresh[runs] = False
+
+    def _lag_drift(self, sel):
+        """Pointwise relative eta drift since each selected run's frozen
+        factorisation; inf where a run has none (coupled_model.py:509-522)."""
+        out = np.full(self.B, np.inf)
+        have = sel & self.lag_have
+        if have.any():
+            d = self.lag_red[0]
+            self.L.eta_drift(self.prec, self.B, self.nru * self.Nz, self.p(self.eta_u),
+                             self.p(self.eta_lag), self.p(d), self.p(self._mask(have)),
+                             self.stream())
+            out[have] = d.cpu().numpy()[have]
+        return out
+
+    def _defect(self, sel):
+        """Picard defect correction with the lagged FP64 factor against the
+        matrix-free operator at the current eta, for the selected runs at
+        once: x0 = A_lag^-1 b, x += A_lag^-1 (b - A x) until the relative
+        correction ||dx|| / ||x0|| <= defect_tol (max_K corrections), or K
+        fixed corrections without a tolerance (RBCCoupled._defect_solve,
+        coupled_model.py:524-573).  b is in ssol.b, x goes to ssol.x.
+        -> the runs whose correction missed the tolerance."""
+        L, st, B, n = self.L, self.stream(), self.B, self.n
+        sv = self.ssol
+        LU, piv = self.p(self.lag_lu), self.p(self.lag_piv)
+        m = self._mask(sel)
+        bn, xn, cn = self.lag_red[1], self.lag_red[2], self.lag_red[3]
+        L.dense_lu_solve(0, B, n, LU, n * n, piv, self.p(sv.b), self.p(sv.x), self.p(m), st)
+        L.rownorm(self.prec, B, n, n, self.p(sv.b), self.p(bn), self.p(m), st)
+        L.rownorm(self.prec, B, n, n, self.p(sv.x), self.p(xn), self.p(m), st)
+        bnh = np.maximum(bn.cpu().numpy(), 1e-300)
+        xnh = np.maximum(xn.cpu().numpy(), 1e-300)
+        tol = self.defect_tol
+        kmax = self.max_K if tol is not None else self.K
+        live = sel.copy()
+        K = np.zeros(B, dtype=np.int64)
+        for _ in range(kmax):
+            lm = self._mask(live)
+            self.schur_matvec(sv.x, self.s2, lm)
+            L.axpby(self.prec, n, B, 1.0, None, self.p(sv.b), -1.0, None, self.p(self.s2),
+                    self.p(self.s1), self.p(lm), st)
+            L.dense_lu_solve(0, B, n, LU, n * n, piv, self.p(self.s1), self.p(self.s3),
+                             self.p(lm), st)
+            L.rownorm(self.prec, B, n, n, self.p(self.s3), self.p(cn), self.p(lm), st)
+            L.axpby(self.prec, n, B, 1.0, None, self.p(sv.x), 1.0, None, self.p(self.s3),
+                    self.p(sv.x), self.p(lm), st)
+            c = cn.cpu().numpy() / xnh
+            self.last_corr[live] = c[live]
+            K[live] += 1
+            if tol is not None:
+                live &= ~(c <= tol)
+                if not live.any():
+                    break
+        self.schur_matvec(sv.x, self.s2, m)
+        L.axpby(self.prec, n, B, 1.0, None, self.p(sv.b), -1.0, None, self.p(self.s2),
+                self.p(self.s1), self.p(m), st)
+        L.rownorm(self.prec, B, n, n, self.p(self.s1), self.p(cn), self.p(m), st)
+        res = cn.cpu().numpy() / bnh
+        self.last_K[sel] = K[sel]
+        self.last_defect_res[sel] = res[sel]
+        self.n_defect[sel] += 1
+        if tol is None:
+            return np.zeros(B, dtype=bool)
+        c = self.last_corr
+        failed = sel & (~np.isfinite(c) | (c > tol))
+        self.n_defect_failed[failed] += 1
+        return failed
+
+    def _lag_krylov(self, sel):
+        """FGMRES in FP64 on the matrix-free Schur operator at the current
+        eta, right-preconditioned by each selected run's lagged FP32 factor
+        (the frozen-preconditioner FGMRES of LaggedStokes.solve_fgmres,
+        lagged_stokes.py:345-360, with the dense factor as the preconditioner).
+        b is in ssol.b, x goes to ssol.x.  -> (accepted, iterations, true
+        relative residual) per run."""
+        L, st, B, n = self.L, self.stream(), self.B, self.n
+        sv = self.ssol
+        rs = getattr(self, "rsol", None)
+        if rs is None:
+            rs = self.rsol = Solver(self, n, RESCUE_RESTART, RESCUE_MAXITER, RESCUE_RTOL)
+        rs.b.copy_(sv.b)
+        LU, piv = self.p(self.lag_lu), self.p(self.lag_piv)
+
+        def precond(vin, zout, mm):
+            # masked rows keep zout (a residual round holds the iterate there)
+            L.dense_lu_solve(2, B, n, LU, n * n, piv, self.p(vin), self.p(zout), self.p(mm), st)
+
+        def op(zin, wout, mm):
+            self.schur_matvec(zin, wout, mm)
+
+        m = self._mask(sel)
+        rs.solve(precond, op, m)
+        it, info, res = rs.results()
+        mb = m.bool()
+        sv.x[mb] = rs.x[mb]
+        ok = sel & np.isfinite(res) & (res <= sv.rtol)
+        self.n_lag_krylov[sel] += 1
+        self.rescue_iters[sel] += it[sel]
+        self.last_defect_res[sel] = res[sel]
+        return ok, it, res
+
+    def _stokes_legacy(self, sel):
+        """The legacy route for the selected runs (coupled_model.py:693-716):
+        refresh the frozen factorisation when a run has none, when ``lag``
+        solves have used it or when eta drifted more than the tolerance since;
+        then the defect correction, and on failure a refresh at this eta and
+        a second correction (a second failure is counted and accepted).  With
+        the FP32 factor (precision 'mixed', or n above the FP64 factor's
+        limit) FGMRES on the lagged factor takes the place of the Picard
+        loop.  ssol.b holds the right-hand side; the solution goes to ssol.x."""
+        drift = self._lag_drift(sel)
+        ref = sel & (~self.lag_have | self.force_refresh | (self.n_since_refresh >= self.lag)
+                     | (drift > self.eta_drift_tol))
+        self._lag_refresh(ref)
+        self.n_since_refresh[sel] += 1
+        if self.lag_prec == 0:
+            failed = self._defect(sel)
+        else:
+            ok, _, _ = self._lag_krylov(sel)
+            failed = sel & ~ok
+            self.n_defect_failed[failed] += 1
+        if failed.any():
+            self._lag_refresh(failed)
+            self.n_since_refresh[failed] += 1
+            if self.lag_prec == 0:
+                failed2 = self._defect(failed)
+            else:
+                ok, _, _ = self._lag_krylov(failed)
+                failed2 = failed & ~ok
+                self.n_defect_failed[failed2] += 1
+            self.n_defect_double_fail[failed2] += 1
+        for r in np.where(sel)[0]:
+            self.last_route[r] = "legacy"
 
     def stokes(self, theta, active=None):
         """(psi, v = -omega) of ``theta`` for the active runs into self.psi /

@@ -63,7 +63,8 @@ class _DeviceModel:
                       eta_min=float(eta_min), eta_max=float(eta_max),
                       eta_smooth=float(eta_smooth),
                       internal_heating=self.internal_heating,
-                      dissipation_number=self.dissipation_number)
+                      dissipation_number=self.dissipation_number,
+                      precond=str(precond).strip().lower(), lag=int(extra.get("lag", 16)))
         kw = dict(defect_tol=defect_tol, max_K=int(max_K), K=int(K), precision=precision,
                   graph_rounds=int(graph_rounds))
         if stokes_maxiter is not None:
@@ -208,7 +209,13 @@ class _DeviceModel:
             d.update(theta_scheme=NUM.THETA_SCHEME, theta_rtol=NUM.THETA_RTOL,
                      n_theta_nonconv=th.n_nonconv, last_theta_iters=th.last_iters)
         if self.kind == KIND_COUPLED:
-            d.update(precond="bfbt", n_T_out_of_range=int(c[0]), n_clip_T=int(c[1]),
+            d.update(precond="legacy" if e.legacy[0] else "bfbt",
+                     stokes_mode="defect", lag=int(e.lag[0]), K=e.K, max_K=e.max_K,
+                     defect_tol=e.defect_tol, eta_drift_tol=e.eta_drift_tol,
+                     n_refresh=int(e.n_refresh[0]), n_defect=int(e.n_defect[0]),
+                     last_K=int(e.last_K[0]),
+                     n_defect_double_fail=int(e.n_defect_double_fail[0]),
+                     n_T_out_of_range=int(c[0]), n_clip_T=int(c[1]),
                      n_clip_eta=int(c[2]), max_T_excursion=float(c[3]),
                      n_clip_eta_proj=int(c[4]), n_memo_hits=int(e.memo_hits[0]),
                      n_bfbt_iters=int(e.bfbt_iters[0]),
@@ -244,8 +251,45 @@ class RBCImplicitTheta(_DeviceModel):
 
 class RBCCoupled(_DeviceModel):
     """eta(T(x,z,t)) coupled Stokes through the BFBT-preconditioned FGMRES
-    route + BDF2 implicit theta (coupled_model.py:123-821)."""
+    route or the legacy lagged-LU route + BDF2 implicit theta
+    (coupled_model.py:123-821)."""
     kind = KIND_COUPLED
+
+    @property
+    def precond(self):
+        return "legacy" if self.eng.legacy[0] else "bfbt"
+
+    @property
+    def lag(self):
+        return int(self.eng.lag[0])
+
+    @property
+    def n_refresh(self):
+        return int(self.eng.n_refresh[0])
+
+    @property
+    def n_since_refresh(self):
+        return int(self.eng.n_since_refresh[0])
+
+    @property
+    def n_defect(self):
+        return int(self.eng.n_defect[0])
+
+    @property
+    def n_defect_failed(self):
+        return int(self.eng.n_defect_failed[0])
+
+    @property
+    def n_defect_double_fail(self):
+        return int(self.eng.n_defect_double_fail[0])
+
+    @property
+    def last_K(self):
+        return int(self.eng.last_K[0])
+
+    @property
+    def last_defect_res(self):
+        return float(self.eng.last_defect_res[0])
 
     @property
     def last_bfbt_iters(self):

@@ -35,7 +35,9 @@ SIGNATURES = {
     "smx_set_gj_reg": [I],
     "smx_set_inv_pairs": [I],
     "smx_dense_lu": [I, I, I, P, L, P, P, P],
-    "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P],
+    "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P, P],
+    "smx_eta_drift": [I, I, L, P, P, P, P, P],
+    "smx_rownorm": [I, I, L, L, P, P, P, P],
     "smx_inv_apply": [I, I, I, P, P, P, I, D, P, P, I, I, P, P],
     "smx_tau_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P],
     "smx_cheb_mult": [I, I, I, P, P, P, P],

@@ -31,13 +31,15 @@ from . import _numerics as NUM
 from .api import model_spec
 from .batch import run_params, _KIND
 from .core.cfl_policy import clamp_safety
-from .engine import Engine, KIND_CONST, KIND_COUPLED
+from .engine import (DENSE_FP64_MAXN, Engine, KIND_CONST, KIND_COUPLED, RESCUE_RESTART)
 
 
 def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=40):
     """Resident HBM bytes of one slot of an FP64 ``Engine`` of this shape
-    (Krylov bases, per-mode blocks, product-grid fields, state), not counting
-    the transient dense rescue matrices (bounded by ``engine.RESCUE_BYTES``)."""
+    (Krylov bases, per-mode blocks, product-grid fields, state, and for the
+    coupled kind the slot's lagged dense factor: FP32 for ``precision``
+    'mixed' or above the FP64 factor's size, else FP64), not counting the
+    transient dense rescue matrices (bounded by ``engine.RESCUE_BYTES``)."""
     Nk, N = Nx // 2 + 1, Nz
     n, nrp, nru = Nk * N, Nx + 1, Nx // 2 + 1
     Q = (3 * N - 2 + 1) // 2 + 1
@@ -51,6 +53,10 @@ def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=4
         b += (2 * stokes_restart + 6) * n * w                       # Stokes FGMRES
         b += Nk * N * N * w                                         # BFBT blocks
         b += (2 * Nk * 4 * Q + Nk * 7 * N + nrp * Q + 2 * nrp * N + 4 * n) * w
+        f32 = precision == "mixed" or n > DENSE_FP64_MAXN
+        b += n * n * (4 if f32 else 8) + n * 4                      # lagged factor
+        if f32:
+            b += (2 * RESCUE_RESTART + 6) * n * w                   # its FGMRES
     else:
         b += Nk * N * N * w                                         # eta(z) operator LU
     return int(b)

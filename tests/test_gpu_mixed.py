@@ -71,11 +71,11 @@ def test_dense_lu_fp32(eng, n):
     b32 = torch.tensor(b.astype(np.float32), device=eng.dev)
     x32 = torch.zeros_like(b32)
     lib.dense_lu_solve(1, nm, n, A.data_ptr(), n * n, piv.data_ptr(), b32.data_ptr(),
-                       x32.data_ptr(), eng.stream())
+                       x32.data_ptr(), None, eng.stream())
     bt = torch.tensor(b, device=eng.dev)
     x = torch.zeros_like(bt)
     lib.dense_lu_solve(2, nm, n, A.data_ptr(), n * n, piv.data_ptr(), bt.data_ptr(),
-                       x.data_ptr(), eng.stream())
+                       x.data_ptr(), None, eng.stream())
     torch.cuda.synchronize()
     assert info.cpu().numpy().tolist() == [0] * nm
     xs = np.stack([np.linalg.solve(M[q], b[q]) for q in range(nm)])
@@ -88,7 +88,7 @@ def test_dense_lu_fp32(eng, n):
         rt = torch.tensor(r, device=eng.dev)
         dx = torch.zeros_like(rt)
         lib.dense_lu_solve(2, nm, n, A.data_ptr(), n * n, piv.data_ptr(), rt.data_ptr(),
-                           dx.data_ptr(), eng.stream())
+                           dx.data_ptr(), None, eng.stream())
         torch.cuda.synchronize()
         xr = xr + dx.cpu().numpy()
     eref = rel(xr, xs)
@@ -178,7 +178,7 @@ def test_dense_lu_fp32_large(eng, n):
     hist = []
     for _ in range(4):
         lib.dense_lu_solve(2, 1, n, A.data_ptr(), n * n, piv.data_ptr(), r.data_ptr(),
-                           dx.data_ptr(), eng.stream())
+                           dx.data_ptr(), None, eng.stream())
         x += dx
         r = b - (M @ x[0])[None, :]
         hist.append(float(torch.linalg.norm(r) / torch.linalg.norm(b)))

@@ -62,7 +62,7 @@ def _ic(model, meta):
                         ic["seed"])
 
 
-GOLDEN = ["c0_const_32x24", "const_h_di_16x24", "etaz_16x24", "etaT_bfbt_16x24",
+GOLDEN = ["c0_const_32x24", "const_h_di_16x24", "etaz_16x24", "etaT_bfbt_16x24", "etaT_legacy_16x24",
           "arrh_bfbt_16x24", "etaT_bfbt_32x48_h", "c1_short_64x96"]
 
 
@@ -225,7 +225,7 @@ def test_dense_lu_matches_lapack(lib, n):
     bt = torch.tensor(b, device=e.dev)
     x = torch.zeros_like(bt)
     lib.dense_lu_solve(0, nm, n, A.data_ptr(), n * n, piv.data_ptr(), bt.data_ptr(),
-                       x.data_ptr(), e.stream())
+                       x.data_ptr(), None, e.stream())
     torch.cuda.synchronize()
     assert info.cpu().numpy().tolist() == [0] * nm
     F, pv, xs = A.cpu().numpy(), piv.cpu().numpy(), x.cpu().numpy()
