@@ -45,6 +45,8 @@ WORKLOAD = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64, Nz=96, Lx=2.
                 precond="bfbt")
 TSTOP = 5e-3
 CHECKPOINTS = (100, 300, 525)
+# timed steps of the rescue phase at least: the default lag (ConvectionConfig.lag)
+HARD_MIN_STEPS = 16
 
 
 def make_states(path=STATES):
@@ -378,19 +380,22 @@ def main():
             continue
         s0 = z["s%d" % step]
         sts = [s0 * (1.0 + 1e-3 * rng.normal()) for _ in range(a.batch)]
-        r = {}
+        # the rescue phase times at least one lag period, so that every run's
+        # lagged-factor refresh falls inside the window at its real rate
+        ns = max(a.steps, HARD_MIN_STEPS) if ph == "hard" else a.steps
+        r = {"steps": ns}
         if "plain" in legs:
             plain = _ensemble([s0], pc=a.pc, ortho=a.ortho)
-            n1, t1 = _timed(plain, a.steps, a.warmup)
+            n1, t1 = _timed(plain, ns, a.warmup)
             r["plain"] = n1 / t1
             del plain
             torch.cuda.empty_cache()
         ref = None
         if "batched" in legs:
             ens = _ensemble(sts, pc=a.pc, ortho=a.ortho)
-            n2, t2 = _timed(ens, a.steps, a.warmup, clock)
+            n2, t2 = _timed(ens, ns, a.warmup, clock)
             ubytes = unit_bytes(ens.eng)
-            r.update(batched=n2 / t2, ms_batched_step=1e3 * t2 / a.steps,
+            r.update(batched=n2 / t2, ms_batched_step=1e3 * t2 / ns,
                      bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
                      rescues=int(ens.eng.n_rescue.sum()), pc_apply=ens.eng.pc_apply,
                      ortho=ens.eng.ortho)
@@ -401,8 +406,9 @@ def main():
         if "mixed" in legs:
             ens = _ensemble(sts, pc=a.pc, ortho=a.ortho, precision="mixed",
                             graph_rounds=a.graph_rounds)
-            n3, t3 = _timed(ens, a.steps, a.warmup)
-            r.update(mixed=n3 / t3, ms_mixed_step=1e3 * t3 / a.steps,
+            n3, t3 = _timed(ens, ns, a.warmup)
+            r.update(mixed=n3 / t3, ms_mixed_step=1e3 * t3 / ns,
+                     lag_refreshes=int(ens.eng.n_refresh.sum()),
                      rescues_fp32=int(ens.eng.n_rescue_fp32.sum()),
                      rescues_fp32_redone=int(ens.eng.n_rescue_fp32_redo.sum()),
                      rescue_fgmres_iters=int(ens.eng.rescue_iters.sum()))

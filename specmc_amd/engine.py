@@ -53,6 +53,9 @@ RESCUE_FP32_BACKOFF = 8
 # the legacy route's frozen factorisation and the mixed-precision rescue's
 # reused FP32 factor
 LAG_STORE_BYTES = 96 << 30
+# the mixed-precision rescue refactors a run's reused FP32 factor once the
+# FGMRES on it took more than this many iterations (or after ``lag`` uses)
+RESCUE_REFRESH_ITERS = 16
 DENSE_FP64_MAXN = 8192     # largest n of the FP64 dense factor (FP32: 32768)
 ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
@@ -434,7 +437,7 @@ class Engine:
                       self.n_rescue_weak, self.n_defect_failed, self.n_defect_double_fail,
                       self.n_rescue_fp32, self.n_rescue_fp32_redo, self.rescue_iters,
                       self.fp32_backoff, self.n_since_refresh, self.n_refresh,
-                      self.n_defect, self.n_lag_krylov, self.last_K):
+                      self.n_defect, self.n_lag_krylov, self.last_K, self.lag_iters):
                 a[i] = 0
             self.last_bfbt_res[i] = 0.0
             self.last_defect_res[i] = 0.0
@@ -786,6 +789,7 @@ class Engine:
         self.n_lag_krylov = np.zeros(B, dtype=np.int64)
         self.last_K = np.zeros(B, dtype=np.int64)
         self.last_corr = np.zeros(B)
+        self.lag_iters = np.zeros(B, dtype=np.int64)
         self.lag_red = torch.zeros(4, B, dtype=torch.float64, device=dev)
         if self.legacy.any() and self._lag_bytes() > LAG_STORE_BYTES:
             raise ValueError("precond='legacy' keeps one %d x %d factor per slot: %d slots "
@@ -1103,27 +1107,31 @@ class Engine:
             self._rescue(theta, redo, rprec=0)
 
     def _rescue_lagged(self, theta, bad):
-        """Mixed-precision rescue on the reused FP32 factor: each listed run
-        keeps the factor of an earlier rescue while it stays valid by the
-        legacy route's refresh rules (lag, eta drift), FGMRES on that factor
-        (``_lag_krylov``) solves at the current eta, a miss with an older
-        factor refreshes it here and solves again, and a miss with a fresh
-        one is redone with the FP64 factor (the weak acceptance above the
-        FP64 factor's size)."""
+        """Mixed-precision rescue on the reused FP32 factor.  FGMRES on the
+        matrix-free operator at the current eta is preconditioned by the
+        factor of an earlier rescue (``_lag_krylov``); the true residual
+        decides acceptance, so the factor's age moves only the iteration
+        count.  A run refactors at this eta when it has no factor, after
+        ``lag`` uses, or when its last solve took more than
+        RESCUE_REFRESH_ITERS iterations; a miss with an older factor
+        refactors and solves again, and a miss with a fresh one is redone
+        with the FP64 factor (the weak acceptance above the FP64 factor's
+        size)."""
         B, n = self.B, self.n
         sel = np.zeros(B, dtype=bool)
         sel[bad] = True
-        drift = self._lag_drift(sel)
         ref = sel & (~self.lag_have | self.force_refresh | (self.n_since_refresh >= self.lag)
-                     | (drift > self.eta_drift_tol))
+                     | (self.lag_iters > RESCUE_REFRESH_ITERS))
         self._lag_refresh(ref)
         self.n_since_refresh[sel] += 1
-        ok, _, res = self._lag_krylov(sel)
+        ok, it, res = self._lag_krylov(sel)
+        self.lag_iters[sel] = it[sel]
         retry = sel & ~ok & ~ref
         if retry.any():
             self._lag_refresh(retry)
             self.n_since_refresh[retry] += 1
-            ok2, _, res2 = self._lag_krylov(retry)
+            ok2, it2, res2 = self._lag_krylov(retry)
+            self.lag_iters[retry] = it2[retry]
             ok = ok | (retry & ok2)
             res = np.where(retry, res2, res)
         good = sel & ok

@@ -83,16 +83,18 @@ def test_served_mix_matches_solo(dev, concurrent):
 
 
 def test_admission_by_memory_budget(dev):
-    """A budget that holds one slot of the larger shape only: the second
-    shape waits until the idle bucket of the first is released, a second
-    request of the first shape waits for the second bucket, and every
-    request completes with its solo result."""
+    """A budget that holds one slot of the larger shape but no two buckets
+    of different shapes: the second shape waits until the idle bucket of the
+    first is released, a request of the first shape that arrives later waits
+    for the second bucket, and every request completes with its solo
+    result."""
     from specmc_amd import api
     from specmc_amd.serve import Server, run_bytes
     reqs = [r for r in _requests() if r.rid in ("a", "f", "g")]
     key_bytes = [run_bytes(Server.key_of(r.cfg)[0], r.cfg.Nx, r.cfg.Nz) for r in reqs]
-    assert 2 * min(key_bytes) > max(key_bytes)
-    srv = Server(slots=4, mem_budget=max(key_bytes))
+    budget = max(key_bytes)
+    assert min(key_bytes) + max(key_bytes) > budget
+    srv = Server(slots=4, mem_budget=budget)
     peak = []
     srv.submit(reqs)
     done = srv.run(max_wall=300, on_retire=lambda r: peak.append(len(srv.buckets)))
