@@ -1172,37 +1172,54 @@ class Engine:
         """Assemble and factorise the Schur matrix at the current eta for the
         selected runs and freeze that eta (LaggedStokes.refresh,
         lagged_stokes.py:274-300; RBCCoupled._refresh, coupled_model.py:575-581).
-        Runs of consecutive slots factor as one batch, in place."""
+        Runs of consecutive slots factor as one batch in place; scattered
+        runs (slots admitted at different times) are gathered into one batch
+        and their factors copied into their slots, so the LU keeps its batch
+        width."""
         runs = np.where(sel)[0]
         if runs.size == 0:
             return
         self._lag_store()
         L, st, n = self.L, self.stream(), self.n
-        A64 = None
-        groups, r0 = [], int(runs[0])
-        for a, b in zip(runs[:-1], runs[1:]):
-            if b != a + 1:
-                groups.append((r0, int(a) + 1))
-                r0 = int(b)
-        groups.append((r0, int(runs[-1]) + 1))
-        for g0, g1 in groups:
-            A = self.lag_lu[g0:g1]
-            for r in range(g0, g1):
-                if self.lag_prec == 0:
-                    self.assemble_schur(r, out=A[r - g0])
-                else:
-                    # assemble in FP64, keep the FP32 rounding
-                    if A64 is None:
-                        A64 = torch.empty(n, n, dtype=self.dtype, device=self.dev)
-                    self.assemble_schur(r, out=A64)
-                    A[r - g0].copy_(A64)
-            L.dense_lu(self.lag_prec, g1 - g0, n, self.p(A), n * n, self.p(self.lag_piv[g0]),
-                       self.p(self.lag_info[g0:g1]), st)
-            self.eta_lag[g0:g1] = self.eta_u[g0:g1]
+        self._A64 = None
+        if runs[-1] - runs[0] + 1 == runs.size:
+            g0, g1 = int(runs[0]), int(runs[-1]) + 1
+            self._assemble_into(range(g0, g1), self.lag_lu[g0:g1])
+            L.dense_lu(self.lag_prec, g1 - g0, n, self.p(self.lag_lu[g0]), n * n,
+                       self.p(self.lag_piv[g0]), self.p(self.lag_info[g0:g1]), st)
+        else:
+            cap = max(1, int(RESCUE_BYTES // (n * n * self.lag_lu.element_size())))
+            for c0 in range(0, runs.size, cap):
+                chunk = runs[c0:c0 + cap]
+                A = torch.empty(chunk.size, n, n, dtype=self.lag_lu.dtype, device=self.dev)
+                piv = torch.empty(chunk.size, n, dtype=torch.int32, device=self.dev)
+                self._assemble_into(chunk, A)
+                L.dense_lu(self.lag_prec, chunk.size, n, self.p(A), n * n, self.p(piv),
+                           self.p(self.lag_info), st)
+                idx = torch.as_tensor(chunk, device=self.dev)
+                self.lag_lu.index_copy_(0, idx, A)
+                self.lag_piv.index_copy_(0, idx, piv)
+                del A, piv
+        self._A64 = None
+        rt = torch.as_tensor(runs, device=self.dev)
+        self.eta_lag[rt] = self.eta_u[rt]
         self.lag_have[runs] = True
         self.n_since_refresh[runs] = 0
         self.n_refresh[runs] += 1
         self.force_refresh[runs] = False
+
+    def _assemble_into(self, runs, A):
+        """Dense Schur matrices of ``runs`` at their current eta into the
+        consecutive matrices of ``A`` (FP64 assembly, rounded once when A is
+        FP32)."""
+        for q, r in enumerate(runs):
+            if A.dtype == self.dtype:
+                self.assemble_schur(int(r), out=A[q])
+            else:
+                if self._A64 is None:
+                    self._A64 = torch.empty(self.n, self.n, dtype=self.dtype, device=self.dev)
+                self.assemble_schur(int(r), out=self._A64)
+                A[q].copy_(self._A64)
 
     def _lag_drift(self, sel):
         """Pointwise relative eta drift since each selected run's frozen
