@@ -81,6 +81,37 @@ PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fg_post_c1_b256_traffic.json")
 
 
+def legs_of(a):
+    return a.legs.split(",")
+
+
+def _tune(z, a, legs):
+    """The autotuner's GEMM pass (autotune.tune_gemms) on a throwaway batch
+    of each batch size the legs run, from the first checkpoint: one step is
+    recorded, every GEMM shape it issued is timed under each kernel choice,
+    and the fastest goes into the library's table for the timed legs."""
+    import torch
+    from specmc_amd import autotune
+    sizes = sorted({1 if "plain" in legs else None, a.batch} - {None})
+    s0 = z["s%d" % CHECKPOINTS[0]]
+    rep = {}
+    for nb in sizes:
+        t0 = time.perf_counter()
+        ens = _ensemble([s0] * nb, pc=a.pc, ortho=a.ortho)
+        g = autotune.tune_gemms(ens.eng, ens.step)
+        del ens
+        torch.cuda.empty_cache()
+        per_step = sum(v["us"] * v["calls_per_step"] for v in g.values())
+        ref = sum(v["candidates"].get("mfma32x32", v["us"]) * v["calls_per_step"]
+                  for v in g.values())
+        rep[str(nb)] = dict(shapes=len(g), seconds=round(time.perf_counter() - t0, 2),
+                            gemm_us_per_step=round(per_step, 1),
+                            gemm_us_per_step_mfma32x32=round(ref, 1),
+                            kernels=sorted({v["kernel"] for v in g.values()}))
+    print("autotune: %s" % json.dumps(rep), file=sys.stderr, flush=True)
+    return rep
+
+
 def _ensemble(states, prec=0, pc=None, ortho=None, precision="fp64", graph_rounds=0):
     from specmc_amd import ConvectionConfig
     from specmc_amd.batch import Ensemble
@@ -311,6 +342,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=0,
                     help="requests of the served leg (default: the batch size)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the GEMM autotuner pass before the timed legs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device once first (a known-size "
                          "reference for hardware byte counters)")
@@ -369,6 +402,7 @@ def main():
     log = z["log"]
     w = phase_weights(log)
     names = ("early", "hard", "late")
+    tuning = None if a.no_tune else _tune(z, a, legs_of(a))
     from specmc_amd.kclock import KernelClock
     legs = a.legs.split(",")
     rng = np.random.default_rng(1)
@@ -442,7 +476,7 @@ def main():
                plain_fp64_steps_per_s=plain_rate, batched_fp64_steps_per_s=fp64_rate,
                batched_mixed_steps_per_s=mixed_rate,
                speedup_vs_plain=(value / plain_rate if plain_rate else None),
-               phases=res)
+               phases=res, autotune=tuning)
     errs = [res[p]["mixed_vs_fp64"] for p in res if "mixed_vs_fp64" in res[p]]
     if errs:
         out["mixed_error_vs_fp64"] = {k: max(e[k] for e in errs) for k in errs[0]}

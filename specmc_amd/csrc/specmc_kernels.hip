@@ -25,6 +25,22 @@ struct GemmArgs {
   const int *active; int mask_mode, rows_per_run;
 };
 
+// Tile of this workgroup in the (N tiles x M tiles) plane of one batch entry.
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2: the launch order would put the N tiles of one row block on different
+// XCDs, and each XCD would fetch that row block of A from HBM again.  The
+// remap hands each XCD a contiguous range of tiles (x fastest), so the N
+// tiles of a row block share one XCD and its L2.  A bijection for any
+// tile count; it changes only which workgroup computes a tile.
+__device__ __forceinline__ void xcd_tile(int &ti, int &tj) {
+  const int gx = gridDim.x, T = gx * gridDim.y;
+  const int L = blockIdx.x + gx * blockIdx.y;
+  const int per = T >> 3, rem = T & 7, xcd = L & 7, idx = L >> 3;
+  const int t = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+  ti = t / gx;
+  tj = t - ti * gx;
+}
+
 // R: role of the product, part of the kernel symbol only (so kernel traces
 // separate them): 0 z-transform / z-derivative, 1 x-transform, 2 per-mode
 // operators (Green applies, block builds), 3 dense-matrix work of the rescue
@@ -35,7 +51,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   __shared__ T As[16][TM + 1];
   __shared__ T Bs[16][TN + 1];
   const int z = blockIdx.z, z1 = z / g.nz2, z2 = z % g.nz2;
-  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  int ti, tj;
+  xcd_tile(ti, tj);
+  const int i0 = ti * TM, j0 = tj * TN;
   if (g.active) {
     if (g.mask_mode == 2 && !g.active[z1]) return;
     if (g.mask_mode == 3 && !g.active[z2]) return;
@@ -163,7 +181,9 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   __shared__ T As[KC][TM + 1];
   __shared__ T Bs[KC][TN + 1];
   const int z = blockIdx.z, z1 = z / g.nz2, z2 = z % g.nz2;
-  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  int ti, tj;
+  xcd_tile(ti, tj);
+  const int i0 = ti * TM, j0 = tj * TN;
   if (g.active) {
     if (g.mask_mode == 2 && !g.active[z1]) return;
     if (g.mask_mode == 3 && !g.active[z2]) return;
