@@ -9,7 +9,6 @@ ConvectionConfig default).
 """
 from __future__ import annotations
 
-import os
 
 import numpy as np
 
@@ -113,15 +112,13 @@ def production_dt(model, that):
 
 
 def load_checkpoint(path):
-    """-> ``(theta_coefficients, meta)`` from a driver checkpoint ``.npz``
-    (io/checkpoint.py:80-103): fields ``that`` and optionally ``t, n, fac``."""
-    if not os.path.exists(path):
-        raise FileNotFoundError(path)
-    with np.load(path, allow_pickle=False) as z:
-        that = np.array(z["that"])
-        meta = {k: (z[k].item() if z[k].shape == () else np.array(z[k]))
-                for k in z.files if k != "that"}
-    return that, meta
+    """-> ``(theta_coefficients, meta)`` from a state file (api.py's
+    load_checkpoint delegating to io/checkpoint.py:82-103): FileNotFoundError
+    for a missing file, ValueError for a file without ``that``; meta holds
+    path, keys, t, n, fac (and whether the file carries a device snapshot,
+    see ``checkpoint.load_run``)."""
+    from . import checkpoint
+    return checkpoint.load(path)
 
 
 def march(model, that, dt, nsteps, keep_states=False, on_step=None):

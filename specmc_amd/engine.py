@@ -22,6 +22,7 @@ Layouts (row-major, run-major)
     Krylov bases         [B][m+1][n], [B][m][n]
     LU factors           [B][Nk][Nz*Nz] column-major, pivots [B][Nk][Nz]
 """
+import json
 import threading
 
 import numpy as np
@@ -450,6 +451,76 @@ class Engine:
                 raise ValueError("precond='legacy' does not fit the lagged-factor store "
                                  "at this shape and slot count")
             self.last_route[i] = "legacy" if self.legacy[i] else "bfbt"
+
+    # coupled-model counters carried by a snapshot (per-run host arrays)
+    _SNAP_COUNTERS = ("memo_hits", "bfbt_iters", "last_bfbt_iters", "last_bfbt_info",
+                      "last_bfbt_res", "n_rescue", "n_rescue_direct", "n_rescue_weak",
+                      "n_defect_failed", "n_defect_double_fail", "n_rescue_fp32",
+                      "n_rescue_fp32_redo", "rescue_iters", "fp32_backoff", "last_defect_res",
+                      "n_since_refresh", "n_refresh", "n_defect", "n_lag_krylov", "last_K",
+                      "last_corr", "lag_iters", "force_refresh")
+
+    def snapshot(self, slot):
+        """-> the state of the run in ``slot`` as a dict of numpy arrays:
+        physics, theta, the BDF2 history, counters, and the lagged factor
+        when the run holds one.  ``restore`` of it into a slot of an engine of
+        the same kind, shape and precision continues the run exactly."""
+        i = int(slot)
+        if not 0 <= i < self.B:
+            raise IndexError("slot %d outside 0..%d" % (i, self.B - 1))
+        h = lambda t: t[i].detach().cpu().numpy().copy()
+        snap = dict(kind=np.array(self.kind), shape=np.array([self.Nx, self.Nz]),
+                    Lx=np.array(float(self.ops.Lx)),
+                    precision=np.array(self.precision), prec=np.array(self.prec),
+                    params=np.array(json.dumps(self.params[i], sort_keys=True)),
+                    theta=h(self.theta), n_steps=np.array(self.n_steps[i]),
+                    n_stokes=np.array(self.n_stokes[i]), counters=h(self.counters))
+        if self.kind != KIND_CONST:
+            snap.update(theta_prev=h(self.theta_prev), psi_prev=h(self.psi_prev),
+                        dt_prev=np.array(self.dt_prev[i]), has_hist=np.array(self.has_hist[i]))
+        if self.kind == KIND_COUPLED:
+            for k in self._SNAP_COUNTERS:
+                snap["c_" + k] = np.array(getattr(self, k)[i])
+            snap["last_route"] = np.array(self.last_route[i])
+            if self.lag_have[i]:
+                snap.update(lag_lu=h(self.lag_lu), lag_piv=h(self.lag_piv),
+                            eta_lag=h(self.eta_lag), lag_prec=np.array(self.lag_prec))
+        return snap
+
+    def restore(self, slot, snap):
+        """Load a ``snapshot`` into ``slot`` (any slot; the other slots are
+        untouched)."""
+        i = int(slot)
+        kind, shape = str(snap["kind"]), [int(v) for v in snap["shape"]]
+        if (kind != self.kind or shape != [self.Nx, self.Nz]
+                or float(snap["Lx"]) != float(self.ops.Lx)):
+            raise ValueError("snapshot of a %s %dx%d run does not fit this %s %dx%d engine"
+                             % (kind, shape[0], shape[1], self.kind, self.Nx, self.Nz))
+        if str(snap["precision"]) != self.precision or int(snap["prec"]) != self.prec:
+            raise ValueError("snapshot precision %s does not match the engine's %s"
+                             % (snap["precision"], self.precision))
+        params = json.loads(str(snap["params"]))
+        self.admit(i, params, np.asarray(snap["theta"]).reshape(self.Nk, self.Nz))
+        d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev)
+        self.n_steps[i] = int(snap["n_steps"])
+        self.n_stokes[i] = int(snap["n_stokes"])
+        self.counters[i] = d(snap["counters"]).to(self.counters.dtype)
+        if self.kind != KIND_CONST:
+            self.theta_prev[i] = d(snap["theta_prev"]).to(self.dtype)
+            self.psi_prev[i] = d(snap["psi_prev"]).to(self.dtype)
+            self.dt_prev[i] = float(snap["dt_prev"])
+            self.has_hist[i] = bool(snap["has_hist"])
+        if self.kind == KIND_COUPLED:
+            for k in self._SNAP_COUNTERS:
+                getattr(self, k)[i] = snap["c_" + k]
+            self.last_route[i] = str(snap["last_route"])
+            if "lag_lu" in snap:
+                if int(snap["lag_prec"]) != self.lag_prec or self._lag_store() is None:
+                    raise ValueError("the snapshot's lagged factor does not fit this engine")
+                self.lag_lu[i] = d(snap["lag_lu"])
+                self.lag_piv[i] = d(snap["lag_piv"])
+                self.eta_lag[i] = d(snap["eta_lag"]).to(self.dtype)
+                self.lag_have[i] = True
 
     # ------------------------------------------------------- shared pieces
     def project(self, src, dst, active=None):

@@ -125,6 +125,11 @@ class _DeviceModel:
         self.eng.set_theta(np.real(a)[None])
         self._dev_key = key
 
+    def _resume_from_device(self):
+        """The state held on the device (after a restore), as the array the
+        next ``step`` call must be handed."""
+        return self._download()
+
     def _download(self):
         out = self.eng.get(self.eng.theta)[0]
         if self.kind == KIND_CONST:

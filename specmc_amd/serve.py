@@ -63,19 +63,26 @@ def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=4
 
 
 class Request:
-    """One run to serve."""
+    """One run to serve.  ``priority``: a request that finds its bucket full
+    preempts a running request of lower priority there (the run is saved as
+    a device snapshot, re-queued, and resumed from the snapshot in the next
+    free slot)."""
 
     __slots__ = ("rid", "cfg", "theta0", "tstop", "arrival", "admitted", "finished",
-                 "steps", "t", "theta", "diag", "bucket", "slot")
+                 "steps", "t", "theta", "diag", "bucket", "slot", "priority", "snapshot",
+                 "preemptions")
 
-    def __init__(self, rid, cfg, theta0, tstop, arrival=0.0):
+    def __init__(self, rid, cfg, theta0, tstop, arrival=0.0, priority=0):
         self.rid, self.cfg = rid, cfg
         self.theta0 = np.asarray(theta0)
         self.tstop, self.arrival = float(tstop), float(arrival)
+        self.priority = priority
         self.admitted = self.finished = None
         self.steps, self.t = 0, 0.0
         self.theta = self.diag = None
         self.bucket = self.slot = None
+        self.snapshot = None
+        self.preemptions = 0
 
     @property
     def latency(self):
@@ -109,12 +116,41 @@ class Bucket:
         return [i for i in range(self.S) if self.req[i] is None]
 
     def admit(self, slot, r, now):
-        self.eng.admit(slot, run_params(r.cfg), r.theta0)
+        """Start ``r`` in ``slot``, or resume it from its snapshot."""
+        if r.snapshot is not None:
+            self.eng.restore(slot, r.snapshot)
+            self.t[slot] = r.t
+            r.snapshot = None
+        else:
+            self.eng.admit(slot, run_params(r.cfg), r.theta0)
+            self.t[slot] = 0.0
+            r.admitted = now
         self.req[slot] = r
-        self.t[slot] = 0.0
         self.tstop[slot] = r.tstop
-        self.live[slot] = r.tstop > 0.0
-        r.admitted, r.bucket, r.slot = now, self.key, slot
+        self.live[slot] = self.t[slot] < r.tstop
+        r.bucket, r.slot = self.key, slot
+
+    def victim(self, r):
+        """The slot of the lowest-priority running request below ``r``'s
+        priority (None if there is none)."""
+        best = None
+        for i, q in enumerate(self.req):
+            if q is not None and self.live[i] and q.priority < r.priority:
+                if best is None or q.priority < self.req[best].priority:
+                    best = i
+        return best
+
+    def preempt(self, slot):
+        """Save the run in ``slot`` as a device snapshot and free the slot;
+        -> its request (to be queued again)."""
+        r = self.req[slot]
+        r.snapshot = self.eng.snapshot(slot)
+        r.t = float(self.t[slot])
+        r.preemptions += 1
+        r.bucket = r.slot = None
+        self.req[slot] = None
+        self.live[slot] = False
+        return r
 
     def step(self, safety, dtmax):
         """One step of every live slot; -> run-steps taken."""
@@ -200,7 +236,9 @@ class Server:
         self.t_end = None
         self.concurrent = bool(concurrent)
         self._finished = queue.Queue()
+        self._requeue = queue.Queue()
         self._steps_lock = threading.Lock()
+        self.n_preempted = 0
 
     # ---------------------------------------------------------------- input
     def submit(self, requests):
@@ -222,7 +260,7 @@ class Server:
         if b is not None:
             if self.concurrent:
                 return b            # its worker admits from the inbox as slots free
-            return b if b.free_slots() else None
+            return b if (b.free_slots() or b.victim(r) is not None) else None
         per = run_bytes(key[0], key[1], key[2], self.precision)
         room = self.mem_budget - self.used_bytes()
         if room < per:
@@ -262,6 +300,12 @@ class Server:
                 while not b.stop:
                     while b.inbox and b.free_slots():
                         b.admit(b.free_slots()[0], b.inbox.popleft(), self.now())
+                    while b.inbox and not b.free_slots():
+                        v = b.victim(b.inbox[0])
+                        if v is None:
+                            break
+                        self._requeue.put((b, b.preempt(v)))
+                        b.admit(v, b.inbox.popleft(), self.now())
                     if not np.any(b.live) and not b.busy():
                         time.sleep(0.001)
                         continue
@@ -287,8 +331,18 @@ class Server:
             b.assigned += 1
             if self.concurrent:
                 b.inbox.append(r)
+                continue
+            free = b.free_slots()
+            if free:
+                b.admit(free[0], r, now)
             else:
-                b.admit(b.free_slots()[0], r, now)
+                # a full bucket and a running request of lower priority
+                v = b.victim(r)
+                q = b.preempt(v)
+                b.assigned -= 1
+                self.n_preempted += 1
+                b.admit(v, r, now)
+                waiting.append(q)
         while waiting:
             self.pending.appendleft(waiting.pop())
 
@@ -355,6 +409,17 @@ class Server:
         got = False
         while True:
             try:
+                b, r = self._requeue.get_nowait()
+            except queue.Empty:
+                break
+            # preempted by the bucket's worker: queued again ahead of later
+            # arrivals, resumed from its snapshot
+            b.assigned -= 1
+            self.n_preempted += 1
+            self.pending.appendleft(r)
+            got = True
+        while True:
+            try:
                 b, r = self._finished.get(timeout=0.005 if (wait and not got) else 0.0)
             except queue.Empty:
                 break
@@ -378,6 +443,7 @@ class Server:
         lat = np.array([r.latency for r in self.done if r.latency is not None])
         wall = self.t_end if self.t_end is not None else self.now()
         return dict(requests=len(self.done), run_steps=int(self.run_steps), wall_s=wall,
+                    preempted=self.n_preempted,
                     steps_per_s=self.run_steps / wall if wall > 0 else None,
                     p50_s=float(np.percentile(lat, 50)) if lat.size else None,
                     p99_s=float(np.percentile(lat, 99)) if lat.size else None,
