@@ -116,12 +116,14 @@ def _counter_means(path, probe_mib=None):
         probe = [r for r in rows if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer")]
         big = max(probe, key=lambda r: float(r["Counter_Value"]))
         scale = probe_mib * 1024.0 / float(big["Counter_Value"])
+    # one row per (dispatch, counter): the counters of a pass are summed per
+    # dispatch (an FP64 and an FP32 MFMA counter of one kernel add up)
     for r in rows:
         k = _short(r["Kernel_Name"])
-        a = acc.setdefault(k, [0.0, 0])
+        a = acc.setdefault(k, [0.0, set()])
         a[0] += float(r["Counter_Value"])
-        a[1] += 1
-    return {k: v[0] / v[1] for k, v in acc.items()}, scale
+        a[1].add(r.get("Dispatch_Id", len(a[1])))
+    return {k: v[0] / len(v[1]) for k, v in acc.items()}, scale
 
 
 def letters_md(stats_csv, fetch_csv, write_csv, probe_mib, title, command, mfma_csv=None):
