@@ -261,6 +261,13 @@ int smx_eta_drift(int prec, int B, long n_per_run, const void *eta, const void *
  * Picard loop in RBCCoupled._defect_solve, coupled_model.py:543-566). */
 int smx_rownorm(int prec, int B, long n, long ld, const void *x, double *out,
                 const int *active, void *stream);
+/* out[b*ldo] = sum_i w[i] x[b*ldx + i] y[b*ldy + i] (y may be NULL), FP64
+ * accumulation, for i < len: the quadratures of the energy budget -- the
+ * volume mean <theta> (heating.py:275-276), <Phi> of the tau-zeroed
+ * dissipation field (dissipation.py:384-392) and Ra <u_z theta> on the
+ * padded grid (dissipation.py:423-444). */
+int smx_wdot(int prec, int B, long len, const void *x, long ldx, const void *y, long ldy,
+             const void *w, double *out, long ldo, const int *active, void *stream);
 /* Elementwise / small kernels of the step.
  * axpby: out = (a*ab[b]) x + (b*bb[b]) y per run (ab, bb, y may be NULL).
  * rowscale: out[b][k][j] = (alpha*ab[b]) ks[k] x[b][k][j], tau entries zeroed

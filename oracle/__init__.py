@@ -14,7 +14,9 @@ of the reference package ``specmc`` for the stepper behind
   heating.py:168-230, dissipation.py:234-487);
 * ``oracle.models``    -- the isoviscous ``RBC`` step, the BDF2 implicit theta
   step, the depth-only and the coupled eta(T) Stokes solves (BFBT route with
-  the legacy / direct rescue), CFL and diagnostics.
+  the legacy / direct rescue, the legacy lagged-LU route), CFL and diagnostics;
+* ``oracle.budget``    -- the energy budget (heating.py:269-306,
+  dissipation.py:365-571).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
 may import this package, and only as the checker or the timed CPU baseline.  The
@@ -23,8 +25,8 @@ product package ``specmc_amd`` never imports it.
 Parity status of every function is listed in the repository's design notes;
 each function here cites the reference file:line it follows.
 """
-from . import spectral, krylov, laws, models  # noqa: F401
+from . import spectral, krylov, laws, models, budget  # noqa: F401
 from .models import build_oracle, march_adaptive, march  # noqa: F401
 
-__all__ = ["spectral", "krylov", "laws", "models", "build_oracle",
+__all__ = ["spectral", "krylov", "laws", "models", "budget", "build_oracle",
            "march_adaptive", "march"]

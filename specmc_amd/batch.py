@@ -80,5 +80,10 @@ class Ensemble:
     def states(self):
         return self.eng.get(self.eng.theta)
 
+    def budgets(self):
+        """Energy budget of every run's last step, on the device (see
+        ``Engine.energy_budget``); -> dict of (B,) arrays."""
+        return self.eng.energy_budget()
+
     def diagnostics(self):
         return self.eng.diagnostics()

@@ -2326,6 +2326,45 @@ extern "C" int smx_rownorm(int prec, int B, long n, long ld, const void *x, doub
   return 0;
 }
 
+// per-run weighted sums out[b*ldo] = sum_i w[i] x[b*ldx + i] (y[b*ldy + i]),
+// accumulated in FP64: quadratures of the energy budget (volume means of
+// theta, of the dissipation field and of u_z theta)
+template <typename T>
+__global__ void __launch_bounds__(1024) k_wdot(long len, const T *x, long ldx, const T *y,
+                                               long ldy, const T *w, double *out, long ldo,
+                                               const int *active) {
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  __shared__ double red[16];
+  double s = 0.0;
+  const T *xb = x + (long)b * ldx;
+  const T *yb = y ? y + (long)b * ldy : nullptr;
+  for (long e = threadIdx.x; e < len; e += blockDim.x) {
+    double v = (double)w[e] * (double)xb[e];
+    if (yb) v *= (double)yb[e];
+    s += v;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[(long)b * ldo] = s;
+}
+extern "C" int smx_wdot(int prec, int B, long len, const void *x, long ldx, const void *y,
+                        long ldy, const void *w, double *out, long ldo, const int *active,
+                        void *stream) {
+  if (B <= 0) return 0;
+  if (len <= 0 || ldx < len || (y && ldy < len) || ldo < 1)
+    return fail_msg("smx_wdot: need 0 < len <= ldx (and <= ldy), ldo >= 1");
+  if (prec == 0)
+    hipLaunchKernelGGL(k_wdot<double>, dim3(B), dim3(1024), 0, STREAM(stream), len,
+                       (const double *)x, ldx, (const double *)y, ldy, (const double *)w, out,
+                       ldo, active);
+  else
+    hipLaunchKernelGGL(k_wdot<float>, dim3(B), dim3(1024), 0, STREAM(stream), len,
+                       (const float *)x, ldx, (const float *)y, ldy, (const float *)w, out,
+                       ldo, active);
+  CHECK_LAUNCH("smx_wdot");
+  return 0;
+}
+
 // dt = min(dtmax, clip(safety * h / max(umax, 1e-12), dtmin, dtmax));
 // params per run: safety, h, dtmin, dtmax
 __global__ void k_cfl(int B, const double *umax, const double *params, double *dt,

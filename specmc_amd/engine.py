@@ -325,6 +325,7 @@ class Engine:
                                    defect_tol, max_K, K)
             else:
                 raise ValueError("unknown engine kind %r" % (kind,))
+        self._init_budget()
 
     # ------------------------------------------------------------- plumbing
     def p(self, t, off=0):
@@ -522,6 +523,88 @@ class Engine:
                 self.eta_lag[i] = d(snap["eta_lag"]).to(self.dtype)
                 self.lag_have[i] = True
 
+    # --------------------------------------------------------- energy budget
+    def _init_budget(self):
+        """Quadrature weights of the energy budget (heating.py:269-306,
+        dissipation.py:380-444, 540-571): <f> = vol_mean(grid(f_hat)) =
+        (wq V) . f_hat[k=0]; on the padded half grid the x-mean weights are
+        the k = 0 row of the analysis matrix."""
+        o, N, dev = self.ops, self.Nz, self.dev
+        c0 = o.wq @ o.V                       # <f> from the k = 0 coefficients
+        ct = c0.copy()
+        ct[-2:] = 0.0                         # the source's tau rows are zeroed
+        wx = o.Ae_p[0]                        # padded-grid x-mean
+        tt = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=self.dtype, device=dev)
+        self.bud_w = dict(c0=tt(c0), phi=tt(np.outer(wx, o.Vi.T @ ct).ravel()),
+                          pi=tt(np.outer(wx, o.wq).ravel()))
+        # <theta^{n-1}>, <theta^n>, <theta^{n+1}>, Ra <u_z theta>, (Di/Ra) <Phi>
+        self.bud = torch.zeros(self.B, 5, dtype=torch.float64, device=dev)
+        self.bud_abc = np.zeros((self.B, 3))
+
+    def _budget_means(self, prev, cur, abc, active):
+        """Record <theta^{n-1}> (``prev``, or None), <theta^n> and the step's
+        (a0, b0, b1) before the step overwrites the history."""
+        L, s, N, n = self.L, self.stream(), self.Nz, self.n
+        c0 = self.p(self.bud_w["c0"])
+        if prev is not None:
+            L.wdot(self.prec, self.B, N, self.p(prev), n, None, 0, c0, self.p(self.bud[:, 0]),
+                   5, self.p(active), s)
+        L.wdot(self.prec, self.B, N, self.p(cur), n, None, 0, c0, self.p(self.bud[:, 1]), 5,
+               self.p(active), s)
+        act = np.ones(self.B, bool) if active is None else active.cpu().numpy().astype(bool)
+        self.bud_abc[act] = abc[act]
+
+    def energy_budget(self, active=None):
+        """The discrete energy budget of the step just taken, for every
+        active run at once (heating.balance, heating.py:279-306, with the
+        dissipation term of dissipation.balance, dissipation.py:540-571):
+
+            storage  = a0 <theta^{n+1}> - b0 <theta^n> + b1 <theta^{n-1}>
+            residual = storage - (nu_bot - nu_top) - H - (Di/Ra) <Phi>
+
+        with <Phi> of the tau-zeroed source field for the stream function of
+        theta^{n+1} and the eta it consumed (dissipation.py:380-392), and
+        Ra <u_z theta> through the same padded pipeline (dissipation.py:423-444).
+        The advection term is 0 (the exact operator).  -> dict of (B,) arrays."""
+        L, t, s, N, n, nrp = self.L, self.t, self.stream(), self.Nz, self.n, self.nrp
+        B = self.B
+        d = self.diagnostics(active)             # Stokes of theta^{n+1}; Nu pair
+        c0 = self.p(self.bud_w["c0"])
+        L.wdot(self.prec, B, N, self.p(self.theta), n, None, 0, c0, self.p(self.bud[:, 2]), 5,
+               self.p(active), s)
+        # Ra <u_z theta> on the padded grid
+        self._padded_velocity(self.psi, active)
+        self.zrows(self.theta, t["V"], self.zv, N, N, N, N, active=active)
+        self.xleft(t["Ce_p"], nrp, self.Nk, self.zv, N, n, self.gp1, N, nrp * N, N,
+                   active=active)
+        L.wdot(self.prec, B, nrp * N, self.p(self.uz_p), nrp * N, self.p(self.gp1), nrp * N,
+               self.p(self.bud_w["pi"]), self.p(self.bud[:, 3]), 5, self.p(active), s)
+        self.bud[:, 4].zero_()
+        if np.any(self.Dih != 0.0):
+            eta_c = (self.eta_c if self.kind == KIND_COUPLED else
+                     self.eta_c_depth if self.kind == KIND_DEPTH else None)
+            self._phi_grid(self.psi, eta_c, self.gp4, active)
+            L.wdot(self.prec, B, nrp * N, self.p(self.gp4), nrp * N, None, 0,
+                   self.p(self.bud_w["phi"]), self.p(self.bud[:, 4]), 5, self.p(active), s)
+        m = self.bud.cpu().numpy()
+        a0, b0, b1 = self.bud_abc.T
+        storage = a0 * m[:, 2] - b0 * m[:, 1] + b1 * m[:, 0]
+        nu_top, nu_bot = d[:, 0], d[:, 1]
+        H = self.Hh.astype(float)
+        diss = m[:, 4]
+        Ra = np.array([float(p.get("Ra", 1e4)) for p in self.params])
+        out = dict(storage=storage, nu_bot=nu_bot, nu_top=nu_top, heating=H,
+                   advection=np.zeros(B), dissipation=diss,
+                   residual=storage - (nu_bot - nu_top) - H - diss,
+                   theta_mean=m[:, 2], T_mean=0.5 + m[:, 2], pi_theta=Ra * m[:, 3],
+                   phi_integral=np.where(self.Dih != 0.0, diss * Ra / np.where(
+                       self.Dih != 0.0, self.Dih, 1.0), 0.0))
+        if active is not None:
+            act = active.cpu().numpy().astype(bool)
+            for k in out:
+                out[k] = np.where(act, out[k], np.nan)
+        return out
+
     # ------------------------------------------------------- shared pieces
     def project(self, src, dst, active=None):
         """Vi (Re V src): project_mirror on real fields (convection_modes.py:134-153)."""
@@ -680,6 +763,8 @@ class Engine:
             L.lu_factor(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv), Nk,
                         self.p(mask), s)
             self._lam_fact[refac] = inv_dt[refac]
+        self._budget_means(None, self.theta, np.stack([inv_dt, inv_dt, np.zeros(B)], axis=1),
+                           active)
         L.axpby(self.prec, n, B, 1.0, self.p(self.lam), self.p(self.theta), 1.0, None,
                 self.p(self.nc), self.p(self.s1), self.p(active), s)
         L.lu_solve(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv), self.p(self.s1),
@@ -1552,6 +1637,8 @@ class Engine:
 
         sv.solve(precond, op, active, key="theta")
         self.last_theta = sv.results()
+        self._budget_means(self.theta_prev if np.any(self.has_hist) else None, th,
+                           np.stack([a0, b0, b1], axis=1), active)
         self.project(sv.x, self.theta, active)           # output projected
         # history (theta^{n-1} = th, dt_prev, psi^{n-1} = psi_n)
         if active is None:

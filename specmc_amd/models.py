@@ -198,6 +198,15 @@ class _DeviceModel:
     def reset_history(self):
         self.eng.reset_history()
 
+    def energy_budget(self):
+        """The discrete energy budget of the last step, evaluated on the
+        device from the model's own states (heating.balance with the
+        dissipation term of dissipation.balance, heating.py:279-306,
+        dissipation.py:540-571), plus Ra <u_z theta> (dissipation.py:423-444).
+        -> dict of floats: storage, nu_bot, nu_top, heating, advection,
+        dissipation, residual, theta_mean, T_mean, pi_theta, phi_integral."""
+        return {k: float(v[0]) for k, v in self.eng.energy_budget().items()}
+
     def project_mirror(self, that):
         X = (self.V @ np.asarray(that).T).T
         return (self.Vi @ X.real.T).T

@@ -38,6 +38,7 @@ SIGNATURES = {
     "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P, P],
     "smx_eta_drift": [I, I, L, P, P, P, P, P],
     "smx_rownorm": [I, I, L, L, P, P, P, P],
+    "smx_wdot": [I, I, L, P, L, P, L, P, P, L, P, P],
     "smx_inv_apply": [I, I, I, P, P, P, I, D, P, P, I, I, P, P],
     "smx_tau_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P],
     "smx_cheb_mult": [I, I, I, P, P, P, P],
