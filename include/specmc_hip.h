@@ -202,13 +202,15 @@ int smx_schur_tau(int prec, int Nk, int N, const void *V0, const void *V1,
 
 /* The viscosity law on the padded half grid (coupled_model.py:333-399):
  * T = 1 - z + theta, counters of T outside [0,1] / [0,2], T clipped to
- * [0,2], eta = exp(-ln(deta) T) (law 0) or exp(-ln(deta)(1+Ts) T/(T+Ts))
- * (law 1), then the window [lo, hi] with the 1e-12 dead-band
- * (eta_bounds.py:360-379) or the C^2 smooth clip (eta_bounds.py:411-480).
- * One block per run; grid rows carry mirror weights (row 0 and row nrow-1
- * count once, the others twice).  law params per run: [ln_deta, Ts, lo, hi,
- * smooth_w, law]; counters per run (double[4]): n_T_out, n_clip_T,
- * n_clip_eta, max_T_excursion (accumulated). */
+ * [0,2], eta = exp(-ln(deta) T) (law 0), exp(-ln(deta)(1+Ts) T/(T+Ts))
+ * (law 1) or the pressure-dependent exp(-E T/(Ts (T+Ts)) + V (1-z)/(T+Ts))
+ * (law 2, arrhenius.py:283-325), then the window [lo, hi] with the 1e-12
+ * dead-band (eta_bounds.py:360-379) or the C^2 smooth clip
+ * (eta_bounds.py:411-480).  One block per run; grid rows carry mirror
+ * weights (row 0 and row nrow-1 count once, the others twice).  law params
+ * per run (double[8]): [ln_deta, Ts, lo, hi, smooth_w, law, E, V]; counters
+ * per run (double[4]): n_T_out, n_clip_T, n_clip_eta, max_T_excursion
+ * (accumulated). */
 int smx_eta_law(int prec, int B, int nrow, int N, const void *theta,
                 const void *z, const double *params, void *eta,
                 double *counters, const int *active, void *stream);
@@ -219,7 +221,7 @@ int smx_eta_law(int prec, int B, int nrow, int N, const void *theta,
  * (profile 0, 'harm'), mean_x(eta) (1, 'diag') or exp(mean_x(log eta))
  * (2, 'geom') over the full Nx grid. */
 int smx_eta_guard_profile(int prec, int B, int Nx, int N, void *eta,
-                          const double *params, void *prof, int profile,
+                          const double *params /* double[8] per run, as above */, void *prof, int profile,
                           double *counters, const int *active, void *stream);
 
 /* Pseudo-spectral eta product of the coupled Schur operator

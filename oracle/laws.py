@@ -30,6 +30,32 @@ def arrhenius_exponent(T, deta_T, Ts):
     return -L * (1.0 + s) * T / (T + s)
 
 
+# reference-state constants of the pV calibration (reference_state.py:142-177):
+# rho0, g, d, DeltaT, T_surf, V_a, E_a, R
+PV_CONSTANTS = dict(rho0=3300.0, g=9.81, d=2.89e6, DeltaT=3000.0, T_surf=273.0, V_a=6e-6,
+                    E_a=5.3e5, R=8.314462618)
+
+
+def calibrate_pv(anchor, V, deta_T=None, c=PV_CONSTANTS):
+    """(E, V, Ts) of the pV law (reference_state.py:277-357): Ts = T_surf /
+    DeltaT; anchor 'Ea': E = E_a / (R DeltaT); anchor 'D': E = ln(deta_T)
+    Ts (1 + Ts) + Ts V."""
+    Ts = c["T_surf"] / c["DeltaT"]
+    if anchor == "Ea":
+        E = c["E_a"] / (c["R"] * c["DeltaT"])
+    else:
+        E = float(np.log(float(deta_T))) * Ts * (1.0 + Ts) + Ts * float(V)
+    return float(E), float(V), float(Ts)
+
+
+def arrhenius_exponent_V(T, p0n, E, V, Ts):
+    """ln eta of the pV law, -E T / (Ts (T + Ts)) + V p0n / (T + Ts)
+    (arrhenius.py:283-325), with p0n = 1 - z (reference_state.py:208-216)."""
+    T = np.asarray(T, dtype=float)
+    s = float(Ts)
+    return -float(E) * T / (s * (T + s)) + float(V) * np.asarray(p0n, dtype=float) / (T + s)
+
+
 def effective_bounds(lo, hi):
     """Dead-band window (eta_bounds.py:238-241)."""
     return float(lo) * (1.0 - CLIP_RTOL), float(hi) * (1.0 + CLIP_RTOL)

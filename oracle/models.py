@@ -571,15 +571,23 @@ class OracleCoupled(OracleDepth):
                  arrhenius_Ts=0.0, precond="legacy", precond_profile="harm",
                  lag=16, K=3, defect_tol=1e-8, max_K=8, stokes_rtol=1e-8,
                  eta_drift_tol=0.1, stokes_restart=80, stokes_maxiter=800,
-                 theta_rtol=1e-10, symmetrize=True, stokes_mode="defect"):
+                 theta_rtol=1e-10, symmetrize=True, stokes_mode="defect",
+                 arrhenius_V=0.0, reference_anchor="D"):
         super().__init__(Ra=Ra, Nx=Nx, Nz=Nz, Lx=Lx, deta_T=deta_T,
                          internal_heating=internal_heating,
                          dissipation_number=dissipation_number,
                          eta_min=eta_min, eta_max=eta_max,
                          eta_smooth=eta_smooth, theta_rtol=theta_rtol,
                          symmetrize=symmetrize)
-        self.kind = "arrhenius" if float(arrhenius_Ts) > 0.0 else "eta_T"
+        self.kind = ("arrhenius" if float(arrhenius_Ts) > 0.0 or float(arrhenius_V) > 0.0
+                     else "eta_T")
         self.arrhenius_Ts = float(arrhenius_Ts)
+        # the pV law (coupled_model.py:186-229): Ts and E from the calibration
+        self.pv = None
+        if float(arrhenius_V) > 0.0:
+            self.pv = laws.calibrate_pv(str(reference_anchor), float(arrhenius_V),
+                                        deta_T=deta_T)
+            self.arrhenius_Ts = self.pv[2]
         self.precond = str(precond).lower()
         self.precond_profile = str(precond_profile)
         self.stokes_mode = str(stokes_mode)
@@ -625,7 +633,11 @@ class OracleCoupled(OracleDepth):
         self.n_T_out_of_range += int(np.count_nonzero((T < 0.0) | (T > 1.0)))
         self.n_clip_T += int(np.count_nonzero((T < 0.0) | (T > 2.0)))
         Tc = np.clip(T, 0.0, 2.0)
-        if self.arrhenius_Ts > 0.0:
+        if self.pv is not None:
+            # coupled_model.py:369-375: p0_norm = 1 - z on the rows of T
+            E, V, Ts = self.pv
+            eta = np.exp(laws.arrhenius_exponent_V(Tc, 1.0 - self.z[None, :], E, V, Ts))
+        elif self.arrhenius_Ts > 0.0:
             eta = np.exp(laws.arrhenius_exponent(Tc, self.deta_T,
                                                  self.arrhenius_Ts))
         else:
@@ -835,6 +847,8 @@ def build_oracle(cfg):
     dtol = float(c.get("defect_tol", 1e-8))
     return OracleCoupled(**common, **win,
                          arrhenius_Ts=float(c.get("arrhenius_Ts", 0.0)),
+                         arrhenius_V=float(c.get("arrhenius_V", 0.0)),
+                         reference_anchor=str(c.get("reference_anchor", "D")),
                          precond=str(c.get("precond", "legacy")).strip().lower(),
                          lag=int(c.get("lag", 16)), K=int(c.get("K", 3)),
                          defect_tol=(None if dtol <= 0.0 else dtol),

@@ -58,7 +58,8 @@ def _kwargs(cfg, vm):
                 symmetrize=bool(cfg.symmetrize))
     if vm == "eta_z":
         return base
-    base.update(arrhenius_Ts=float(cfg.arrhenius_Ts),
+    base.update(arrhenius_Ts=float(cfg.arrhenius_Ts), arrhenius_V=float(cfg.arrhenius_V),
+                reference_anchor=str(cfg.reference_anchor),
                 precond=str(cfg.precond).strip().lower(), lag=int(cfg.lag),
                 K=int(cfg.K), max_K=int(cfg.max_K),
                 defect_tol=(None if float(cfg.defect_tol) <= 0.0 else float(cfg.defect_tol)))

@@ -21,6 +21,7 @@ def run_params(cfg):
     """Per-run physics and Stokes route of a ConvectionConfig as the engine's
     parameter dict."""
     return dict(Ra=float(cfg.Ra), deta_T=float(cfg.deta_T), Ts=float(cfg.arrhenius_Ts),
+                arrhenius_V=float(cfg.arrhenius_V), reference_anchor=str(cfg.reference_anchor),
                 eta_min=float(cfg.eta_min), eta_max=float(cfg.eta_max),
                 eta_smooth=float(cfg.eta_smooth),
                 internal_heating=float(cfg.internal_heating),

@@ -2,8 +2,7 @@
 reference), with the same fields, defaults, aliases and error behaviour.
 
 Fields whose physics this build does not carry raise ``NotImplementedError``
-with a message containing "not implemented": the pressure-dependent (pV)
-Arrhenius branch (``arrhenius_V > 0``) and the TALA / ALA compressibility
+with a message containing "not implemented": the TALA / ALA compressibility
 models, in addition to the reference's own reserved fields.
 """
 from __future__ import annotations
@@ -64,7 +63,19 @@ def arrhenius_E(deta_T, Ts):
     return float(math.log(D) * s * (1.0 + s))
 
 
+def _pv_record(cfg):
+    from .reference_state import REFERENCE_DEFAULTS, calibrate_pv
+    a = str(cfg.reference_anchor)
+    return calibrate_pv(REFERENCE_DEFAULTS, anchor=a, V=float(cfg.arrhenius_V),
+                        deta_T=(float(cfg.deta_T) if a == "D" else None))
+
+
 def arrhenius_effective_E(cfg):
+    """The activation E the config's law uses (config.py:210-236): the
+    calibrated E-tilde of the pV law when arrhenius_V > 0, else
+    ln(deta_T) Ts (1 + Ts), or 0 when the Arrhenius law is off."""
+    if float(cfg.arrhenius_V) > 0.0:
+        return float(_pv_record(cfg)["E"])
     Ts = float(cfg.arrhenius_Ts)
     return 0.0 if Ts <= 0.0 else arrhenius_E(cfg.deta_T, Ts)
 
@@ -139,16 +150,32 @@ class ConvectionConfig:
         if V < 0.0:
             raise ValueError("arrhenius_V cannot be negative (got %r)" % (self.arrhenius_V,))
         if V > 0.0:
-            raise _not_impl("arrhenius_V=%r (the pressure-dependent pV Arrhenius law)"
-                            % (self.arrhenius_V,))
-        if str(self.reference_anchor) != "D":
+            # the pV law: Ts and E are the calibration's; nonzero echoes must
+            # agree with it (config.py:533-565)
+            from .reference_state import PV_ANCHORS
+            if vm != "arrhenius":
+                raise ValueError("arrhenius_V=%r selects the pV(L) Arrhenius law, which "
+                                 "needs viscosity_model='arrhenius' (got %r)"
+                                 % (self.arrhenius_V, self.viscosity_model))
+            if str(self.reference_anchor) not in PV_ANCHORS:
+                raise ValueError("unknown reference_anchor %r; use one of %r"
+                                 % (self.reference_anchor, PV_ANCHORS))
+            rec = _pv_record(self)
+            if Ts != 0.0 and abs(Ts - rec["Ts"]) > 1e-12 * rec["Ts"]:
+                raise ValueError("arrhenius_Ts=%r contradicts the reference-state value "
+                                 "Ts = T_surf/DeltaT = %r; under pV(L) Ts is fixed by the "
+                                 "constants (leave it 0.0)" % (self.arrhenius_Ts, rec["Ts"]))
+            if E != 0.0 and abs(E - rec["E"]) > 1e-12 * abs(rec["E"]):
+                raise ValueError("arrhenius_E=%r contradicts the calibrated pV activation "
+                                 "E-tilde = %r (leave it 0.0)" % (self.arrhenius_E, rec["E"]))
+        elif str(self.reference_anchor) != "D":
             raise ValueError("reference_anchor=%r has no effect while arrhenius_V == 0, "
                              "so it must stay at its default 'D'" % (self.reference_anchor,))
-        if Ts == 0.0 and E != 0.0:
+        elif Ts == 0.0 and E != 0.0:
             raise ValueError("arrhenius_E=%r is set but arrhenius_Ts is 0, which means "
                              "'Arrhenius off'; the activation would be silently ignored"
                              % (self.arrhenius_E,))
-        if Ts > 0.0:
+        if Ts > 0.0 and V == 0.0:
             if vm != "arrhenius":
                 raise ValueError("arrhenius_Ts=%r selects the Arrhenius law, which needs "
                                  "viscosity_model='arrhenius' (got %r)"

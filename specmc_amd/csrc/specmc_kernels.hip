@@ -1494,6 +1494,10 @@ __device__ __forceinline__ double smooth5(double t) {
   return t * t * t * (t * (6.0 * t - 15.0) + 10.0);
 }
 
+// doubles per run in the viscosity-law parameter rows: ln deta_T, Ts, eta_min,
+// eta_max, eta_smooth, law (0 exp, 1 Arrhenius, 2 pV Arrhenius), E, V
+#define LAW_ROW 8
+
 template <typename T>
 __global__ void __launch_bounds__(1024) k_eta_law(int nrow, int N, const T *theta, const T *z,
                                                   const double *params, T *eta, double *counters,
@@ -1501,7 +1505,7 @@ __global__ void __launch_bounds__(1024) k_eta_law(int nrow, int N, const T *thet
   const int b = blockIdx.x;
   if (active && !active[b]) return;
   __shared__ double red[16];
-  const double *pp = params + 6 * b;
+  const double *pp = params + LAW_ROW * b;
   const double L = pp[0], Ts = pp[1], lo = pp[2], hi = pp[3], w = pp[4];
   const int law = (int)pp[5];
   const double elo = lo * (1.0 - 1e-12), ehi = hi * (1.0 + 1e-12);
@@ -1518,8 +1522,12 @@ __global__ void __launch_bounds__(1024) k_eta_law(int nrow, int N, const T *thet
     if (T_ < 0.0 || T_ > 2.0) n_clipT += wgt;
     double Tc = fmin(fmax(T_, 0.0), 2.0);
     double v;
-    if (law == 1) v = exp(-L * (1.0 + Ts) * Tc / (Tc + Ts));
-    else v = exp(-L * Tc);
+    if (law == 2)        // pV: -E T / (Ts (T + Ts)) + V (1 - z) / (T + Ts)
+      v = exp(-pp[6] * Tc / (Ts * (Tc + Ts)) + pp[7] * (1.0 - (double)z[q]) / (Tc + Ts));
+    else if (law == 1)
+      v = exp(-L * (1.0 + Ts) * Tc / (Tc + Ts));
+    else
+      v = exp(-L * Tc);
     if (v < elo || v > ehi) n_clipE += wgt;
     et[e] = (T)v;
   }
@@ -1576,7 +1584,7 @@ __global__ void __launch_bounds__(1024) k_eta_guard(int Nx, int N, T *eta, const
   const int nrow = Nx / 2 + 1;
   const long np = (long)nrow * N;
   T *et = eta + (long)b * np;
-  const double *pp = params + 6 * b;
+  const double *pp = params + LAW_ROW * b;
   const double lo = pp[2], hi = pp[3], w = pp[4];
   double mn = 1e300;
   for (long e = threadIdx.x; e < np; e += blockDim.x) mn = fmin(mn, (double)et[e]);

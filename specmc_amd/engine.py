@@ -62,13 +62,27 @@ ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
 
 
+LAW_ROW = 8      # doubles per run in the viscosity-law parameter rows
+
+
 def _law_row(p):
     """Per-run viscosity-law parameters as the eta kernels read them:
-    (ln deta_T, Ts, eta_min, eta_max, eta_smooth, arrhenius flag)."""
+    (ln deta_T, Ts, eta_min, eta_max, eta_smooth, law, E, V) with law 0 the
+    exponential law, 1 the T-only Arrhenius law, 2 the pressure-dependent pV
+    law calibrated by ``reference_state.calibrate_pv`` (Ts, E, V its own;
+    coupled_model.py:186-229, 369-375)."""
     Ts = float(p.get("Ts", 0.0))
+    V = float(p.get("arrhenius_V", 0.0))
+    law, E = (1.0 if Ts > 0.0 else 0.0), 0.0
+    if V > 0.0:
+        from .reference_state import REFERENCE_DEFAULTS, calibrate_pv
+        a = str(p.get("reference_anchor", "D"))
+        rec = calibrate_pv(REFERENCE_DEFAULTS, anchor=a, V=V,
+                           deta_T=float(p.get("deta_T", 1000.0)) if a == "D" else None)
+        law, Ts, E, V = 2.0, rec["Ts"], rec["E"], rec["V"]
     return np.array([np.log(float(p.get("deta_T", 1000.0))), Ts,
                      float(p.get("eta_min", 1e-3)), float(p.get("eta_max", 1e3)),
-                     float(p.get("eta_smooth", 0.0)), 1.0 if Ts > 0.0 else 0.0])
+                     float(p.get("eta_smooth", 0.0)), law, E, V])
 
 
 def _is_legacy(p):

@@ -50,6 +50,7 @@ class _DeviceModel:
     def __init__(self, Ra=1e4, Nx=64, Nz=48, Lx=2.0, deta_T=1.0,
                  internal_heating=0.0, dissipation_number=0.0,
                  eta_min=1e-3, eta_max=1e3, eta_smooth=0.0, arrhenius_Ts=0.0,
+                 arrhenius_V=0.0, reference_anchor="D",
                  precond="bfbt", symmetrize=True, stokes_maxiter=None,
                  defect_tol=1e-8, max_K=8, K=3, pc_apply=None, ortho=None,
                  precision="fp64", graph_rounds=0, **extra):
@@ -60,6 +61,7 @@ class _DeviceModel:
         self.symmetrize = bool(symmetrize)
         self.extra = dict(extra)
         params = dict(Ra=self.Ra, deta_T=self.deta_T, Ts=float(arrhenius_Ts),
+                      arrhenius_V=float(arrhenius_V), reference_anchor=str(reference_anchor),
                       eta_min=float(eta_min), eta_max=float(eta_max),
                       eta_smooth=float(eta_smooth),
                       internal_heating=self.internal_heating,
