@@ -91,7 +91,9 @@ int smx_set_gemm_mfma(int mode);
  * the FP64 smx_xprod on the matrix cores, 0 on the vector-FMA kernel;
  * smx_set_gj_rows(1) inverts with the row-owner Gauss-Jordan kernel, 0
  * (default) with the first one; smx_set_inv_pairs(1) (default) applies FP64
- * inverses of even order with the row-pair kernel, 0 with the first one. */
+ * inverses of even order with the row-pair kernel that issues all of a
+ * block's loads first, 2 with the row-pair kernel that loads as it
+ * accumulates, 0 with the first one. */
 int smx_set_xprod_mfma(int on);
 int smx_set_gj_rows(int on);
 /* smx_set_gj_reg(1) (default) inverts FP64 blocks of order 16..128 with the

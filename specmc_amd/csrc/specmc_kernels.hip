@@ -300,10 +300,12 @@ extern "C" int smx_set_gj_rows(int on) {
   return 0;
 }
 
-// inverse-apply variant for FP64: 1 row-pair kernel (default for even N), 0 the first
+// inverse-apply variant for FP64 and even N: 1 the row-pair kernel with the
+// whole block's loads issued before the right-hand side is staged (default),
+// 2 the row-pair kernel loading as it accumulates, 0 the first kernel
 static int g_inv_pairs = 1;
 extern "C" int smx_set_inv_pairs(int on) {
-  g_inv_pairs = on ? 1 : 0;
+  g_inv_pairs = (on == 1 || on == 2) ? on : 0;
   return 0;
 }
 
@@ -1128,6 +1130,64 @@ __global__ void __launch_bounds__(256) k_inv_apply_pairs(const double *Inv, cons
   }
 }
 
+// The row-pair apply with every column load of the thread in flight at once:
+// the block's MAXIT double2 loads per thread are issued first, the right-hand
+// side is staged in LDS meanwhile, and the accumulation then runs from
+// registers in the same order as k_inv_apply_pairs (bit-identical results).
+// Needs ceil(N / CG) <= MAXIT with CG = 256 / (N / 2) column groups.
+template <int MAXIT>
+__global__ void __launch_bounds__(256) k_inv_apply_pairs_pf(
+    const double *Inv, const double *rhs, double *x, int N, int tau_zero, double scale,
+    const double *kscale, const double *bscale, int per_run, int shared, const int *active,
+    unsigned long long *units) {
+  __shared__ double bs[LUS_MAXN];
+  __shared__ double part[2 * 256];
+  const int m = blockIdx.x;
+  if (active && !active[m / per_run]) return;
+  if (units && threadIdx.x == 0) atomicAdd(units, 1ull);
+  const long mi = shared ? (m % per_run) : m;
+  const double *A = Inv + mi * (long)N * N;
+  const int tid = threadIdx.x;
+  const int RP = N >> 1, CG = 256 / RP;
+  const int rp = tid % RP, cg = tid / RP;
+  const bool on = cg < CG;
+  double2 v[MAXIT];
+#pragma unroll
+  for (int i = 0; i < MAXIT; ++i) {
+    const int c = cg + i * CG;
+    v[i] = (on && c < N) ? *(const double2 *)(A + 2 * rp + (long)c * N)
+                         : make_double2(0.0, 0.0);
+  }
+  double s = kscale ? scale * kscale[m % per_run] : scale;
+  if (bscale) s = s * bscale[m / per_run];
+  for (int r = tid; r < N; r += 256) {
+    double u = rhs[(long)m * N + r] * s;
+    if (tau_zero && r >= N - 2) u = 0.0;
+    bs[r] = u;
+  }
+  __syncthreads();
+  if (on) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXIT; ++i) {
+      const int c = cg + i * CG;
+      if (c < N) {
+        const double b = bs[c];
+        a0 = fma(v[i].x, b, a0);
+        a1 = fma(v[i].y, b, a1);
+      }
+    }
+    part[cg * N + 2 * rp] = a0;
+    part[cg * N + 2 * rp + 1] = a1;
+  }
+  __syncthreads();
+  for (int r = tid; r < N; r += 256) {
+    double acc = 0.0;
+    for (int g = 0; g < CG; ++g) acc += part[g * N + r];
+    x[(long)m * N + r] = acc;
+  }
+}
+
 extern "C" int smx_lu_factor(int prec, int nmat, int N, void *LU, int *piv,
                              int per_run, const int *active, void *stream) {
   if (nmat <= 0) return 0;
@@ -1304,7 +1364,15 @@ extern "C" int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const v
   if (N <= 0 || N > LUS_MAXN || per_run <= 0) return fail_msg("smx_inv_apply: bad sizes");
   const bool clk = clk_on(2);
   if (clk) clk_begin(STREAM(stream));
-  if (prec == 0 && (N % 2) == 0 && N >= 4 && g_inv_pairs)
+  const bool pairs = prec == 0 && (N % 2) == 0 && N >= 4 && N <= 512 && g_inv_pairs;
+  const int cgs = pairs ? 256 / (N / 2) : 1;
+  const int iters = pairs && cgs > 0 ? (N + cgs - 1) / cgs : 1 << 30;
+  if (pairs && g_inv_pairs == 1 && cgs > 0 && iters <= 24)
+    hipLaunchKernelGGL(k_inv_apply_pairs_pf<24>, dim3(nmat), dim3(256), 0, STREAM(stream),
+                       (const double *)Inv, (const double *)rhs, (double *)x, N, tau_zero,
+                       rhs_scale, (const double *)kscale, (const double *)bscale, per_run,
+                       shared, active, clk_units(2));
+  else if (pairs && cgs > 0)
     hipLaunchKernelGGL(k_inv_apply_pairs, dim3(nmat), dim3(256), 0, STREAM(stream),
                        (const double *)Inv, (const double *)rhs, (double *)x, N, tau_zero,
                        rhs_scale, (const double *)kscale, (const double *)bscale, per_run,

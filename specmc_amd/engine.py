@@ -1011,10 +1011,12 @@ class Engine:
         # truncation to Nz Chebyshev modes, then D2 t0, D2 t1, D t3
         self.gemm(B * Nk * 4, N, Q, W.tz, Q, 1, t["Tp"], 1, Q, W.tt, N, 1,
                   active=active, mask_mode=1, rpr=Nk * 4)
-        # one GEMM over all B*Nk mode rows per field (rows stride 4N in, 3N out)
-        for f_in, f_out, Mat in ((0, 0, "D2"), (1, 1, "D2"), (3, 2, "D")):
-            self.gemm(B * Nk, N, N, W.tt, 4 * N, 1, t[Mat], 1, N, W.dtt, 3 * N, 1,
-                      aoff=f_in * N, coff=f_out * N, active=active, mask_mode=1, rpr=Nk)
+        # GEMMs over all B*Nk mode rows (rows stride 4N in, 3N out): D2 t0 and
+        # D2 t1 as one launch batched over the field offset, then D t3
+        self.gemm(B * Nk, N, N, W.tt, 4 * N, 1, t["D2"], 1, N, W.dtt, 3 * N, 1,
+                  nz1=2, sa=(N, 0), sc=(N, 0), active=active, mask_mode=1, rpr=Nk)
+        self.gemm(B * Nk, N, N, W.tt, 4 * N, 1, t["D"], 1, N, W.dtt, 3 * N, 1,
+                  aoff=3 * N, coff=2 * N, active=active, mask_mode=1, rpr=Nk)
         L.stokes_combine(self.prec, B, Nk, N, self.p(W.tt), self.p(W.dtt), self.p(t["kx"]),
                          self.p(v), self.p(t["V0"]), self.p(t["V1"]), self.p(out),
                          self.p(active), s)

@@ -110,7 +110,7 @@ def test_inv_apply_pairs(lib, N):
     dI = torch.tensor(np.ascontiguousarray(Inv.transpose(0, 2, 1)), device=dev)
     db, dk, dbs = (torch.tensor(a, device=dev) for a in (b, ks, bsc))
     got = []
-    for on in (1, 0):
+    for on in (1, 0, 2):
         lib.set_inv_pairs(on)
         x = torch.zeros(nm, N, dtype=torch.float64, device=dev)
         lib.inv_apply(0, nm, N, dI.data_ptr(), db.data_ptr(), x.data_ptr(), 1, -0.5,
@@ -125,3 +125,5 @@ def test_inv_apply_pairs(lib, N):
     print("inv_apply N=%d: pairs %.2e, first %.2e" % (N, rel(got[0], want), rel(got[1], want)),
           flush=True)
     assert rel(got[0], want) < 1e-13 and rel(got[1], want) < 1e-13
+    # the prefetching row-pair kernel accumulates in the same order
+    assert np.array_equal(got[0], got[2])
