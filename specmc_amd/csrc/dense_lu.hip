@@ -340,6 +340,25 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
 // wavefront-serial solve of the diagonal block staged in LDS.  T is the
 // factor's type and the arithmetic type; TV the type of b and x.
 // ---------------------------------------------------------------------------
+// acc[u] += M[row u, j0:j1) . y[j0:j1) for four rows, FP32 factor: each lane
+// takes four consecutive columns with one 16-byte load per row (j0, j1 and the
+// row starts multiples of 4 floats), so a wavefront keeps four times the
+// bytes in flight of a column-per-lane loop
+__device__ __forceinline__ void lud_rowdots4(const float *const rows[4], const float *y, int j0,
+                                             int j1, int lane, float acc[4]) {
+  for (int j = j0 + 4 * lane; j < j1; j += 256) {
+    const float y0 = y[j], y1 = y[j + 1], y2 = y[j + 2], y3 = y[j + 3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 a = *(const float4 *)(rows[u] + j);
+      acc[u] = fma(a.x, y0, acc[u]);
+      acc[u] = fma(a.y, y1, acc[u]);
+      acc[u] = fma(a.z, y2, acc[u]);
+      acc[u] = fma(a.w, y3, acc[u]);
+    }
+  }
+}
+
 template <typename T, typename TV>
 __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
                                                     const int *piv, const TV *b, TV *x,
@@ -350,6 +369,10 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
   const int q = blockIdx.x;
   if (active && !active[q]) return;
   const T *M = A + (long)q * ms;
+  // FP32 factors with rows of a multiple of 4 floats (and 16-byte aligned
+  // matrices) take the 16-byte row loads
+  const bool vec4 = sizeof(T) == 4 && (n % 4) == 0 && (ms % 4) == 0 &&
+                    (((unsigned long)A) & 15) == 0;
   const int *P = piv + (long)q * n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int i = tid; i < n; i += 1024) y[i] = (T)b[(long)q * n + i];
@@ -373,11 +396,15 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
       const T *rows[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) rows[u] = M + (long)(i0 + min(w + 16 * u, bs - 1)) * n;
-      for (int j = lane; j < i0; j += 64) {
-        const T yj = y[j];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
+      if constexpr (sizeof(T) == 4) {
+        if (vec4) lud_rowdots4(rows, y, 0, i0, lane, acc);
       }
+      if (!vec4)
+        for (int j = lane; j < i0; j += 64) {
+          const T yj = y[j];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
+        }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const T a = wave_sum(acc[u]);
@@ -406,11 +433,15 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
       const T *rows[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) rows[u] = M + (long)(i0 + min(w + 16 * u, bs - 1)) * n;
-      for (int j = j1 + lane; j < n; j += 64) {
-        const T yj = y[j];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
+      if constexpr (sizeof(T) == 4) {
+        if (vec4) lud_rowdots4(rows, y, j1, n, lane, acc);
       }
+      if (!vec4)
+        for (int j = j1 + lane; j < n; j += 64) {
+          const T yj = y[j];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[u] = fma(rows[u][j], yj, acc[u]);
+        }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const T a = wave_sum(acc[u]);
