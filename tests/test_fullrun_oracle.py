@@ -4,10 +4,16 @@ run (tests/golden/oracle_c1_full_64x96.npz, written by
 tests/golden/make_oracle_c1.py).
 
 * CPU: the device trajectory recorded on MI355X by ``bench.py --make-states``
-  (bench_states_c1_64x96.npz) -- step count, per-step Stokes iteration counts
-  and routes, final theta and diagnostics.
+  (bench_states_c1_64x96.npz) -- step count, per-step time, Stokes iteration
+  counts and rescue routes, final theta and diagnostics.
 * GPU: the whole run marched again, FP64 (1e-10 relative on theta, Nu, Vrms)
   and mixed precision (1e-6 on Nu and Vrms, the mixed bar).
+
+Iteration counts: the BFBT solves after the rescue phase take 300-800
+iterations, and their residual crosses 1e-8 so slowly that roundoff can move
+the count.  Two of the 552 recorded steps differ (581 vs 582, 743 vs 747),
+with theta still at 3e-13 of the oracle's, so counts must match exactly on
+all but 1 % of the steps and within 1 % on those.
 """
 import json
 import os
@@ -33,6 +39,16 @@ def _oracle():
     return d
 
 
+def _check_iters(its, oits):
+    its, oits = np.asarray(its), np.asarray(oits)
+    assert its.shape == oits.shape
+    off = np.where(its != oits)[0]
+    print("iteration counts differ at %d of %d steps: %s" % (
+        len(off), len(its), [(int(i), int(its[i]), int(oits[i])) for i in off[:10]]))
+    assert len(off) <= max(1, len(its) // 100)
+    assert np.all(np.abs(its[off] - oits[off]) <= np.ceil(0.01 * oits[off]))
+
+
 def test_recorded_device_trajectory_matches_oracle():
     o = _oracle()
     with np.load(STATES, allow_pickle=False) as z:
@@ -41,7 +57,7 @@ def test_recorded_device_trajectory_matches_oracle():
     olog = o["log"]
     assert len(dev_log) == len(olog) == o["info"]["steps"]
     # per-step Stokes iterations and rescue routes
-    assert np.array_equal(dev_log[:, 2].astype(int), olog[:, 2].astype(int))
+    _check_iters(dev_log[:, 2].astype(int), olog[:, 2].astype(int))
     assert np.array_equal(dev_log[:, 4] > 0, olog[:, 3] > 0)
     np.testing.assert_allclose(dev_log[:, 1], olog[:, 1], rtol=1e-10, atol=0)
     err = rel(np.real(dev_final), np.real(o["final"]))
@@ -79,7 +95,7 @@ def test_device_full_run_matches_oracle(precision, tol):
           flush=True)
     if precision == "fp64":
         assert info["steps"] == o["info"]["steps"]
-        assert its == [int(v) for v in o["log"][:, 2]]
+        _check_iters(its, [int(v) for v in o["log"][:, 2]])
         assert eth < tol
     else:
         # dt follows the velocity, which differs at roundoff; the last step can
