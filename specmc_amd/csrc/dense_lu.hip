@@ -467,6 +467,129 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
   for (int i = tid; i < n; i += 1024) x[(long)q * n + i] = (TV)y[i];
 }
 
+// ---------------------------------------------------------------------------
+// wide solve for a few matrices: the one-workgroup-per-matrix solve streams a
+// whole factor through one CU, so with fewer matrices than CUs the triangular
+// sweeps run block by block over many workgroups instead: x = P b (swaps
+// replayed on an index array in LDS), then per 64-row block a one-wavefront
+// diagonal solve and a right-looking update of every remaining row, spread
+// over workgroups of 64 rows.  Vectors and accumulation in TV.
+// ---------------------------------------------------------------------------
+#define LUDW_CHUNK 2048
+template <typename TV>
+__global__ void __launch_bounds__(1024) k_ludw_perm(int n, const int *piv, const TV *b, TV *x,
+                                                    const int *active) {
+  extern __shared__ int perm[];                 // [n] + [LUDW_CHUNK]
+  int *pc = perm + n;
+  const int q = blockIdx.x;
+  if (active && !active[q]) return;
+  const int *P = piv + (long)q * n;
+  for (int i = threadIdx.x; i < n; i += 1024) perm[i] = i;
+  for (int c0 = 0; c0 < n; c0 += LUDW_CHUNK) {
+    const int c1 = min(n, c0 + LUDW_CHUNK);
+    __syncthreads();
+    for (int i = c0 + threadIdx.x; i < c1; i += 1024) pc[i - c0] = P[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = c0; i < c1; ++i) {
+        const int p = pc[i - c0];
+        if (p != i && p >= 0 && p < n) {
+          const int t = perm[i];
+          perm[i] = perm[p];
+          perm[p] = t;
+        }
+      }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 1024) x[(long)q * n + i] = b[(long)q * n + perm[i]];
+}
+
+// diagonal block [i0, i0 + bs) of matrix blockIdx.x: unit lower (upper = 0)
+// or upper with its diagonal (upper = 1), one wavefront
+template <typename T, typename TV>
+__global__ void __launch_bounds__(64) k_ludw_diag(const T *A, long ms, int n, TV *x, int i0,
+                                                  int upper, const int *active) {
+  __shared__ TV Db[64][65];
+  const int q = blockIdx.x;
+  if (active && !active[q]) return;
+  const T *M = A + (long)q * ms;
+  TV *y = x + (long)q * n;
+  const int bs = min(64, n - i0), lane = threadIdx.x;
+  for (int r = 0; r < bs; ++r)
+    if (lane < bs) Db[r][lane] = (TV)M[(long)(i0 + r) * n + i0 + lane];
+  __syncthreads();
+  TV yv = lane < bs ? y[i0 + lane] : TV(0);
+  if (!upper) {
+    for (int t = 0; t < bs; ++t) {
+      const TV yt = __shfl(yv, t, 64);
+      if (lane > t && lane < bs) yv = fma(-Db[lane][t], yt, yv);
+    }
+  } else {
+    for (int t = bs - 1; t >= 0; --t) {
+      if (lane == t) yv = yv / Db[t][t];
+      const TV yt = __shfl(yv, t, 64);
+      if (lane < t) yv = fma(-Db[lane][t], yt, yv);
+    }
+  }
+  if (lane < bs) y[i0 + lane] = yv;
+}
+
+// y[i] -= M[i, i0:i0+bs) . y[i0:i0+bs) for rows rlo <= i < rhi; 64 rows per
+// workgroup (16 per wavefront), grid.y = matrix
+template <typename T, typename TV>
+__global__ void __launch_bounds__(256) k_ludw_upd(const T *A, long ms, int n, TV *x, int i0,
+                                                  int bs, int rlo, int rhi, const int *active) {
+  __shared__ TV yb[64];
+  const int q = blockIdx.y;
+  if (active && !active[q]) return;
+  const T *M = A + (long)q * ms;
+  TV *y = x + (long)q * n;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid < bs) yb[tid] = y[i0 + tid];
+  __syncthreads();
+  const TV yl = lane < bs ? yb[lane] : TV(0);
+  const int base = rlo + blockIdx.x * 64 + w * 16;
+#pragma unroll 4
+  for (int u = 0; u < 16; ++u) {
+    const int i = base + u;
+    if (i < rhi) {
+      TV a = lane < bs ? (TV)M[(long)i * n + i0 + lane] * yl : TV(0);
+      a = wave_sum(a);
+      if (lane == 0) y[i] -= a;
+    }
+  }
+}
+
+template <typename T, typename TV>
+static int ludw_solve(int nmat, int n, const T *LU, long ms, const int *piv, const TV *b, TV *x,
+                      const int *active, hipStream_t st) {
+  const size_t sh = (size_t)(n + LUDW_CHUNK) * sizeof(int);
+  if (!lds_optin(k_ludw_perm<TV>, sh)) return fail_msg("smx_dense_lu_solve: n exceeds LDS");
+  hipLaunchKernelGGL(k_ludw_perm<TV>, dim3(nmat), dim3(1024), sh, st, n, piv, b, x, active);
+  const int nblk = (n + 63) / 64;
+  for (int bk = 0; bk < nblk; ++bk) {
+    const int i0 = bk * 64, bs = min(64, n - i0), r0 = i0 + bs;
+    hipLaunchKernelGGL((k_ludw_diag<T, TV>), dim3(nmat), dim3(64), 0, st, LU, ms, n, x, i0, 0,
+                       active);
+    if (r0 < n)
+      hipLaunchKernelGGL((k_ludw_upd<T, TV>), dim3(cdiv(n - r0, 64), nmat), dim3(256), 0, st,
+                         LU, ms, n, x, i0, bs, r0, n, active);
+  }
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int i0 = bk * 64, bs = min(64, n - i0);
+    hipLaunchKernelGGL((k_ludw_diag<T, TV>), dim3(nmat), dim3(64), 0, st, LU, ms, n, x, i0, 1,
+                       active);
+    if (i0 > 0)
+      hipLaunchKernelGGL((k_ludw_upd<T, TV>), dim3(cdiv(i0, 64), nmat), dim3(256), 0, st, LU,
+                         ms, n, x, i0, bs, 0, i0, active);
+  }
+  return 0;
+}
+
+// matrices at most this many and order at least LUDW_MIN_N take the wide solve
+#define LUDW_MAX_MATS 16
+#define LUDW_MIN_N 1024
+
 extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
                                   const int *piv, const void *b, void *x, const int *active,
                                   void *stream) {
@@ -480,6 +603,21 @@ extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, lon
   const size_t es = prec == 0 ? 8 : 4;
   const size_t sh = (size_t)n * es + 64 * 65 * es;
   hipStream_t st = STREAM(stream);
+  if (nmat <= LUDW_MAX_MATS && n >= LUDW_MIN_N && b != x) {
+    int rc;
+    if (prec == 0)
+      rc = ludw_solve<double, double>(nmat, n, (const double *)LU, mstride, piv,
+                                      (const double *)b, (double *)x, active, st);
+    else if (prec == 1)
+      rc = ludw_solve<float, float>(nmat, n, (const float *)LU, mstride, piv, (const float *)b,
+                                    (float *)x, active, st);
+    else
+      rc = ludw_solve<float, double>(nmat, n, (const float *)LU, mstride, piv,
+                                     (const double *)b, (double *)x, active, st);
+    if (rc) return rc;
+    CHECK_LAUNCH("smx_dense_lu_solve");
+    return 0;
+  }
   if (prec == 0) {
     if (!lds_optin(k_lud_solve<double, double>, sh))
       return fail_msg("smx_dense_lu_solve: vector exceeds LDS");

@@ -238,6 +238,40 @@ def test_dense_lu_matches_lapack(lib, n):
         assert r < 1e-10, r
 
 
+@pytest.mark.parametrize("prec", [0, 2])
+def test_dense_lu_wide_solve_matches_batched_solve(lib, prec):
+    """A few matrices (the wide solve: row blocks over many workgroups) give
+    the solutions of the one-workgroup-per-matrix solve of a larger batch."""
+    from specmc_amd.engine import Engine
+    e = Engine("const", 16, 24, 2.0, [dict(Ra=1e4)])
+    n, nm = 2112, 20
+    rng = np.random.default_rng(5)
+    M = rng.normal(size=(nm, n, n)) + 2.0 * np.sqrt(n) * np.eye(n)
+    dt = torch.float64 if prec == 0 else torch.float32
+    A = torch.tensor(M, device=e.dev).to(dt)
+    piv = torch.zeros(nm, n, dtype=torch.int32, device=e.dev)
+    info = torch.ones(nm, dtype=torch.int32, device=e.dev)
+    lib.dense_lu(min(prec, 1), nm, n, A.data_ptr(), n * n, piv.data_ptr(), info.data_ptr(),
+                 e.stream())
+    b = torch.tensor(rng.normal(size=(nm, n)), device=e.dev)
+    xb, xw = torch.zeros_like(b), torch.zeros_like(b)
+    print("dense lu wide solve prec %d n=%d" % (prec, n), flush=True)
+    lib.dense_lu_solve(prec, nm, n, A.data_ptr(), n * n, piv.data_ptr(), b.data_ptr(),
+                       xb.data_ptr(), None, e.stream())
+    lib.dense_lu_solve(prec, 3, n, A.data_ptr(), n * n, piv.data_ptr(), b.data_ptr(),
+                       xw.data_ptr(), None, e.stream())
+    torch.cuda.synchronize()
+    xb, xw = xb.cpu().numpy(), xw.cpu().numpy()
+    tol = 1e-12 if prec == 0 else 1e-5
+    for q in range(3):
+        r = np.linalg.norm(M[q] @ xw[q] - b.cpu().numpy()[q]) / np.linalg.norm(b.cpu().numpy()[q])
+        print("  q=%d wide vs batched %.2e, residual %.2e" % (q, rel(xw[q], xb[q]), r),
+              flush=True)
+        assert rel(xw[q], xb[q]) < tol
+        assert r < (1e-12 if prec == 0 else 1e-5)
+    assert not np.any(xw[3:])
+
+
 def test_batched_runs_match_solo_runs(lib):
     """Runs of different physics advancing together in one Ensemble (per-run
     dt, retirement at their own tstop) reproduce their solo marches."""
