@@ -2,7 +2,7 @@
 single-run device stepper, with p50/p99 run latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-    python bench.py --workload c2|c3|c4      # the other scenarios
+    python bench.py --workload heated-2to1|ensemble-256|stream   # the other scenarios
     python bench.py --make-states            # record the 64x96 trajectory once
 
 Workload (the headline): eta_T viscosity, deta_T = 1e3, Ra = 1e4, Nx = 64,
@@ -50,7 +50,7 @@ HARD_MIN_STEPS = 16
 
 
 def make_states(path=STATES):
-    """March the plain FP64 stepper through config 1 and record checkpoints."""
+    """March the plain FP64 stepper through the 64x96 eta_T run and record checkpoints."""
     from specmc_amd import ConvectionConfig, api
     from specmc_amd.core import bench_ic
     m = api.build_model(ConvectionConfig(**WORKLOAD))
@@ -176,7 +176,7 @@ def phase_weights(log):
 
 def cpu_baseline(state, nsteps=2):
     """The oracle (numpy/scipy FP64 restatement) on the host cores: ``nsteps``
-    steps of the config-1 run from the early checkpoint."""
+    steps of the 64x96 eta_T run from the early checkpoint."""
     import oracle
     m = oracle.build_oracle(WORKLOAD)
     s = np.array(state, dtype=complex)
@@ -191,14 +191,14 @@ def cpu_baseline(state, nsteps=2):
     except Exception:
         cores = os.cpu_count() or 1
     return dict(value=nsteps / el, unit="steps/s", cores=cores, kind="port",
-                sample="%d steps of the config-1 run from its step-%d checkpoint (after "
+                sample="%d steps of the 64x96 eta_T run from its step-%d checkpoint (after "
                        "1 warm-up step) on the oracle; the device rates of that "
                        "phase are early_plain/early_batched" % (nsteps, CHECKPOINTS[0]))
 
 
 def served_leg(z, log, w, B, value, horizon, load, nreq, precision, graph_rounds=0,
                seed=5):
-    """A Poisson stream of config-1 requests resumed from the phase checkpoints
+    """A Poisson stream of 64x96 eta_T requests resumed from the phase checkpoints
     (phase drawn with the phase weights, state perturbed by 1e-3 relative),
     each marching ``horizon`` steps' worth of simulated time, arriving at
     ``load`` x the completion rate the batched form sustains, served by one
@@ -230,15 +230,15 @@ class _WallLimit(Exception):
     pass
 
 
-def run_c2(precisions=("fp64", "mixed"), max_wall=None):
+def run_heated(precisions=("fp64", "mixed"), max_wall=None):
     """The heated 2:1 case marched to t = 0.03 with the single-run model,
     diagnostics at the first steps with t >= 0.01, 0.02, 0.03 (stopped early,
     and reported as such, past ``max_wall`` seconds per precision)."""
     from specmc_amd import api
-    from specmc_amd.workloads import c2_requests, C2_LOG_TIMES
+    from specmc_amd.workloads import heated_2to1_requests, HEATED_LOG_TIMES
     import threading
     import torch
-    r = c2_requests()[0]
+    r = heated_2to1_requests()[0]
     out, errors = {}, []
 
     def one(prec):
@@ -251,7 +251,7 @@ def run_c2(precisions=("fp64", "mixed"), max_wall=None):
 
     def march_one(prec):
         m = api.build_model(r.cfg, precision=prec)
-        logs, marks = [], list(C2_LOG_TIMES)
+        logs, marks = [], list(HEATED_LOG_TIMES)
         t0 = time.perf_counter()
         last = {}
 
@@ -260,14 +260,14 @@ def run_c2(precisions=("fp64", "mixed"), max_wall=None):
             if max_wall is not None and time.perf_counter() - t0 > max_wall:
                 raise _WallLimit()
             if n % 25 == 0:
-                print("c2 %s: step %d t=%.5f wall %.1fs" % (prec, n, t, time.perf_counter() - t0),
+                print("heated-2to1 %s: step %d t=%.5f wall %.1fs" % (prec, n, t, time.perf_counter() - t0),
                       file=sys.stderr, flush=True)
             if marks and t >= marks[0] - 1e-15:
                 d = m.diagnostics(s)
                 logs.append(dict(t_mark=marks.pop(0), step=n, t=t,
                                  Nu_top=d["Nu_top"], Nu_energy=d["Nu_energy"], urms=d["urms"],
                                  wall_s=time.perf_counter() - t0))
-                print("c2 %s: %s" % (prec, json.dumps(logs[-1])), file=sys.stderr, flush=True)
+                print("heated-2to1 %s: %s" % (prec, json.dumps(logs[-1])), file=sys.stderr, flush=True)
 
         try:
             s, info = api.march_adaptive(m, r.theta0, r.tstop, on_step=cb)
@@ -290,7 +290,7 @@ def run_c2(precisions=("fp64", "mixed"), max_wall=None):
     if errors:
         raise errors[0]
     res = dict(metric="heated 2:1 run (64x96, Ra=1.25e4, H=1.5, eta_T 1e3) to t=0.03",
-               unit="steps/s", config=dict(workload="c2", Nx=64, Nz=96, Lx=4.0))
+               unit="steps/s", config=dict(workload="heated-2to1", Nx=64, Nz=96, Lx=4.0))
     for prec in precisions:
         o = dict(out[prec])
         o.pop("theta")
@@ -347,20 +347,22 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device once first (a known-size "
                          "reference for hardware byte counters)")
-    ap.add_argument("--workload", default="c1", choices=("c1", "c2", "c3", "c4"),
-                    help="c1: the 64x96 headline (default); c2: heated 2:1 run to t=0.03; "
-                         "c3: 256-run uneven-grid ensemble; c4: Poisson stream")
+    ap.add_argument("--workload", default="etaT-64x96",
+                    choices=("etaT-64x96", "heated-2to1", "ensemble-256", "stream"),
+                    help="etaT-64x96: the batched 64x96 eta_T headline (default); heated-2to1: "
+                         "the heated 2:1 run to t=0.03; ensemble-256: 256-run uneven-grid "
+                         "ensemble; stream: Poisson stream of requests up to 256x192")
     ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"),
-                    help="precision of the c3 / c4 server")
-    ap.add_argument("--slots", type=int, default=64, help="slots per bucket (c3, c4)")
+                    help="precision of the ensemble / stream server")
+    ap.add_argument("--slots", type=int, default=64, help="slots per bucket (ensemble, stream)")
     ap.add_argument("--requests", type=int, default=0,
-                    help="c3: runs (default 256); c4: requests (default 48)")
-    ap.add_argument("--rate", type=float, default=0.5, help="c4 arrivals per second")
-    ap.add_argument("--tstop", type=float, default=1e-3, help="c3 run length")
-    ap.add_argument("--max-wall", type=float, default=600.0, help="c3 / c4 wall limit (s)")
+                    help="ensemble-256: runs (default 256); stream: requests (default 48)")
+    ap.add_argument("--rate", type=float, default=0.5, help="stream arrivals per second")
+    ap.add_argument("--tstop", type=float, default=1e-3, help="ensemble-256 run length")
+    ap.add_argument("--max-wall", type=float, default=600.0, help="ensemble / stream wall limit (s)")
     ap.add_argument("--sequential", action="store_true",
-                    help="c3 / c4: step the buckets one after another on one stream")
-    ap.add_argument("--c2-precision", default="fp64,mixed")
+                    help="ensemble / stream: step the buckets one after another on one stream")
+    ap.add_argument("--heated-precision", default="fp64,mixed")
     ap.add_argument("--make-states", action="store_true")
     ap.add_argument("--states-out", default=STATES)
     a = ap.parse_args()
@@ -375,19 +377,19 @@ def main():
     if a.make_states:
         make_states(a.states_out)
         return
-    if a.workload != "c1":
+    if a.workload != "etaT-64x96":
         from specmc_amd import workloads as W
-        if a.workload == "c2":
-            out = run_c2(tuple(a.c2_precision.split(",")), a.max_wall)
-        elif a.workload == "c3":
+        if a.workload == "heated-2to1":
+            out = run_heated(tuple(a.heated_precision.split(",")), a.max_wall)
+        elif a.workload == "ensemble-256":
             out = run_served(W.ensemble_requests(n=a.requests or 256, tstop=a.tstop),
                              a.slots, a.precision, a.graph_rounds, a.max_wall,
-                             "c3: 256-run uneven-grid ensemble, tstop=%g" % a.tstop,
+                             "ensemble-256: 256-run uneven-grid ensemble, tstop=%g" % a.tstop,
                              not a.sequential)
         else:
             out = run_served(W.stream_requests(n=a.requests or 48, rate=a.rate),
                              a.slots, a.precision, a.graph_rounds, a.max_wall,
-                             "c4: Poisson stream at %g/s" % a.rate, not a.sequential)
+                             "stream: Poisson stream at %g/s" % a.rate, not a.sequential)
         print(json.dumps(out), flush=True)
         return
     if not os.path.exists(STATES):

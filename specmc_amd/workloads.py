@@ -1,9 +1,9 @@
 """Request sets of the benchmark workloads (seeded, reproducible).
 
-* ``c1_config``       -- 64x96 eta_T, deta_T = 1e3, Ra = 1e4, unit box (Lx = 2
+* ``bench_64x96_config``       -- 64x96 eta_T, deta_T = 1e3, Ra = 1e4, unit box (Lx = 2
   on the mirror-doubled carrier), BFBT route; theta0 = conductive +
   3e-2 cos(pi x) sin(pi z); tstop = 5e-3.
-* ``c2_requests``     -- the heated time-dependent case: Lx = 4 (2:1 box),
+* ``heated_2to1_requests``     -- the heated time-dependent case: Lx = 4 (2:1 box),
   64x96, Ra = 1.25e4, H = 1.5, eta_T deta_T = 1e3, theta0 = heated conductive
   profile + seeded low modes (``default_rng(2)``); march to t = 0.03.
 * ``ensemble_requests`` -- 256 runs from ``default_rng(3)``: grids uniform
@@ -26,12 +26,12 @@ from .core import bench_ic
 from .basis import shape_ops
 from .serve import Request
 
-C1_FIELDS = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64, Nz=96, Lx=2.0,
+BENCH_64x96_FIELDS = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64, Nz=96, Lx=2.0,
                  precond="bfbt")
 C1_TSTOP = 5e-3
-C2_FIELDS = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1.25e4, Nx=64, Nz=96, Lx=4.0,
+HEATED_2TO1_FIELDS = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1.25e4, Nx=64, Nz=96, Lx=4.0,
                  internal_heating=1.5, precond="bfbt")
-C2_LOG_TIMES = (0.01, 0.02, 0.03)
+HEATED_LOG_TIMES = (0.01, 0.02, 0.03)
 
 
 class _Grid:
@@ -51,20 +51,20 @@ class _Grid:
         return self.coeffs(amp * np.cos(m * np.pi * X) * np.sin(n * np.pi * Z))
 
 
-def c1_config():
-    return ConvectionConfig(**C1_FIELDS)
+def bench_64x96_config():
+    return ConvectionConfig(**BENCH_64x96_FIELDS)
 
 
-def c1_theta0(amp=3e-2):
+def bench_64x96_theta0(amp=3e-2):
     g = _Grid(64, 96, 2.0)
     return bench_ic.conductive_theta(g, amp=amp)
 
 
-def c2_requests():
-    cfg = ConvectionConfig(**C2_FIELDS)
+def heated_2to1_requests():
+    cfg = ConvectionConfig(**HEATED_2TO1_FIELDS)
     g = _Grid(64, 96, 4.0)
     th = bench_ic.seeded_theta(g, H=1.5, seed=2, sigma=1e-2)
-    return [Request("c2", cfg, th.real, C2_LOG_TIMES[-1])]
+    return [Request("heated-2to1", cfg, th.real, HEATED_LOG_TIMES[-1])]
 
 
 def _physics_cfg(model, Nx, Nz, Lx, Ra, deta, H, Di=0.0, window=None):

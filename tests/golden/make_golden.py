@@ -133,7 +133,7 @@ CASES = {
         fields=dict(viscosity_model="eta_T", deta_T=1e3, Ra=1.25e4, Nx=32,
                     Nz=48, Lx=4.0, internal_heating=1.5, precond="bfbt"),
         ic=dict(kind="seeded", seed=2), tstop=4e-4),
-    # config 1 shape, the first steps of the north-star run
+    # 64x96 eta_T shape, the first steps of the benchmark run
     "c1_short_64x96": dict(
         fields=dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64,
                     Nz=96, Lx=2.0, precond="bfbt"),
@@ -141,7 +141,7 @@ CASES = {
 }
 
 LONG = {
-    # config 1 in full (CFL-limited: on the order of 10^3 steps)
+    # the 64x96 eta_T run in full (CFL-limited: on the order of 10^3 steps)
     "c1_full_64x96": dict(
         fields=dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64,
                     Nz=96, Lx=2.0, precond="bfbt"),

@@ -47,8 +47,8 @@ def test_workloads_are_seeded_and_well_formed():
         assert 1e-4 <= r.tstop <= 1e-2
         assert r.cfg.eta_min == 1e-5 and r.cfg.eta_max == 1.0
         assert 0.0 <= r.cfg.dissipation_number <= 0.5
-    c2 = W.c2_requests()[0]
-    assert c2.cfg.Lx == 4.0 and c2.cfg.internal_heating == 1.5 and c2.tstop == 0.03
+    heated = W.heated_2to1_requests()[0]
+    assert heated.cfg.Lx == 4.0 and heated.cfg.internal_heating == 1.5 and heated.tstop == 0.03
 
 
 def test_theta0_vanishes_at_the_walls():
@@ -61,7 +61,7 @@ def test_theta0_vanishes_at_the_walls():
 
 
 def test_request_latency():
-    r = Request("x", W.c1_config(), W.c1_theta0().real, 1e-3, arrival=2.0)
+    r = Request("x", W.bench_64x96_config(), W.bench_64x96_theta0().real, 1e-3, arrival=2.0)
     assert r.latency is None
     r.finished = 3.5
     assert r.latency == 1.5
