@@ -433,10 +433,17 @@ static hipEvent_t *g_clk_ev = nullptr;
 // themselves (fg_post: vectors of length n moved; inv_apply: systems solved)
 static unsigned long long *g_clk_units = nullptr;
 
+// the first failure of an event record, reported by smx_clock_stop
+static hipError_t g_clk_err = hipSuccess;
+
 static inline bool clk_on(int tag) { return g_clk_tag == tag && g_clk_n < g_clk_cap; }
-static inline void clk_begin(hipStream_t st) { hipEventRecord(g_clk_ev[2 * g_clk_n], st); }
+static inline void clk_rec(hipEvent_t ev, hipStream_t st) {
+  const hipError_t e = hipEventRecord(ev, st);
+  if (e != hipSuccess && g_clk_err == hipSuccess) g_clk_err = e;
+}
+static inline void clk_begin(hipStream_t st) { clk_rec(g_clk_ev[2 * g_clk_n], st); }
 static inline void clk_end(hipStream_t st) {
-  hipEventRecord(g_clk_ev[2 * g_clk_n + 1], st);
+  clk_rec(g_clk_ev[2 * g_clk_n + 1], st);
   ++g_clk_n;
 }
 
@@ -445,10 +452,11 @@ static inline unsigned long long *clk_units(int tag) {
 }
 
 static void clk_free() {
-  for (int i = 0; i < 2 * g_clk_cap; ++i) hipEventDestroy(g_clk_ev[i]);
+  // teardown of the clock's own events and counter: nothing to report
+  for (int i = 0; i < 2 * g_clk_cap; ++i) (void)hipEventDestroy(g_clk_ev[i]);
   free(g_clk_ev);
   g_clk_ev = nullptr;
-  if (g_clk_units) hipFree(g_clk_units);
+  if (g_clk_units) (void)hipFree(g_clk_units);
   g_clk_units = nullptr;
   g_clk_cap = g_clk_n = g_clk_tag = 0;
 }
@@ -478,6 +486,12 @@ extern "C" int smx_clock_start(int tag, int capacity) {
 }
 
 extern "C" int smx_clock_stop(double *total_ms, int *count, long long *units) {
+  if (g_clk_err != hipSuccess) {
+    const hipError_t e = g_clk_err;
+    g_clk_err = hipSuccess;
+    clk_free();
+    return fail("smx_clock_stop (an event record failed)", e);
+  }
   double tot = 0.0;
   if (units) {
     *units = 0;
