@@ -21,8 +21,10 @@ each phase checkpoint, ``--steps`` steps of
 * batched -- B runs (the checkpoint with seeded 1e-3 relative perturbations)
   advancing together in FP64, each with its own dt;
 * mixed   -- the same B runs in mixed precision (FP32 dense rescue factor
-  under FP64 FGMRES refinement) with graph-replayed FGMRES rounds: ``value``;
-  its errors against the batched FP64 leg (same inputs) are reported;
+  under FP64 FGMRES refinement, reused across steps; after a rejected BFBT
+  solve a run goes straight to that factor for up to 16 solves, then tries
+  BFBT again) with graph-replayed FGMRES rounds: ``value``; its errors
+  against the batched FP64 leg (same inputs) are reported;
 * served  -- a Poisson stream of B requests resumed from the checkpoints,
   served with continuous admission and retirement: p50/p99.
 
@@ -45,8 +47,10 @@ WORKLOAD = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64, Nz=96, Lx=2.
                 precond="bfbt")
 TSTOP = 5e-3
 CHECKPOINTS = (100, 300, 525)
-# timed steps of the rescue phase at least: the default lag (ConvectionConfig.lag)
-HARD_MIN_STEPS = 16
+# timed steps of the rescue phase at least: one carry period of the mixed
+# rescue plus its BFBT attempt (17), which also holds one refresh of the reused
+# factor at the default lag (ConvectionConfig.lag = 16)
+HARD_MIN_STEPS = 17
 
 
 def make_states(path=STATES):
@@ -447,7 +451,9 @@ def main():
                      lag_refreshes=int(ens.eng.n_refresh.sum()),
                      rescues_fp32=int(ens.eng.n_rescue_fp32.sum()),
                      rescues_fp32_redone=int(ens.eng.n_rescue_fp32_redo.sum()),
-                     rescue_fgmres_iters=int(ens.eng.rescue_iters.sum()))
+                     rescue_fgmres_iters=int(ens.eng.rescue_iters.sum()),
+                     rescues_carried=int(ens.eng.n_rescue_carry.sum()),
+                     bfbt_rejections=int(ens.eng.n_rescue.sum()))
             if ref is not None:
                 r["mixed_vs_fp64"] = _diag_errors(ens, *ref)
             del ens

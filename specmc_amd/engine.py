@@ -57,6 +57,13 @@ LAG_STORE_BYTES = 96 << 30
 # the mixed-precision rescue refactors a run's reused FP32 factor once the
 # FGMRES on it took more than this many iterations (or after ``lag`` uses)
 RESCUE_REFRESH_ITERS = 16
+# mixed precision: a run whose last BFBT solve was rejected and rescued
+# skips the BFBT attempt (800 iterations that end in the rescue anyway) and
+# solves on its reused FP32 factor directly, for at most this many solves in
+# a row; then it tries BFBT again with the full budget.  The rescue starts
+# from zero, so a carried solve equals the rescue after a failed attempt
+# bit for bit.  0 turns the carry off (the FP64 precision never carries).
+RESCUE_CARRY = 16
 DENSE_FP64_MAXN = 8192     # largest n of the FP64 dense factor (FP32: 32768)
 ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
@@ -254,6 +261,7 @@ class Engine:
             raise ValueError("precision='mixed' runs on the FP64 engine (prec=0)")
         self.precision = precision
         self.rescue_prec = 1 if precision == "mixed" else 0
+        self.rescue_carry = RESCUE_CARRY if self.rescue_prec == 1 else 0
         pc_apply = PC_APPLY if pc_apply is None else pc_apply
         if pc_apply not in ("lu", "inv"):
             raise ValueError("pc_apply must be 'lu' or 'inv', got %r" % (pc_apply,))
@@ -453,7 +461,8 @@ class Engine:
                       self.n_rescue_weak, self.n_defect_failed, self.n_defect_double_fail,
                       self.n_rescue_fp32, self.n_rescue_fp32_redo, self.rescue_iters,
                       self.fp32_backoff, self.n_since_refresh, self.n_refresh,
-                      self.n_defect, self.n_lag_krylov, self.last_K, self.lag_iters):
+                      self.n_defect, self.n_lag_krylov, self.last_K, self.lag_iters,
+                      self.rescue_streak, self.carry_run, self.n_rescue_carry):
                 a[i] = 0
             self.last_bfbt_res[i] = 0.0
             self.last_defect_res[i] = 0.0
@@ -473,7 +482,8 @@ class Engine:
                       "n_defect_failed", "n_defect_double_fail", "n_rescue_fp32",
                       "n_rescue_fp32_redo", "rescue_iters", "fp32_backoff", "last_defect_res",
                       "n_since_refresh", "n_refresh", "n_defect", "n_lag_krylov", "last_K",
-                      "last_corr", "lag_iters", "force_refresh")
+                      "last_corr", "lag_iters", "force_refresh", "rescue_streak",
+                      "carry_run", "n_rescue_carry")
 
     def snapshot(self, slot):
         """-> the state of the run in ``slot`` as a dict of numpy arrays:
@@ -527,7 +537,8 @@ class Engine:
             self.has_hist[i] = bool(snap["has_hist"])
         if self.kind == KIND_COUPLED:
             for k in self._SNAP_COUNTERS:
-                getattr(self, k)[i] = snap["c_" + k]
+                if "c_" + k in snap:      # snapshots written before a counter existed
+                    getattr(self, k)[i] = snap["c_" + k]
             self.last_route[i] = str(snap["last_route"])
             if "lag_lu" in snap:
                 if int(snap["lag_prec"]) != self.lag_prec or self._lag_store() is None:
@@ -939,6 +950,11 @@ class Engine:
         self.n_rescue_fp32_redo = np.zeros(B, dtype=np.int64)
         self.rescue_iters = np.zeros(B, dtype=np.int64)
         self.fp32_backoff = np.zeros(B, dtype=np.int64)
+        # rescue carry (mixed precision): 1 while a run's last BFBT attempt
+        # was rejected; carried solves since that attempt; carried in total
+        self.rescue_streak = np.zeros(B, dtype=np.int64)
+        self.carry_run = np.zeros(B, dtype=np.int64)
+        self.n_rescue_carry = np.zeros(B, dtype=np.int64)
         self.last_defect_res = np.zeros(B)
         self.last_route = ["bfbt"] * B
         self.defect_tol = defect_tol
@@ -1080,37 +1096,62 @@ class Engine:
         """BFBT preconditioner + FGMRES on the Schur system, with the rescue
         of rejected solves (coupled_model.py:588-654, precond='bfbt'); the
         solution goes to self.vsol.  sv.b holds the right-hand side."""
-        L, t, B, Nk, N, n = self.L, self.t, self.B, self.Nk, self.Nz, self.n
-        s = self.stream()
-        self.gemm(B, N, N, self.prof, N, 1, t["Qprof"], 1, N, self.prof_c, N, 1,
-                  active=active, mask_mode=1, rpr=1)
-        self._reduced_op_factor(self.prof_c, self.bf_lu, self.bf_piv, active, as_pc=True)
+        t, B, N = self.t, self.B, self.Nz
         sv = self.ssol
-
-        def precond(vin, zout, mask):
-            self._pc_apply(self.bf_lu, self.bf_piv, vin, zout, 1.0, mask)
-
-        def op(zin, wout, mask):
-            self.schur_matvec(zin, wout, mask)
-
-        sv.solve(precond, op, active, key="stokes")
-        it, info, res = sv.results()
         act = np.ones(B, bool) if active is None else active.cpu().numpy().astype(bool)
-        self.bfbt_iters[act] += it[act]
-        self.last_bfbt_iters[act] = it[act]
-        self.last_bfbt_info[act] = info[act]
-        self.last_bfbt_res[act] = res[act]
-        for r in np.where(act)[0]:
-            self.last_route[r] = "bfbt"
-        bad = act & (info != 0) & ~(res <= sv.rtol)
-        if np.any(bad):
-            self.n_rescue[bad] += 1
-            self._rescue(theta, np.where(bad)[0])
+        carry = self._carry_select(act)
+        bf = act & ~carry
+        if bf.any():
+            amask = self._mask(bf) if carry.any() else active
+            self.gemm(B, N, N, self.prof, N, 1, t["Qprof"], 1, N, self.prof_c, N, 1,
+                      active=amask, mask_mode=1, rpr=1)
+            self._reduced_op_factor(self.prof_c, self.bf_lu, self.bf_piv, amask, as_pc=True)
+
+            def precond(vin, zout, mask):
+                self._pc_apply(self.bf_lu, self.bf_piv, vin, zout, 1.0, mask)
+
+            def op(zin, wout, mask):
+                self.schur_matvec(zin, wout, mask)
+
+            sv.solve(precond, op, amask, key="stokes")
+            it, info, res = sv.results()
+            self.bfbt_iters[bf] += it[bf]
+            self.last_bfbt_iters[bf] = it[bf]
+            self.last_bfbt_info[bf] = info[bf]
+            self.last_bfbt_res[bf] = res[bf]
+            for r in np.where(bf)[0]:
+                self.last_route[r] = "bfbt"
+            bad = bf & (info != 0) & ~(res <= sv.rtol)
+            self.carry_run[bf] = 0
+            self.rescue_streak[bf] = bad[bf]
+            if np.any(bad):
+                self.n_rescue[bad] += 1
+                self._rescue(theta, np.where(bad)[0])
+        if carry.any():
+            self.last_bfbt_iters[carry] = 0
+            self.carry_run[carry] += 1
+            self.n_rescue_carry[carry] += 1
+            self._rescue(theta, np.where(carry)[0])
+            for r in np.where(carry)[0]:
+                self.last_route[r] = "carry:" + self.last_route[r]
         if active is None:
             self.vsol.copy_(sv.x)
         else:
             m = active.bool()
             self.vsol[m] = sv.x[m]
+
+    def _carry_select(self, act):
+        """Runs of ``act`` whose Stokes solve skips the BFBT attempt and goes
+        straight to the mixed-precision rescue on the run's reused FP32
+        factor: the run's last BFBT attempt was rejected, it holds a factor,
+        and fewer than ``rescue_carry`` solves were carried since that
+        attempt.  Nothing is carried in FP64, or when the factors do not fit
+        the lagged-factor store (a fresh FP32 factorisation per step would
+        then replace the attempt)."""
+        if self.rescue_carry <= 0 or self._lag_bytes() > LAG_STORE_BYTES:
+            return np.zeros(self.B, dtype=bool)
+        return (act & (self.rescue_streak > 0) & self.lag_have
+                & (self.carry_run < self.rescue_carry))
 
     def _rescue(self, theta, idx, rprec=None):
         """Recompute rejected BFBT solves with the robust route

@@ -149,13 +149,42 @@ def test_mixed_trajectory_through_rescues(eng, shape):
     eth = rel(s, s_r.real)
     enu = abs(d["Nu_top"] - dr["Nu_top"]) / abs(dr["Nu_top"])
     evr = abs(d["urms"] - dr["urms"]) / abs(dr["urms"])
-    print("mixed %dx%d: rescues %d (fp32 %d, redo %d) theta %.2e Nu %.2e Vrms %.2e" % (
-        Nx, Nz, dsc["n_bfbt_rescue"], dsc["n_rescue_fp32"], dsc["n_rescue_fp32_redo"],
-        eth, enu, evr), flush=True)
-    assert dsc["n_bfbt_rescue"] >= 6
+    print("mixed %dx%d: rescues %d + carried %d (fp32 %d, redo %d) theta %.2e Nu %.2e "
+          "Vrms %.2e" % (Nx, Nz, dsc["n_bfbt_rescue"], dsc["n_rescue_carry"],
+                         dsc["n_rescue_fp32"], dsc["n_rescue_fp32_redo"], eth, enu, evr),
+          flush=True)
+    assert dsc["n_bfbt_rescue"] + dsc["n_rescue_carry"] >= 6
     assert dsc["n_rescue_fp32"] >= 1
     assert eth < 1e-8
     assert enu < 1e-6 and evr < 1e-6
+
+
+def test_rescue_carry_equals_attempt_then_rescue(eng):
+    """Mixed precision carries a run whose BFBT solve was rejected straight to
+    the rescue on its reused FP32 factor.  The rescue starts from zero, so
+    when every BFBT attempt fails (stokes_maxiter=2) the carried march must
+    equal the march that makes every attempt, bit for bit, with one attempt
+    per carry period."""
+    from specmc_amd import ConvectionConfig, api
+    from specmc_amd.models import RBCCoupled
+    cfg = ConvectionConfig(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=16, Nz=24,
+                           Lx=2.0, precond="bfbt")
+    kw = api.model_kwargs(cfg)
+    runs = {}
+    for carry in (0, 4):
+        m = RBCCoupled(stokes_maxiter=2, precision="mixed", **kw)
+        m.eng.rescue_carry = carry
+        th0 = seeded_theta(m, 0.0, 3, sigma=3e-2).real
+        s, _ = api.march(m, th0, 2e-5, 12)
+        runs[carry] = (s, m.describe())
+    (s0, d0), (s4, d4) = runs[0], runs[4]
+    nsol = d0["n_bfbt_rescue"]
+    print("carry: %d solves, %d attempts + %d carried; max |diff| %.1e" % (
+        nsol, d4["n_bfbt_rescue"], d4["n_rescue_carry"], np.abs(s0 - s4).max()), flush=True)
+    assert d0["n_rescue_carry"] == 0 and nsol >= 12
+    assert d4["n_bfbt_rescue"] + d4["n_rescue_carry"] == nsol
+    assert d4["n_bfbt_rescue"] == -(-nsol // 5)
+    assert np.array_equal(s0, s4)
 
 
 @pytest.mark.parametrize("n", [12480, 24768])
