@@ -2742,11 +2742,13 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
         }
       }
     } else if constexpr (E > 0) {
-      // modified Gram-Schmidt in the reference's order, with the basis
-      // vectors v_{i+1}, v_{i+2} already loading while h_i is reduced
+      // modified Gram-Schmidt in the reference's order, with the next PF-1
+      // basis vectors already loading while h_i is reduced (a register ring
+      // of PF vectors: one step's reduction is short next to an HBM load)
+      constexpr int PF = E <= 4 ? 6 : 3;
       __shared__ double red2[32];
       __syncthreads();
-      T v0[E], v1[E], v2[E];
+      T vb[PF][E];
       auto ld = [&](T *v, int i) {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
@@ -2754,34 +2756,28 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
           v[q] = (i <= j && q < per && e < n) ? Vr[(long)i * n + e] : T(0);
         }
       };
+      // entries past n are zero in both W and the loaded v, so the inner
+      // product and the update need no per-element guards (the valid terms
+      // come first, in the same order)
       auto mgs = [&](const T *v, int i) {
         double s = 0;
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-          long e = tid + (long)q * nt;
-          if (q < per && e < n) s += (double)v[q] * (double)W.v[q];
-        }
+        for (int q = 0; q < E; ++q) s += (double)v[q] * (double)W.v[q];
         s = block_sum_db(s, red2, i);
         const T h = (T)s;
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-          long e = tid + (long)q * nt;
-          if (q < per && e < n) W.v[q] = W.v[q] - h * v[q];
-        }
+        for (int q = 0; q < E; ++q) W.v[q] = W.v[q] - h * v[q];
         if (tid == 0) hcol[i] = h;
       };
-      ld(v0, 0);
-      ld(v1, 1);
-      ld(v2, 2);
-      for (int i = 0; i <= j; i += 3) {
-        mgs(v0, i);
-        ld(v0, i + 3);
-        if (i + 1 > j) break;
-        mgs(v1, i + 1);
-        ld(v1, i + 4);
-        if (i + 2 > j) break;
-        mgs(v2, i + 2);
-        ld(v2, i + 5);
+#pragma unroll
+      for (int p = 0; p < PF; ++p) ld(vb[p], p);
+      for (int i0 = 0; i0 <= j; i0 += PF) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+          if (i0 + p > j) break;
+          mgs(vb[p], i0 + p);
+          ld(vb[p], i0 + p + PF);
+        }
       }
     } else {
       for (int i = 0; i <= j; ++i) {
