@@ -866,96 +866,114 @@ __global__ void __launch_bounds__(256) k_gj_invert(T *Am, int N, int per_run, co
 
 // FP64 Gauss-Jordan inversion with the matrix in registers: 4N threads;
 // thread (r = tid % N, g = tid / N) holds row r of the columns q = g + 4 i
-// (i < CPT).  Per column step only the pivot column and the two swapped rows
-// pass through LDS.  Same pivot choice and the same arithmetic per entry as
+// (i < CPT).  Per column step only the pivot column, row c and the scaled
+// pivot row pass through LDS.  Same pivot choice and the same arithmetic per entry as
 // k_gj_invert: the pivot row is scaled by 1/a_cc first, every other row
 // subtracts (its column-c entry) x (the scaled pivot row), column c becomes
 // -a_rc / a_cc with 1/a_cc on the diagonal.  The row swaps are undone at the
 // end as a permutation of the columns on the way out.
+// Two barriers per column: every wavefront finds the pivot itself from the
+// pivot column (the same search, so the same row), and the LDS rows are
+// double-buffered by the parity of the column, so step c + 2 rewrites a
+// buffer only after every thread has passed step c + 1's first barrier,
+// that is after it finished reading step c.  The block is rounded up to whole
+// wavefronts (the search uses every lane); threads past 4N own no entries.
+// FULL: N == 4 CPT, so every column slot of a thread is a column of the
+// matrix and the per-entry bounds tests vanish at compile time.
 #define GJR_MAXN 128
-template <int CPT>
+template <int CPT, bool FULL>
 __global__ void __launch_bounds__(512) k_gj_invert_reg(double *Am, int N, int per_run,
                                                        const int *active) {
-  __shared__ double colv[GJR_MAXN], rowa[GJR_MAXN], rowb[GJR_MAXN];
+  __shared__ double colv[2][GJR_MAXN], rowa[2][GJR_MAXN], rows_[2][GJR_MAXN];
   __shared__ int sw[GJR_MAXN], dest[GJR_MAXN];
-  __shared__ int s_piv;
   const int m = blockIdx.x;
   if (active && !active[m / per_run]) return;
   double *Gm = Am + (long)m * N * N;
-  const int tid = threadIdx.x, r = tid % N, g = tid / N;
+  const int tid = threadIdx.x, r = tid % N, lane = tid & 63;
+  const bool own = tid < 4 * N;
+  const int g = own ? tid / N : 0;
   double a[CPT];
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
     const int q = g + 4 * i;
-    a[i] = (q < N) ? Gm[(long)q * N + r] : 0.0;
+    a[i] = (own && (FULL || q < N)) ? Gm[(long)q * N + r] : 0.0;
   }
   for (int c = 0; c < N; ++c) {
-    const int gc = c & 3, ic = c >> 2;
-    if (g == gc) {
+    const int b = c & 1, gc = c & 3, ic = c >> 2;
+    double *cv = colv[b], *ra = rowa[b], *rs = rows_[b];
+    if (own && g == gc) {
+      double v = a[0];
 #pragma unroll
-      for (int i = 0; i < CPT; ++i)
-        if (i == ic) colv[r] = a[i];
+      for (int i = 1; i < CPT; ++i) v = (i == ic) ? a[i] : v;
+      cv[r] = v;
     }
     __syncthreads();
-    if (tid < 64) {
-      double best = -1.0;
-      int bi = N;
-      for (int rr = c + tid; rr < N; rr += 64) {
-        const double v = fabs(colv[rr]);
-        if (v > best) { best = v; bi = rr; }
+    // pivot: the first largest |a| at or below the diagonal (every wavefront)
+    double best = -1.0;
+    int bi = N;
+    for (int rr = c + lane; rr < N; rr += 64) {
+      const double v = fabs(cv[rr]);
+      if (v > best) { best = v; bi = rr; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    const int p = bi >= N ? c : bi;
+    if (tid == 0) sw[c] = p;
+    // a_pc (the pivot after the swap) is cv[p]: every thread forms 1 / a_cc
+    const double inv = 1.0 / cv[p];
+    if (own && r == c) {
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const int q = g + 4 * i;
+        if (FULL || q < N) ra[q] = a[i];
       }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-      }
-      if (tid == 0) {
-        if (bi >= N) bi = c;
-        s_piv = bi;
-        sw[c] = bi;
+    }
+    if (own && r == p) {    // also when p == c: rs is the scaled pivot row
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const int q = g + 4 * i;
+        if (FULL || q < N) rs[q] = a[i] * inv;
       }
     }
     __syncthreads();
-    const int p = s_piv;
+    // after the swap, row p holds the old row c (ra); row c becomes the old
+    // row p scaled (rs), written in full below
+    if (p != c && r == p) {
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) {
+        const int q = g + 4 * i;
+        if (FULL || q < N) a[i] = ra[q];
+      }
+    }
+    const double fr = cv[(r == c) ? p : (r == p ? c : r)];
     if (r == c) {
+      // the pivot row: scaled, with 1/a_cc in column c
 #pragma unroll
       for (int i = 0; i < CPT; ++i) {
         const int q = g + 4 * i;
-        if (q < N) rowa[q] = a[i];
+        if (!FULL && q >= N) continue;
+        a[i] = (q == c) ? inv : rs[q];
       }
-    }
-    if (r == p) {           // also when p == c: rowb is the pivot row
+    } else {
+      // every entry takes the elimination; column c (slot ic of the threads
+      // with g == gc) is then replaced by -a_rc / a_cc
 #pragma unroll
       for (int i = 0; i < CPT; ++i) {
         const int q = g + 4 * i;
-        if (q < N) rowb[q] = a[i];
+        if (!FULL && q >= N) continue;
+        a[i] = a[i] - fr * rs[q];
       }
-    }
-    __syncthreads();
-    // after the swap, row c holds the old row p (rowb) and row p the old row c
-    if (p != c && (r == c || r == p)) {
-      const double *src = (r == c) ? rowb : rowa;
+      if (g == gc) {
+        const double dg = -fr * inv;
 #pragma unroll
-      for (int i = 0; i < CPT; ++i) {
-        const int q = g + 4 * i;
-        if (q < N) a[i] = src[q];
+        for (int i = 0; i < CPT; ++i) a[i] = (i == ic) ? dg : a[i];
       }
     }
-    const double inv = 1.0 / rowb[c];
-    const double fr = colv[(r == c) ? p : (r == p ? c : r)];
-#pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      const int q = g + 4 * i;
-      if (q >= N) continue;
-      if (q == c) {
-        a[i] = (r == c) ? inv : -fr * inv;
-      } else {
-        const double pr = rowb[q] * inv;
-        a[i] = (r == c) ? pr : a[i] - fr * pr;
-      }
-    }
-    __syncthreads();
   }
+  __syncthreads();
   // undo the row swaps as column swaps, last step first: position j ends up
   // holding column colat[j]; column q goes to dest[q]
   if (tid == 0) {
@@ -973,7 +991,7 @@ __global__ void __launch_bounds__(512) k_gj_invert_reg(double *Am, int N, int pe
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
     const int q = g + 4 * i;
-    if (q < N) Gm[(long)sw[q] * N + r] = a[i];
+    if (own && (FULL || q < N)) Gm[(long)sw[q] * N + r] = a[i];
   }
 }
 
@@ -1324,16 +1342,22 @@ extern "C" int smx_invert(int prec, int nmat, int N, void *A, int per_run, const
   if (prec == 0 && N >= 16 && N <= GJR_MAXN && g_gj_reg) {
     // register-resident kernel: 4N threads, ceil(N / 4) columns per thread
     const int cpt = (N + 3) / 4;
-    const dim3 grid(nmat), blk(4 * N);
+    const dim3 grid(nmat), blk((4 * N + 63) / 64 * 64);
     hipStream_t st = STREAM(stream);
-    if (cpt <= 6)
-      hipLaunchKernelGGL(k_gj_invert_reg<6>, grid, blk, 0, st, (double *)A, N, per_run, active);
-    else if (cpt <= 12)
-      hipLaunchKernelGGL(k_gj_invert_reg<12>, grid, blk, 0, st, (double *)A, N, per_run, active);
-    else if (cpt <= 24)
-      hipLaunchKernelGGL(k_gj_invert_reg<24>, grid, blk, 0, st, (double *)A, N, per_run, active);
-    else
-      hipLaunchKernelGGL(k_gj_invert_reg<32>, grid, blk, 0, st, (double *)A, N, per_run, active);
+#define GJR(C_)                                                                        \
+  do {                                                                                 \
+    if (N == 4 * (C_))                                                                 \
+      hipLaunchKernelGGL((k_gj_invert_reg<C_, true>), grid, blk, 0, st, (double *)A, N, \
+                         per_run, active);                                             \
+    else                                                                               \
+      hipLaunchKernelGGL((k_gj_invert_reg<C_, false>), grid, blk, 0, st, (double *)A,   \
+                         N, per_run, active);                                          \
+  } while (0)
+    if (cpt <= 6) GJR(6);
+    else if (cpt <= 12) GJR(12);
+    else if (cpt <= 24) GJR(24);
+    else GJR(32);
+#undef GJR
     CHECK_LAUNCH("smx_invert");
     return 0;
   }
