@@ -22,8 +22,9 @@ each phase checkpoint, ``--steps`` steps of
   advancing together in FP64, each with its own dt;
 * mixed   -- the same B runs in mixed precision (FP32 dense rescue factor
   under FP64 FGMRES refinement, reused across steps; after a rejected BFBT
-  solve a run goes straight to that factor for up to 16 solves, then tries
-  BFBT again) with graph-replayed FGMRES rounds: ``value``; its errors
+  solve a run goes straight to that factor for up to 16 solves, then probes
+  BFBT again with a 100-iteration budget) with graph-replayed FGMRES rounds:
+  ``value``; its errors
   against the batched FP64 leg (same inputs) are reported;
 * served  -- a Poisson stream of B requests resumed from the checkpoints,
   served with continuous admission and retirement: p50/p99.
@@ -453,6 +454,7 @@ def main():
                      rescues_fp32_redone=int(ens.eng.n_rescue_fp32_redo.sum()),
                      rescue_fgmres_iters=int(ens.eng.rescue_iters.sum()),
                      rescues_carried=int(ens.eng.n_rescue_carry.sum()),
+                     bfbt_probes=int(ens.eng.n_bfbt_probe.sum()),
                      bfbt_rejections=int(ens.eng.n_rescue.sum()))
             if ref is not None:
                 r["mixed_vs_fp64"] = _diag_errors(ens, *ref)

@@ -64,6 +64,10 @@ RESCUE_REFRESH_ITERS = 16
 # from zero, so a carried solve equals the rescue after a failed attempt
 # bit for bit.  0 turns the carry off (the FP64 precision never carries).
 RESCUE_CARRY = 16
+# iteration budget of such a renewed BFBT attempt (a probe): the rescue route
+# costs about as much as ~110 BFBT iterations per solve, so a BFBT that needs
+# more is not worth returning to; a probe that misses keeps the run carried
+RESCUE_PROBE_ITERS = 100
 DENSE_FP64_MAXN = 8192     # largest n of the FP64 dense factor (FP32: 32768)
 ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
@@ -165,9 +169,11 @@ class Solver:
         self.cnt_host[slot:slot + 1].copy_(self.cnt[slot:slot + 1], non_blocking=True)
         self.cnt_ev[slot].record()
 
-    def solve(self, precond, op, active=None, maxiter=None, key=None):
+    def solve(self, precond, op, active=None, maxiter=None, key=None, budget=None):
         """Run FGMRES on every active run; ``precond(vin, zout, mask)`` and
         ``op(zin, wout, mask)`` apply M^{-1} and A to the masked runs.
+        ``budget`` (host int array [B], optional) replaces the iteration
+        budget per run.
 
         With ``key`` set and the engine's ``graph_rounds`` G > 0, the rounds
         run as replays of a HIP graph of G rounds captured once per ``key``
@@ -180,6 +186,9 @@ class Solver:
         mi = self.maxiter if maxiter is None else int(maxiter)
         L.fg_init(P, B, n, m, e.p(self.b), e.p(self.x), e.p(self.V), e.p(self.g),
                   e.p(self.ctl), e.p(self.sc), self.rtol, 0.0, mi, e.p(active), e.stream())
+        if budget is not None:
+            self.ctl[7].copy_(torch.as_tensor(np.asarray(budget, dtype=np.int32),
+                                              device=e.dev))
         G = int(getattr(e, "graph_rounds", 0)) if key is not None else 0
         if G > 0 and kclock.armed():
             G = 0             # clocked launches stay eager
@@ -462,7 +471,8 @@ class Engine:
                       self.n_rescue_fp32, self.n_rescue_fp32_redo, self.rescue_iters,
                       self.fp32_backoff, self.n_since_refresh, self.n_refresh,
                       self.n_defect, self.n_lag_krylov, self.last_K, self.lag_iters,
-                      self.rescue_streak, self.carry_run, self.n_rescue_carry):
+                      self.rescue_streak, self.carry_run, self.n_rescue_carry,
+                      self.n_bfbt_probe):
                 a[i] = 0
             self.last_bfbt_res[i] = 0.0
             self.last_defect_res[i] = 0.0
@@ -483,7 +493,7 @@ class Engine:
                       "n_rescue_fp32_redo", "rescue_iters", "fp32_backoff", "last_defect_res",
                       "n_since_refresh", "n_refresh", "n_defect", "n_lag_krylov", "last_K",
                       "last_corr", "lag_iters", "force_refresh", "rescue_streak",
-                      "carry_run", "n_rescue_carry")
+                      "carry_run", "n_rescue_carry", "n_bfbt_probe")
 
     def snapshot(self, slot):
         """-> the state of the run in ``slot`` as a dict of numpy arrays:
@@ -955,6 +965,7 @@ class Engine:
         self.rescue_streak = np.zeros(B, dtype=np.int64)
         self.carry_run = np.zeros(B, dtype=np.int64)
         self.n_rescue_carry = np.zeros(B, dtype=np.int64)
+        self.n_bfbt_probe = np.zeros(B, dtype=np.int64)
         self.last_defect_res = np.zeros(B)
         self.last_route = ["bfbt"] * B
         self.defect_tol = defect_tol
@@ -1113,7 +1124,13 @@ class Engine:
             def op(zin, wout, mask):
                 self.schur_matvec(zin, wout, mask)
 
-            sv.solve(precond, op, amask, key="stokes")
+            # a run back from carried solves attempts BFBT as a probe
+            probe = bf & (self.rescue_streak > 0) & self.lag_have & self._carries()
+            budget = None
+            if probe.any():
+                self.n_bfbt_probe[probe] += 1
+                budget = np.where(probe, min(RESCUE_PROBE_ITERS, sv.maxiter), sv.maxiter)
+            sv.solve(precond, op, amask, key="stokes", budget=budget)
             it, info, res = sv.results()
             self.bfbt_iters[bf] += it[bf]
             self.last_bfbt_iters[bf] = it[bf]
@@ -1145,13 +1162,19 @@ class Engine:
         straight to the mixed-precision rescue on the run's reused FP32
         factor: the run's last BFBT attempt was rejected, it holds a factor,
         and fewer than ``rescue_carry`` solves were carried since that
-        attempt.  Nothing is carried in FP64, or when the factors do not fit
+        attempt.  The attempt after them is a probe with a budget of
+        RESCUE_PROBE_ITERS iterations.  Nothing is carried in FP64, or when the factors do not fit
         the lagged-factor store (a fresh FP32 factorisation per step would
         then replace the attempt)."""
-        if self.rescue_carry <= 0 or self._lag_bytes() > LAG_STORE_BYTES:
+        if not self._carries():
             return np.zeros(self.B, dtype=bool)
         return (act & (self.rescue_streak > 0) & self.lag_have
                 & (self.carry_run < self.rescue_carry))
+
+    def _carries(self):
+        """Whether this engine carries rescued runs (mixed precision, the
+        reused factors fit the lagged-factor store)."""
+        return self.rescue_carry > 0 and self._lag_bytes() <= LAG_STORE_BYTES
 
     def _rescue(self, theta, idx, rprec=None):
         """Recompute rejected BFBT solves with the robust route

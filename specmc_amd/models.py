@@ -246,6 +246,7 @@ class _DeviceModel:
                      n_rescue_fp32_redo=int(e.n_rescue_fp32_redo[0]),
                      n_rescue_fp32_iters=int(e.rescue_iters[0]),
                      n_rescue_carry=int(e.n_rescue_carry[0]),
+                     n_bfbt_probe=int(e.n_bfbt_probe[0]),
                      last_defect_res=float(e.last_defect_res[0]),
                      last_stokes_route=e.last_route[0])
         return d

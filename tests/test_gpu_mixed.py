@@ -184,6 +184,8 @@ def test_rescue_carry_equals_attempt_then_rescue(eng):
     assert d0["n_rescue_carry"] == 0 and nsol >= 12
     assert d4["n_bfbt_rescue"] + d4["n_rescue_carry"] == nsol
     assert d4["n_bfbt_rescue"] == -(-nsol // 5)
+    # every attempt after the first is a probe (its budget, 100, is above 2)
+    assert d4["n_bfbt_probe"] == d4["n_bfbt_rescue"] - 1 and d0["n_bfbt_probe"] == 0
     assert np.array_equal(s0, s4)
 
 
