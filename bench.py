@@ -261,7 +261,7 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
         last = {}
 
         def cb(n, t, s):
-            last.update(n=n, t=t, s=s)
+            last.update(n=n, t_prev=last.get("t", 0.0), t=t, s=s)
             if max_wall is not None and time.perf_counter() - t0 > max_wall:
                 raise _WallLimit()
             if n % 25 == 0:
@@ -285,7 +285,13 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
                          steps_per_s=info["steps"] / wall,
                          log=logs, rescues=dsc.get("n_bfbt_rescue"),
                          rescues_fp32=dsc.get("n_rescue_fp32"),
-                         rescues_fp32_redone=dsc.get("n_rescue_fp32_redo"), theta=np.real(s))
+                         rescues_fp32_redone=dsc.get("n_rescue_fp32_redo"),
+                         rescues_carried=dsc.get("n_rescue_carry"),
+                         bfbt_probes=dsc.get("n_bfbt_probe"),
+                         theta_nonconv=m.theta.n_nonconv,
+                         last_rescue_res=dsc.get("last_defect_res"),
+                         dt_last=(info["t"] - last.get("t_prev", 0.0)) if last else None,
+                         theta=np.real(s))
     # the runs of the two precisions march concurrently, each on its own stream
     threads = [threading.Thread(target=one, args=(p,)) for p in precisions]
     for t in threads:
