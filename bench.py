@@ -265,7 +265,15 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
             if max_wall is not None and time.perf_counter() - t0 > max_wall:
                 raise _WallLimit()
             if n % 25 == 0:
-                print("heated-2to1 %s: step %d t=%.5f wall %.1fs" % (prec, n, t, time.perf_counter() - t0),
+                dsc = m.describe()
+                th = m.theta
+                print("heated-2to1 %s: step %d t=%.5f dt=%.3e wall %.1fs | theta its %d info %d "
+                      "nonconv %d | stokes %s bfbt its %d res %.1e rescue res %.1e" % (
+                          prec, n, t, t - last.get("t_prev", 0.0), time.perf_counter() - t0,
+                          th.last_iters, th.last_info, th.n_nonconv,
+                          dsc.get("last_stokes_route"), dsc.get("last_bfbt_iters", -1),
+                          dsc.get("last_bfbt_true_res", float("nan")),
+                          dsc.get("last_defect_res", float("nan"))),
                       file=sys.stderr, flush=True)
             if marks and t >= marks[0] - 1e-15:
                 d = m.diagnostics(s)
