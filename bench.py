@@ -493,8 +493,10 @@ def main():
                config=dict(workload="64x96 eta_T deta_T=1e3 Ra=1e4 unit box (Lx=2 carrier), "
                                     "BFBT route, tstop=5e-3 trajectory phases",
                            batch=a.batch, Nx=64, Nz=96,
-                           precision=("mixed: FP64 Krylov/operators/blocks, FP32 dense "
-                                      "rescue LU under FP64 defect correction"
+                           precision=("mixed: FP64 Krylov/operators/blocks; FP32 dense "
+                                      "rescue LU under FP64 FGMRES refinement, reused "
+                                      "across steps; rescued runs carried past BFBT "
+                                      "with a 100-iteration probe every 17th solve"
                                       if mixed_rate is not None else "fp64"),
                            phase_weights=dict(zip(names, [float(x) for x in w]))),
                plain_fp64_steps_per_s=plain_rate, batched_fp64_steps_per_s=fp64_rate,
