@@ -1163,9 +1163,9 @@ class Engine:
         factor: the run's last BFBT attempt was rejected, it holds a factor,
         and fewer than ``rescue_carry`` solves were carried since that
         attempt.  The attempt after them is a probe with a budget of
-        RESCUE_PROBE_ITERS iterations.  Nothing is carried in FP64, or when the factors do not fit
-        the lagged-factor store (a fresh FP32 factorisation per step would
-        then replace the attempt)."""
+        RESCUE_PROBE_ITERS iterations.  Nothing is carried in FP64, or when
+        the factors do not fit the lagged-factor store (a fresh FP32
+        factorisation per step would then replace the attempt)."""
         if not self._carries():
             return np.zeros(self.B, dtype=bool)
         return (act & (self.rescue_streak > 0) & self.lag_have
