@@ -31,7 +31,8 @@ from . import _numerics as NUM
 from .api import model_spec
 from .batch import run_params, _KIND
 from .core.cfl_policy import clamp_safety
-from .engine import (DENSE_FP64_MAXN, Engine, KIND_CONST, KIND_COUPLED, RESCUE_RESTART)
+from .engine import (DENSE_FP64_MAXN, Engine, KIND_CONST, KIND_COUPLED, LAG_STORE_BYTES,
+                     RESCUE_RESTART)
 
 
 def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=40):
@@ -60,6 +61,16 @@ def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=4
     else:
         b += Nk * N * N * w                                         # eta(z) operator LU
     return int(b)
+
+
+def lag_slots(Nx, Nz, precision="fp64"):
+    """Most slots of a coupled bucket whose lagged dense factors (one n x n
+    factor per slot, FP32 for ``precision`` 'mixed' or above the FP64
+    factor's size) fit the engine's lagged-factor store: the legacy route
+    needs them, and the mixed-precision rescue reuses and carries with them."""
+    n = (Nx // 2 + 1) * Nz
+    es = 4 if (precision == "mixed" or n > DENSE_FP64_MAXN) else 8
+    return max(1, int(LAG_STORE_BYTES // (n * n * es)))
 
 
 class Request:
@@ -271,6 +282,9 @@ class Server:
                 if room >= self.slots * per:
                     break
         S = min(self.slots, room // per)
+        if key[0] == KIND_COUPLED:
+            # slots whose lagged factors fit the store (large grids only)
+            S = min(S, lag_slots(key[1], key[2], self.precision))
         if S < 1:
             return None
         b = Bucket(key, r.cfg, S, precision=self.precision, **self.engine_kw)

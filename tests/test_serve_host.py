@@ -3,7 +3,7 @@ import numpy as np
 
 from specmc_amd import workloads as W
 from specmc_amd.engine import KIND_CONST, KIND_COUPLED, KIND_DEPTH
-from specmc_amd.serve import Request, Server, run_bytes
+from specmc_amd.serve import Request, Server, lag_slots, run_bytes
 
 
 def test_run_bytes_grows_with_shape_and_solver():
@@ -65,3 +65,18 @@ def test_request_latency():
     assert r.latency is None
     r.finished = 3.5
     assert r.latency == 1.5
+
+
+def test_lag_slots_cap_only_the_largest_coupled_grids():
+    from specmc_amd.engine import LAG_STORE_BYTES
+    # 256 x 192: one FP32 factor of n = 24768 is 2.45 GB, so 64 slots do not
+    # fit the store; the cap keeps every slot's factor inside it
+    n = 129 * 192
+    S = lag_slots(256, 192, "mixed")
+    assert S < 64 and S * n * n * 4 <= LAG_STORE_BYTES < (S + 1) * n * n * 4
+    # n > 8192 takes the FP32 factor in either precision
+    assert lag_slots(256, 192, "fp64") == S
+    # every grid of the ensemble and at 64 x 96 keeps its 64 slots
+    for Nx, Nz in ((32, 24), (64, 48), (64, 96), (128, 96), (128, 192)):
+        for prec in ("fp64", "mixed"):
+            assert lag_slots(Nx, Nz, prec) >= 64, (Nx, Nz, prec)
