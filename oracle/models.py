@@ -608,6 +608,7 @@ class OracleCoupled(OracleDepth):
         self.last_stokes_route = "bfbt"
         self.n_clip_eta = self.n_clip_T = self.n_T_out_of_range = 0
         self.n_clip_eta_proj = 0
+        self.n_smooth_eta = self.n_smooth_eta_proj = 0
         self.max_T_excursion = 0.0
         # legacy route state (lagged_stokes.py:229-300)
         self.lu_lag = None
@@ -618,6 +619,7 @@ class OracleCoupled(OracleDepth):
         self._force_refresh = False
         self._last_failed = False
         self.n_defect_failed = self.n_defect_double_fail = 0
+        self.n_defect = 0
         self.last_K = 0
         self.last_corr = 0.0
         self.last_defect_res = 0.0
@@ -645,8 +647,9 @@ class OracleCoupled(OracleDepth):
         if self.eta_smooth > 0.0:
             self.n_clip_eta += laws.n_clipped(eta, self.eta_clip_lo,
                                               self.eta_clip_hi)
-            eta, _ = laws.smooth_clip_eta(eta, self.eta_clip_lo,
-                                          self.eta_clip_hi, self.eta_smooth)
+            eta, n_sm = laws.smooth_clip_eta(eta, self.eta_clip_lo,
+                                             self.eta_clip_hi, self.eta_smooth)
+            self.n_smooth_eta += n_sm
             return eta
         eta, moved = laws.clip_eta(eta, self.eta_clip_lo, self.eta_clip_hi)
         self.n_clip_eta += moved
@@ -669,8 +672,9 @@ class OracleCoupled(OracleDepth):
         if self.eta_smooth > 0.0:
             self.n_clip_eta_proj += laws.n_clipped(eta, self.eta_clip_lo,
                                                    self.eta_clip_hi)
-            eta, _ = laws.smooth_clip_eta(eta, self.eta_clip_lo,
-                                          self.eta_clip_hi, self.eta_smooth)
+            eta, n_sm = laws.smooth_clip_eta(eta, self.eta_clip_lo,
+                                             self.eta_clip_hi, self.eta_smooth)
+            self.n_smooth_eta_proj += n_sm
             return eta
         eta, moved = laws.clip_eta(eta, self.eta_clip_lo, self.eta_clip_hi)
         self.n_clip_eta_proj += moved
@@ -715,6 +719,7 @@ class OracleCoupled(OracleDepth):
         self.last_K = len(corr)
         self.last_corr = corr[-1] if corr else 0.0
         self.last_defect_res = res[-1]
+        self.n_defect += 1
         if tol is None:
             self._last_failed = False
         else:
@@ -822,9 +827,12 @@ class OracleCoupled(OracleDepth):
 # --------------------------------------------------------------------------
 # config -> model recipes (models/const.py:37-57, eta_z.py:32-59, eta_T.py:44-102)
 # --------------------------------------------------------------------------
-def build_oracle(cfg):
+def build_oracle(cfg, **model_kw):
     """``cfg``: a mapping with ConvectionConfig field names (or an object
-    with ``as_dict()``); returns the oracle model the reference would build."""
+    with ``as_dict()``); returns the oracle model the reference would build.
+    ``model_kw``: constructor arguments of the coupled model beyond the
+    config's (``stokes_maxiter``, ``stokes_rtol``), as the reference's
+    self-tests pass them (coupled_model.py:955-972)."""
     if hasattr(cfg, "as_dict"):
         cfg = cfg.as_dict()
     c = dict(cfg)
@@ -852,7 +860,7 @@ def build_oracle(cfg):
                          precond=str(c.get("precond", "legacy")).strip().lower(),
                          lag=int(c.get("lag", 16)), K=int(c.get("K", 3)),
                          defect_tol=(None if dtol <= 0.0 else dtol),
-                         max_K=int(c.get("max_K", 8)))
+                         max_K=int(c.get("max_K", 8)), **model_kw)
 
 
 def clamp_safety(s):

@@ -1641,15 +1641,20 @@ __global__ void __launch_bounds__(1024) k_eta_law(int nrow, int N, const T *thet
   n_clipT = block_sum(n_clipT, red);
   n_clipE = block_sum(n_clipE, red);
   exc = block_max(exc, red);
+  double n_band = 0;
   if (w > 0.0) {
-    // C^2 smooth clamp in ln(eta); n_clip_eta keeps the hard-clip count
+    // C^2 smooth clamp in ln(eta); n_clip_eta keeps the hard-clip count and
+    // n_band counts the points strictly inside a transition band
     const double ylo = log(lo), yhi = log(hi);
     for (long e = threadIdx.x; e < np; e += blockDim.x) {
       double y = log((double)et[e]);
+      if ((y > ylo && y < ylo + w) || (y > yhi - w && y < yhi))
+        n_band += ((e / N) == 0 || (e / N) == nrow - 1) ? 1.0 : 2.0;
       double Y = ylo + (y - ylo) * smooth5((y - ylo) / w);
       double yo = yhi - (yhi - Y) * smooth5((yhi - Y) / w);
       et[e] = (T)exp(yo);
     }
+    n_band = block_sum(n_band, red);
   } else if (n_clipE > 0.0) {
     for (long e = threadIdx.x; e < np; e += blockDim.x)
       et[e] = (T)fmin(fmax((double)et[e], lo), hi);
@@ -1660,6 +1665,7 @@ __global__ void __launch_bounds__(1024) k_eta_law(int nrow, int N, const T *thet
     c[1] += n_clipT;
     c[2] += n_clipE;
     c[3] = fmax(c[3], fmax(exc, 0.0));
+    c[5] += n_band;
   }
 }
 
@@ -1706,19 +1712,26 @@ __global__ void __launch_bounds__(1024) k_eta_guard(int Nx, int N, T *eta, const
     }
     nc = block_sum(nc, red);
     __syncthreads();
+    double nb = 0;
     if (w > 0.0) {
       const double ylo = log(lo), yhi = log(hi);
       for (long e = threadIdx.x; e < np; e += blockDim.x) {
         double y = log((double)et[e]);
+        if ((y > ylo && y < ylo + w) || (y > yhi - w && y < yhi))
+          nb += ((e / N) == 0 || (e / N) == nrow - 1) ? 1.0 : 2.0;
         double Y = ylo + (y - ylo) * smooth5((y - ylo) / w);
         double yo = yhi - (yhi - Y) * smooth5((yhi - Y) / w);
         et[e] = (T)exp(yo);
       }
+      nb = block_sum(nb, red);
     } else if (nc > 0.0) {
       for (long e = threadIdx.x; e < np; e += blockDim.x)
         et[e] = (T)fmin(fmax((double)et[e], lo), hi);
     }
-    if (threadIdx.x == 0) counters[8 * b + 4] += nc;
+    if (threadIdx.x == 0) {
+      counters[8 * b + 4] += nc;
+      counters[8 * b + 6] += nb;
+    }
     __syncthreads();
   }
   // profile per z node: reduce over the full Nx grid (mirror weights)

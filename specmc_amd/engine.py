@@ -1136,6 +1136,9 @@ class Engine:
             self.last_bfbt_iters[bf] = it[bf]
             self.last_bfbt_info[bf] = info[bf]
             self.last_bfbt_res[bf] = res[bf]
+            # coupled_model.py:607-610: a BFBT solve resets the defect fields
+            self.last_K[bf] = 0
+            self.last_defect_res[bf] = res[bf]
             for r in np.where(bf)[0]:
                 self.last_route[r] = "bfbt"
             bad = bf & (info != 0) & ~(res <= sv.rtol)
@@ -1253,21 +1256,33 @@ class Engine:
         kmax = self.max_K if tol is not None else self.K
         live = torch.ones(nb, dtype=torch.bool, device=self.dev)
         last = torch.zeros(nb, dtype=torch.float64, device=self.dev)
+        kused = torch.zeros(nb, dtype=torch.int64, device=self.dev)
         for _ in range(kmax):
             dx = solve(b - afree(x))
             c = torch.linalg.norm(dx, dim=1) / xn
             x = torch.where(live[:, None], x + dx, x)
             last = torch.where(live, c.to(last.dtype), last)
+            kused += live.to(torch.int64)
             if tol is not None:
                 live = live & ~(c <= tol)
                 if not bool(live.any()):
                     break
         res = (torch.linalg.norm(b - afree(x), dim=1) / bn).cpu().numpy()
         lastc = last.cpu().numpy()
+        kused = kused.cpu().numpy()
         res_direct = None
         for q, r in enumerate(bad):
-            if tol is not None and (not np.isfinite(lastc[q]) or lastc[q] > tol):
-                self.n_defect_failed[r] += 1
+            # the rescue refreshes the lagged factor at this eta and runs the
+            # defect solve; a failed solve is refreshed and run once more
+            # (coupled_model.py:634-641), which repeats the same arithmetic at
+            # the same eta, so its counters count both passes
+            failed = tol is not None and (not np.isfinite(lastc[q]) or lastc[q] > tol)
+            passes = 2 if failed else 1
+            self.n_refresh[r] += passes
+            self.n_defect[r] += passes
+            self.last_K[r] = int(kused[q])
+            if failed:
+                self.n_defect_failed[r] += 2
                 self.n_defect_double_fail[r] += 1
             self.last_defect_res[r] = res[q]
             if res[q] <= sv.rtol or not np.isfinite(res[q]):
@@ -1598,7 +1613,9 @@ class Engine:
         elif self.kind == KIND_DEPTH:
             self._stokes_depth(theta, active)
         else:
+            # the coupled model counts fresh solves only (coupled_model.py:758-761)
             self._stokes_coupled_memo(theta, active)
+            return
         act = np.ones(self.B, bool) if active is None else active.cpu().numpy().astype(bool)
         self.n_stokes[act] += 1
 
@@ -1627,6 +1644,7 @@ class Engine:
             m = torch.as_tensor(need, device=self.dev)
             self.memo_X[m] = self.memo_new[m]
             self.memo_valid |= need
+            self.n_stokes[need] += 1
 
     # ------------------------------------------------- BDF2 implicit theta
     def theta_matvec(self, v, out, active):

@@ -51,7 +51,7 @@ class _DeviceModel:
                  internal_heating=0.0, dissipation_number=0.0,
                  eta_min=1e-3, eta_max=1e3, eta_smooth=0.0, arrhenius_Ts=0.0,
                  arrhenius_V=0.0, reference_anchor="D",
-                 precond="bfbt", symmetrize=True, stokes_maxiter=None,
+                 precond="bfbt", symmetrize=True, stokes_maxiter=None, stokes_rtol=None,
                  defect_tol=1e-8, max_K=8, K=3, pc_apply=None, ortho=None,
                  precision="fp64", graph_rounds=0, **extra):
         self.Ra, self.Nx, self.Nz, self.Lx = float(Ra), int(Nx), int(Nz), float(Lx)
@@ -71,6 +71,8 @@ class _DeviceModel:
                   graph_rounds=int(graph_rounds))
         if stokes_maxiter is not None:
             kw["stokes_maxiter"] = int(stokes_maxiter)
+        if stokes_rtol is not None:
+            kw["stokes_rtol"] = float(stokes_rtol)
         if pc_apply is not None:
             kw["pc_apply"] = pc_apply
         if ortho is not None:
@@ -233,7 +235,8 @@ class _DeviceModel:
                      n_defect_double_fail=int(e.n_defect_double_fail[0]),
                      n_T_out_of_range=int(c[0]), n_clip_T=int(c[1]),
                      n_clip_eta=int(c[2]), max_T_excursion=float(c[3]),
-                     n_clip_eta_proj=int(c[4]), n_memo_hits=int(e.memo_hits[0]),
+                     n_clip_eta_proj=int(c[4]), n_smooth_eta=int(c[5]),
+                     n_smooth_eta_proj=int(c[6]), n_memo_hits=int(e.memo_hits[0]),
                      n_bfbt_iters=int(e.bfbt_iters[0]),
                      last_bfbt_iters=int(e.last_bfbt_iters[0]),
                      last_bfbt_info=int(e.last_bfbt_info[0]),
