@@ -242,6 +242,26 @@ int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
               const void *So, const void *Ce, const void *Ao, const void *Ae,
               const int *active, void *stream);
 
+/* The coupled Schur operator apply (FreeSchurOperator.matvec,
+ * lagged_stokes.py:202-220, with mult_eta of lagged_stokes.py:162-193
+ * evaluated pseudo-spectrally) for B runs in three launches:
+ *   yz[b][k] = W_k v[b][k]     (Wz [Nk][Nz][4Q]: the transposed stacks
+ *                               [Vp; 2k^2 Vp G_k; Vp D2 G_k; k Vp D G_k])
+ *   tz = smx_xprod(yz, eta)    (4 fields, the first 3 odd in x)
+ *   out[b][k] = Mc^T (t0 + t1, k^2 (2 t2 - t0), -4k t3) on rows < Nz-2
+ *               (Mc [3Q][Nz] = [(D2 Tp)^T; Tp^T; (D Tp)^T]), and the tau rows
+ *               out[Nz-2] = V0 . v[b][k], out[Nz-1] = V1 . v[b][k].
+ * v and out are [B][Nk][Nz]; eta of run b at eta + b*eta_bs, [nrow][Q];
+ * So/Ce [nrow][Nk], Ao/Ae [Nk][nrow] as smx_xprod; kx [Nk].  The workspace
+ * holds yz and tz: at least smx_schur_apply_work(B, Nk, Q) elements of the
+ * precision's type. */
+long long smx_schur_apply_work(int B, int Nk, int Q);
+int smx_schur_apply(int prec, int B, int Nk, int N, int nrow, int Q, const void *v,
+                    const void *eta, long eta_bs, void *out, const void *Wz,
+                    const void *Mc, const void *kx, const void *So, const void *Ce,
+                    const void *Ao, const void *Ae, const void *V0, const void *V1,
+                    void *work, long long work_elems, const int *active, void *stream);
+
 /* Advection product of the implicit theta operator
  * (ImplicitThetaSolver.advect, implicit_theta.py:300-304):
  *   a = Ae * ( -(ux * So (ks0 y0) + uz * Ce y1) [+ e1] [+ h_b] [+ e2] )

@@ -125,9 +125,33 @@ class ShapeOps:
         self.Tp = (2.0 / (M * gam))[:, None] * (Tfull[:, :N].T * wts[None, :])
         self.VpD = self.Vp @ self.D
         self.VpD2 = self.Vp @ self.D2
+        self._schur = None
 
-    def device_tensors(self, dtype, device):
-        """-> dict of torch tensors of every operator in ``dtype`` on ``device``."""
+    def schur_apply_ops(self):
+        """Operators of the three-launch Schur apply (FreeSchurOperator.matvec,
+        lagged_stokes.py:202-220), built once per shape in FP64:
+
+        * ``Wz`` (Nk, Nz, 4Q): per mode the transposed stack
+          ``[Vp; 2k^2 Vp G_k; Vp D2 G_k; k Vp D G_k]``, so that one GEMM gives
+          the four z-fields of v and of w = G_k v on the Q product nodes with
+          the convolution factors 2k^2 and k applied;
+        * ``Mc`` (3Q, Nz): ``[(D2 Tp)^T; Tp^T; (D Tp)^T]``, applied to the
+          products ``(t0 + t1, k^2 (2 t2 - t0), -4k t3)``.
+        """
+        if self._schur is None:
+            N, Q, Nk = self.Nz, self.Q, self.Nk
+            W = np.empty((Nk, N, 4 * Q))
+            for i, k in enumerate(self.kx):
+                Gk = self.G[i]
+                W[i] = np.concatenate([self.Vp, 2.0 * k * k * (self.Vp @ Gk),
+                                       self.VpD2 @ Gk, k * (self.VpD @ Gk)], 0).T
+            Mc = np.concatenate([(self.D2 @ self.Tp).T, self.Tp.T, (self.D @ self.Tp).T], 0)
+            self._schur = (W, np.ascontiguousarray(Mc))
+        return self._schur
+
+    def device_tensors(self, dtype, device, schur=True):
+        """-> dict of torch tensors of every operator in ``dtype`` on ``device``
+        (with ``schur``, also the composite operators of the Schur apply)."""
         import torch
         names = ["kx", "z", "zp", "V", "Vi", "D", "D2", "VD", "V0", "V1", "wq",
                  "G", "Qprof", "Ce_p", "So_p", "Ae_p", "Ao_p", "Ce_u", "So_u",
@@ -139,6 +163,10 @@ class ShapeOps:
         # stacked z operators used by single GEMM launches
         out["V_VD"] = torch.cat([out["V"], out["VD"]], 0).contiguous()
         out["VpD2_VpD"] = torch.cat([out["Vp"], out["VpD2"], out["VpD"]], 0).contiguous()
+        if schur:
+            W, Mc = self.schur_apply_ops()
+            out["Wz"] = torch.from_numpy(W).to(device=device, dtype=dtype)
+            out["Mc"] = torch.from_numpy(Mc).to(device=device, dtype=dtype)
         return out
 
 

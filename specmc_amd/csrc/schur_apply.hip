@@ -1,0 +1,181 @@
+// schur_apply.hip -- the coupled Schur operator apply of the BFBT route
+// (FreeSchurOperator.matvec, lagged_stokes.py:202-220) in three launches:
+//
+//   1. z-synthesis   yz[b][k] = W_k v[b][k]           per-mode GEMM, K = Nz
+//      W_k = [Vp; 2k^2 Vp G_k; Vp D2 G_k; k Vp D G_k] (4Q x Nz): the Green
+//      apply G_k and the synthesis of v, w = G_k v, D2 w, D w on the Q padded
+//      Chebyshev nodes, with the x-convolution factors 2k^2 and k folded in
+//   2. x-products    tz = Ae/Ao (eta . (Ce/So yz))     smx_xprod (mirror half
+//      of the padded x grid, all four fields)
+//   3. combine       out[b][k] = D2Tp (t0 + t1) + Tp k^2 (2 t2 - t0)
+//                                - 4k DTp t3         matrix-core GEMM, K = 3Q
+//      with the tau rows out[Nz-2] = V0 . v, out[Nz-1] = V1 . v
+//      (lagged_stokes.py:212-220; Tp truncates a degree < 2Nz-1 product on the
+//      Q nodes to its first Nz Chebyshev coefficients)
+//
+// The intermediates yz and tz ([B][Nk][4][Q] each) live in a workspace: the
+// caller's (smx_schur_apply) or the library's own device array
+// (smx_schur_apply_static).
+#include "smx_common.h"
+#include "specmc_hip.h"
+
+// combine: rows i = b*Nk + k (run-major), columns j < Nz, K = 3Q; A(i, kk) is
+// formed while loading from the four product fields of row i:
+//   kk <  Q : t0 + t1          (against (D2 Tp)^T)
+//   kk < 2Q : k^2 (2 t2 - t0)  (against Tp^T)
+//   kk < 3Q : -4k t3           (against (D Tp)^T)
+// 32 x 32 tiles, four wavefronts each owning one 16 x 16 matrix-core block,
+// K chunks of 16 double-buffered through registers.
+#define SC_TM 32
+#define SC_TN 32
+#define SC_KC 16
+template <typename T>
+__global__ void __launch_bounds__(256) k_schur_combine(int Nk, int N, int Q, int M,
+                                                       const T *tz, const T *Mc, const T *kx,
+                                                       const T *v, const T *V0, const T *V1,
+                                                       T *out, const int *active) {
+  typedef typename MfmaAcc<T>::V VT;
+  __shared__ T As[SC_KC][SC_TM + 1];
+  __shared__ T Bs[SC_KC][SC_TN + 1];
+  __shared__ T tau[SC_TM][2];
+  int ti, tj;
+  xcd_tile(ti, tj);
+  const int i0 = ti * SC_TM, j0 = tj * SC_TN;
+  if (active) {
+    const int r0 = i0 / Nk, r1 = min(i0 + SC_TM - 1, M - 1) / Nk;
+    bool any = false;
+    for (int r = r0; r <= r1; ++r) any |= (active[r] != 0);
+    if (!any) return;
+  }
+  const int K = 3 * Q, F = 4 * Q;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * 16, wc = (w & 1) * 16;
+  const int lr = lane & 15, lk = lane >> 4;
+  // tau rows of this tile's rows, when the tile holds columns Nz-2 or Nz-1
+  if (j0 + SC_TN > N - 2 && tid < 2 * SC_TM) {
+    const int ii = tid >> 1, which = tid & 1, i = i0 + ii;
+    T s = T(0);
+    if (i < M) {
+      const T *vr = v + (long)i * N;
+      const T *Vt = which ? V1 : V0;
+      for (int n = 0; n < N; ++n) s = fma(vr[n], Vt[n], s);
+    }
+    tau[ii][which] = s;
+  }
+  constexpr int NA = SC_KC * SC_TM / 256, NB = SC_KC * SC_TN / 256;
+  T ra[NA], rb[NB];
+  auto load_chunk = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = u * 256 + tid, ii = e / SC_KC, kk = e % SC_KC;
+      const int i = i0 + ii, c = k0 + kk;
+      T a = T(0);
+      if (i < M && c < K) {
+        const T *tr = tz + (long)i * F;
+        const int seg = c / Q, q = c - seg * Q;
+        const T kv = kx[i % Nk];
+        if (seg == 0) a = tr[q] + tr[Q + q];
+        else if (seg == 1) a = kv * kv * (T(2) * tr[2 * Q + q] - tr[q]);
+        else a = T(-4) * kv * tr[3 * Q + q];
+      }
+      ra[u] = a;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = u * 256 + tid, kb = e / SC_TN, jj = e % SC_TN;
+      const int j = j0 + jj, c = k0 + kb;
+      rb[u] = (j < N && c < K) ? Mc[(long)c * N + j] : T(0);
+    }
+  };
+  VT acc = VT{T(0), T(0), T(0), T(0)};
+  load_chunk(0);
+  for (int k0 = 0; k0 < K; k0 += SC_KC) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = u * 256 + tid;
+      As[e % SC_KC][e / SC_KC] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = u * 256 + tid;
+      Bs[e / SC_TN][e % SC_TN] = rb[u];
+    }
+    __syncthreads();
+    if (k0 + SC_KC < K) load_chunk(k0 + SC_KC);
+#pragma unroll
+    for (int ks = 0; ks < SC_KC; ks += 4)
+      acc = MfmaAcc<T>::step(As[ks + lk][wr + lr], Bs[ks + lk][wc + lr], acc);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ii = wr + MfmaAcc<T>::row(lk, r), i = i0 + ii;
+    const int j = j0 + wc + lr;
+    if (i >= M || j >= N) continue;
+    if (active && !active[i / Nk]) continue;
+    out[(long)i * N + j] = j < N - 2 ? acc[r] : tau[ii][j - (N - 2)];
+  }
+}
+
+extern "C" long long smx_schur_apply_work(int B, int Nk, int Q) {
+  return 2LL * B * Nk * 4 * Q;
+}
+
+static int schur_apply_run(int prec, int B, int Nk, int N, int nrow, int Q, const void *v,
+                           const void *eta, long eta_bs, void *out, const void *Wz,
+                           const void *Mc, const void *kx, const void *So, const void *Ce,
+                           const void *Ao, const void *Ae, const void *V0, const void *V1,
+                           void *work, const int *active, void *stream) {
+  const long F = 4L * Q;
+  const size_t es = prec == 0 ? 8 : 4;
+  char *yz = (char *)work;
+  char *tz = yz + (size_t)B * Nk * F * es;
+  // 1. yz[b][k][:] = W_k v[b][k]: M = B runs, N = 4Q, K = Nz, batch over k;
+  //    A(b, n) = v[b][k][n], B(n, c) = Wz[k][n][c], C(b, c) = yz[b][k][c]
+  int rc = smx_gemm_ex(2, prec, B, (int)F, N, 1.0, 0.0, v, (long)Nk * N, 1, N, 0, Wz, F, 1,
+                       (long)N * F, 0, yz, (long)Nk * F, 1, F, 0, Nk, 1, nullptr, 1, active,
+                       active ? 1 : 0, 1, stream);
+  if (rc) return rc;
+  // 2. the four x-products with eta on the padded grid (factors folded in W)
+  rc = smx_xprod(prec, B, 4, 3, Nk, nrow, Q, yz, nullptr, eta, eta_bs, tz, So, Ce, Ao, Ae,
+                 active, stream);
+  if (rc) return rc;
+  // 3. truncation, derivatives, combination and tau rows
+  const int M = B * Nk;
+  const dim3 grid(cdiv(N, SC_TN), cdiv(M, SC_TM));
+  if (prec == 0)
+    hipLaunchKernelGGL(k_schur_combine<double>, grid, dim3(256), 0, STREAM(stream), Nk, N, Q,
+                       M, (const double *)tz, (const double *)Mc, (const double *)kx,
+                       (const double *)v, (const double *)V0, (const double *)V1,
+                       (double *)out, active);
+  else
+    hipLaunchKernelGGL(k_schur_combine<float>, grid, dim3(256), 0, STREAM(stream), Nk, N, Q,
+                       M, (const float *)tz, (const float *)Mc, (const float *)kx,
+                       (const float *)v, (const float *)V0, (const float *)V1, (float *)out,
+                       active);
+  CHECK_LAUNCH("smx_schur_apply");
+  return 0;
+}
+
+static int schur_apply_check(int prec, int B, int Nk, int N, int nrow, int Q) {
+  if (prec != 0 && prec != 1) return fail_msg("smx_schur_apply: prec must be 0 or 1");
+  if (Nk <= 0 || N < 3 || nrow <= 0 || Q < N)
+    return fail_msg("smx_schur_apply: bad sizes (Nk >= 1, Nz >= 3, Q >= Nz, nrow >= 1)");
+  if ((long)cdiv((long)B * Nk, SC_TM) > 65535)
+    return fail_msg("smx_schur_apply: B * Nk above 65535 * 32 rows");
+  return 0;
+}
+
+extern "C" int smx_schur_apply(int prec, int B, int Nk, int N, int nrow, int Q, const void *v,
+                               const void *eta, long eta_bs, void *out, const void *Wz,
+                               const void *Mc, const void *kx, const void *So, const void *Ce,
+                               const void *Ao, const void *Ae, const void *V0, const void *V1,
+                               void *work, long long work_elems, const int *active,
+                               void *stream) {
+  if (B <= 0) return 0;
+  if (int rc = schur_apply_check(prec, B, Nk, N, nrow, Q)) return rc;
+  if (!work || work_elems < smx_schur_apply_work(B, Nk, Q))
+    return fail_msg("smx_schur_apply: workspace smaller than smx_schur_apply_work(B, Nk, Q)");
+  return schur_apply_run(prec, B, Nk, N, nrow, Q, v, eta, eta_bs, out, Wz, Mc, kx, So, Ce, Ao,
+                         Ae, V0, V1, work, active, stream);
+}

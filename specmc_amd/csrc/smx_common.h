@@ -93,6 +93,45 @@ __device__ T block_min(T v, T *red) {
   return s;
 }
 
+// Tile of this workgroup in the (N tiles x M tiles) plane of one batch entry.
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2: the launch order would put the N tiles of one row block on different
+// XCDs, and each XCD would fetch that row block of A from HBM again.  The
+// remap hands each XCD a contiguous range of tiles (x fastest), so the N
+// tiles of a row block share one XCD and its L2.  A bijection for any
+// tile count; it changes only which workgroup computes a tile.
+__device__ __forceinline__ void xcd_tile(int &ti, int &tj) {
+  const int gx = gridDim.x, T = gx * gridDim.y;
+  const int L = blockIdx.x + gx * blockIdx.y;
+  const int per = T >> 3, rem = T & 7, xcd = L & 7, idx = L >> 3;
+  const int t = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+  ti = t / gx;
+  tj = t - ti * gx;
+}
+
+// matrix-core operand types: v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32.
+// Lane l supplies A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16];
+// accumulator element r of lane l is C[row(l / 16, r)][l % 16].
+typedef double v4f64 __attribute__((ext_vector_type(4)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<double> {
+  typedef v4f64 V;
+  static __device__ __forceinline__ V step(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator element r for lane-row group lk
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct MfmaAcc<float> {
+  typedef v4f32 V;
+  static __device__ __forceinline__ V step(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
 // dynamic LDS beyond 64 KB needs a per-kernel opt-in
 static inline size_t lds_limit() {
   static size_t lim = 0;

@@ -25,22 +25,6 @@ struct GemmArgs {
   const int *active; int mask_mode, rows_per_run;
 };
 
-// Tile of this workgroup in the (N tiles x M tiles) plane of one batch entry.
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2: the launch order would put the N tiles of one row block on different
-// XCDs, and each XCD would fetch that row block of A from HBM again.  The
-// remap hands each XCD a contiguous range of tiles (x fastest), so the N
-// tiles of a row block share one XCD and its L2.  A bijection for any
-// tile count; it changes only which workgroup computes a tile.
-__device__ __forceinline__ void xcd_tile(int &ti, int &tj) {
-  const int gx = gridDim.x, T = gx * gridDim.y;
-  const int L = blockIdx.x + gx * blockIdx.y;
-  const int per = T >> 3, rem = T & 7, xcd = L & 7, idx = L >> 3;
-  const int t = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-  ti = t / gx;
-  tj = t - ti * gx;
-}
-
 // R: role of the product, part of the kernel symbol only (so kernel traces
 // separate them): 0 z-transform / z-derivative, 1 x-transform, 2 per-mode
 // operators (Green applies, block builds), 3 dense-matrix work of the rescue
@@ -154,26 +138,6 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 // A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]; accumulator element r of
 // lane l is C[l / 16 + 4 r][l % 16] for the FP64 form and
 // C[4 (l / 16) + r][l % 16] for the FP32 form (measured on gfx950).
-typedef double v4f64 __attribute__((ext_vector_type(4)));
-typedef float v4f32 __attribute__((ext_vector_type(4)));
-
-template <typename T> struct MfmaAcc;
-template <> struct MfmaAcc<double> {
-  typedef v4f64 V;
-  static __device__ __forceinline__ V step(double a, double b, V c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // row of accumulator element r for lane-row group lk
-  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
-};
-template <> struct MfmaAcc<float> {
-  typedef v4f32 V;
-  static __device__ __forceinline__ V step(float a, float b, V c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
-};
-
 template <typename T, int TM, int TN, int KC, int R>
 __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   constexpr int WM = TM / 2, WN = TN / 2, BM = WM / 16, BN = WN / 16;

@@ -238,18 +238,6 @@ class Solver:
         return c[3].copy(), c[4].copy(), r[3].copy()
 
 
-class _Workspace:
-    """Scratch of one Schur operator apply for ``B`` columns."""
-
-    def __init__(self, eng, B):
-        z = lambda *s: torch.zeros(*s, dtype=eng.dtype, device=eng.dev)
-        Nk, N, Q, n = eng.Nk, eng.Nz, eng.Q, eng.n
-        self.B = B
-        self.yz, self.tz = z(B, Nk * 4 * Q), z(B, Nk * 4 * Q)
-        self.tt, self.dtt = z(B, Nk * 4 * N), z(B, Nk * 3 * N)
-        self.wg, self.vin, self.out = z(B, n), z(B, n), z(B, n)
-
-
 class Engine:
     """B runs of one kind and one shape.  ``params`` is a list of per-run
     dicts with keys Ra, deta_T, Ts, eta_min, eta_max, eta_smooth,
@@ -293,7 +281,7 @@ class Engine:
         self.nrp, self.nru = o.nrp, o.nru
         self.n = self.Nk * self.Nz
         self.B = B = len(params)
-        self.t = o.device_tensors(self.dtype, self.dev)
+        self.t = o.device_tensors(self.dtype, self.dev, schur=(kind == KIND_COUPLED))
         self.profile = PROFILES[profile]
         dt, dev, N, Nk, n = self.dtype, self.dev, self.Nz, self.Nk, self.n
         z = lambda *s: torch.zeros(*s, dtype=dt, device=dev)
@@ -934,15 +922,10 @@ class Engine:
         self.bf_lu = z(B, Nk, N * N)
         self.bf_piv = torch.zeros(B, Nk, N, dtype=torch.int32, device=dev)
         self.yz = z(B, Nk * 4 * Q)
-        self.tz = z(B, Nk * 4 * Q)
-        self.tt = z(B, Nk * 4 * N)
-        self.dtt = z(B, Nk * 3 * N)
-        self.wg = z(B, self.n)
+        # the intermediates of the three-launch Schur operator apply
+        self.sw_elems = int(self.L.schur_apply_work(B, Nk, Q))
+        self.sw = z(self.sw_elems)
         self.ssol = Solver(self, self.n, restart, maxiter, rtol)
-        fs = torch.ones(4, Nk, dtype=self.dtype, device=dev)
-        fs[1] = 2.0 * self.k2
-        fs[3] = self.t["kx"]
-        self.fscale = fs.contiguous()
         self.memo_X = z(B, self.n)
         self.memo_new = z(B, self.n)
         self.memo_valid = np.zeros(B, dtype=bool)
@@ -1017,36 +1000,22 @@ class Engine:
         self.xleft(t["Ce_p"], nrp, Nk, self.yz, Q, Nk * Q, self.eta_S, Q, nrp * Q, Q,
                    active=active)
 
-    def schur_matvec(self, v, out, active, W=None, eta=None, eta_bs=None):
+    def schur_matvec(self, v, out, active, eta=None, eta_bs=None):
         """Coupled Schur operator (FreeSchurOperator.matvec, lagged_stokes.py:202-220)
-        with the eta products evaluated pseudo-spectrally.  ``W`` is a
-        workspace (default: the engine's own, one column per run); ``eta`` /
-        ``eta_bs`` the eta field on the product grid and its batch stride."""
-        L, t, Nk, N, Q = self.L, self.t, self.Nk, self.Nz, self.Q
-        W = self if W is None else W
-        B = W.B
+        with the eta products evaluated pseudo-spectrally, in three launches
+        (``smx_schur_apply``): the per-mode z-synthesis of v and of its Green
+        field, the four x-products with eta, and the truncation/derivative
+        combination with the tau rows.  ``eta`` / ``eta_bs``: the eta field on
+        the product grid and its batch stride (default: this engine's)."""
+        t = self.t
         eta = self.eta_S if eta is None else eta
-        eta_bs = self.nrp * Q if eta_bs is None else eta_bs
-        s = self.stream()
-        self.green(v, W.wg, active, nb=B)
-        # z-values on the padded product grid: f0 = v, (f1, f2, f3) = (w, D2 w, D w)
-        self.zrows(v, t["Vp"], W.yz, N, Q, N, 4 * Q, active=active, nb=B)
-        self.zrows(W.wg, t["VpD2_VpD"], W.yz, N, 3 * Q, N, 4 * Q, doff=Q, active=active, nb=B)
-        L.xprod(self.prec, B, 4, 3, Nk, self.nrp, Q, self.p(W.yz), self.p(self.fscale),
-                self.p(eta), eta_bs, self.p(W.tz), self.p(t["So_p"]),
-                self.p(t["Ce_p"]), self.p(t["Ao_p"]), self.p(t["Ae_p"]), self.p(active), s)
-        # truncation to Nz Chebyshev modes, then D2 t0, D2 t1, D t3
-        self.gemm(B * Nk * 4, N, Q, W.tz, Q, 1, t["Tp"], 1, Q, W.tt, N, 1,
-                  active=active, mask_mode=1, rpr=Nk * 4)
-        # GEMMs over all B*Nk mode rows (rows stride 4N in, 3N out): D2 t0 and
-        # D2 t1 as one launch batched over the field offset, then D t3
-        self.gemm(B * Nk, N, N, W.tt, 4 * N, 1, t["D2"], 1, N, W.dtt, 3 * N, 1,
-                  nz1=2, sa=(N, 0), sc=(N, 0), active=active, mask_mode=1, rpr=Nk)
-        self.gemm(B * Nk, N, N, W.tt, 4 * N, 1, t["D"], 1, N, W.dtt, 3 * N, 1,
-                  aoff=3 * N, coff=2 * N, active=active, mask_mode=1, rpr=Nk)
-        L.stokes_combine(self.prec, B, Nk, N, self.p(W.tt), self.p(W.dtt), self.p(t["kx"]),
-                         self.p(v), self.p(t["V0"]), self.p(t["V1"]), self.p(out),
-                         self.p(active), s)
+        eta_bs = self.nrp * self.Q if eta_bs is None else eta_bs
+        self.L.schur_apply(self.prec, self.B, self.Nk, self.Nz, self.nrp, self.Q, self.p(v),
+                           self.p(eta), eta_bs, self.p(out), self.p(t["Wz"]), self.p(t["Mc"]),
+                           self.p(t["kx"]), self.p(t["So_p"]), self.p(t["Ce_p"]),
+                           self.p(t["Ao_p"]), self.p(t["Ae_p"]), self.p(t["V0"]),
+                           self.p(t["V1"]), self.p(self.sw), self.sw_elems, self.p(active),
+                           self.stream())
 
     def assemble_schur(self, r, out=None):
         """Dense real Schur matrix of run ``r`` at its current eta (the

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libspecmc_hip.so")
 P = ctypes.c_void_p
 I = ctypes.c_int
 L = ctypes.c_long
+LL = ctypes.c_longlong
 D = ctypes.c_double
 
 # name -> argtypes, in the order of include/specmc_hip.h
@@ -47,6 +48,8 @@ SIGNATURES = {
     "smx_eta_law": [I, I, I, I, P, P, P, P, P, P, P],
     "smx_eta_guard_profile": [I, I, I, I, P, P, P, I, P, P, P],
     "smx_xprod": [I, I, I, I, I, I, I, P, P, P, L, P, P, P, P, P, P, P],
+    "smx_schur_apply_work": [I, I, I],
+    "smx_schur_apply": [I, I, I, I, I, I, P, P, L, P, P, P, P, P, P, P, P, P, P, P, LL, P, P],
     "smx_advect_prod": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P],
     "smx_axpby": [I, L, I, D, P, P, D, P, P, P, P, P],
     "smx_rowscale": [I, I, I, I, P, P, D, P, I, P, P, P],
@@ -63,6 +66,9 @@ SIGNATURES = {
     "smx_fg_count": [I, P, P, P],
     "smx_cfl_dt": [I, P, P, P, P, P],
 }
+
+# entry points that return a value rather than a status
+_VALUE_RETURNING = ("smx_schur_apply_work",)
 
 _lib = None
 
@@ -94,6 +100,9 @@ class _Lib:
         for name in SIGNATURES:
             if name in ("smx_version", "smx_last_error"):
                 continue
+            if name in _VALUE_RETURNING:
+                setattr(self, name[4:], getattr(cdll, name))
+                continue
             setattr(self, name[4:], _Checked(cdll, name))
 
     def version(self):
@@ -115,6 +124,7 @@ def load():
             fn.argtypes = args
             fn.restype = ctypes.c_int
         cdll.smx_last_error.restype = ctypes.c_char_p
+        cdll.smx_schur_apply_work.restype = ctypes.c_longlong
         _lib = _Lib(cdll)
     return _lib
 

@@ -15,7 +15,7 @@ def _declarations():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     out = {}
-    for m in re.finditer(r"^(?:int|const char \*)\s*(smx_\w+)\(([^)]*)\);", text, re.M | re.S):
+    for m in re.finditer(r"^(?:int|long long|const char \*)\s*(smx_\w+)\(([^)]*)\);", text, re.M | re.S):
         args = " ".join(m.group(2).split())
         out[m.group(1)] = [] if args in ("", "void") else [a.strip() for a in args.split(",")]
     return out
@@ -30,6 +30,8 @@ def test_header_declares_the_bound_entry_points():
         for a, t in zip(args, SIGNATURES[name]):
             if "*" in a:
                 assert t is ctypes.c_void_p, (name, a)
+            elif a.startswith("long long"):
+                assert t is ctypes.c_longlong, (name, a)
             elif a.startswith("long"):
                 assert t is ctypes.c_long, (name, a)
             elif a.startswith("double"):

@@ -114,6 +114,47 @@ def test_pseudospectral_schur_matvec_equals_matrix_form(shape):
     assert np.abs(want.real).max() <= 1e-12 * np.abs(want.imag).max()
 
 
+def _twin_schur_apply(o, eta_c, v):
+    """The three launches of smx_schur_apply on the host: per-mode composite
+    synthesis, x-products, combination against Mc with the tau rows."""
+    W, Mc = o.schur_apply_ops()
+    Q, k = o.Q, o.kx[:, None]
+    yz = np.einsum("kn,knc->kc", v, W).reshape(o.Nk, 4, Q)
+    etaS = o.Ce_p @ (eta_c @ o.Vp.T)
+    tz = np.empty_like(yz)
+    for f in range(4):
+        odd = f < 3
+        g = (o.So_p if odd else o.Ce_p) @ yz[:, f]
+        tz[:, f] = (o.Ao_p if odd else o.Ae_p) @ (g * etaS)
+    a = np.concatenate([tz[:, 0] + tz[:, 1], k ** 2 * (2.0 * tz[:, 2] - tz[:, 0]),
+                        -4.0 * k * tz[:, 3]], 1)
+    out = a @ Mc
+    out[:, -2] = v @ o.V0
+    out[:, -1] = v @ o.V1
+    return out
+
+
+def test_three_launch_schur_apply_equals_matrix_form(shape):
+    """The composite operators of the fused apply (Wz, Mc) give the
+    reference's matrix-form Schur matvec (lagged_stokes.py:202-220)."""
+    o = shape
+    m = oracle.build_oracle(dict(viscosity_model="eta_T", Nx=o.Nx, Nz=o.Nz,
+                                 Lx=o.Lx, precond="bfbt", deta_T=1e3))
+    X, Z = np.meshgrid(m.x, m.z, indexing="ij")
+    th = m.coeffs(0.2 * np.cos(2 * np.pi * X / o.Lx) * np.sin(np.pi * Z)
+                  + 0.05 * np.cos(4 * np.pi * X / o.Lx) * np.sin(2 * np.pi * Z))
+    ops = _SchurOps(m, m.eta_of(th.real))
+    rng = np.random.default_rng(5)
+    s = rng.normal(size=(o.Nk, o.Nz)) * np.exp(-np.arange(o.Nz) / 6.0)
+    # the eight-launch form it replaces, operation for operation (every row)
+    assert rel(_twin_schur_apply(o, ops.eta_c.real, s),
+               _twin_schur_matvec(o, ops.eta_c.real, s)) < 1e-12
+    s[0] = 0.0            # the k = 0 row of a Stokes unknown is zero
+    want = ops.matvec(1j * s).reshape(o.Nk, o.Nz).imag
+    got = _twin_schur_apply(o, ops.eta_c.real, s)
+    assert rel(got, want) < 1e-11
+
+
 def test_pseudospectral_theta_advection_equals_reference(shape):
     o = shape
     m = oracle.build_oracle(dict(viscosity_model="eta_z", Nx=o.Nx, Nz=o.Nz,
