@@ -262,6 +262,24 @@ int smx_schur_apply(int prec, int B, int Nk, int N, int nrow, int Q, const void 
                     const void *Ao, const void *Ae, const void *V0, const void *V1,
                     void *work, long long work_elems, const int *active, void *stream);
 
+/* smx_schur_apply without a workspace argument; no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * intermediates live in the library's own device array of
+ * smx_schur_apply_static_bytes() bytes (64 MiB, one copy per device, held
+ * until the process ends); the call allocates nothing.  Sizes whose
+ * workspace (smx_schur_apply_work elements) exceeds it are rejected with an
+ * error.  The first call on a device looks up the array's address there (make
+ * that call before a capture).  All calls on one device share the array, so
+ * calls that overlap on one device race; the caller serialises them (one
+ * stream, or events). */
+long long smx_schur_apply_static_bytes(void);
+int smx_schur_apply_static(int prec, int B, int Nk, int N, int nrow, int Q, const void *v,
+                           const void *eta, long eta_bs, void *out, const void *Wz,
+                           const void *Mc, const void *kx, const void *So, const void *Ce,
+                           const void *Ao, const void *Ae, const void *V0, const void *V1,
+                           const int *active, void *stream);
+
 /* Advection product of the implicit theta operator
  * (ImplicitThetaSolver.advect, implicit_theta.py:300-304):
  *   a = Ae * ( -(ux * So (ks0 y0) + uz * Ce y1) [+ e1] [+ h_b] [+ e2] )

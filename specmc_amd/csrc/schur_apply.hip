@@ -16,6 +16,8 @@
 // The intermediates yz and tz ([B][Nk][4][Q] each) live in a workspace: the
 // caller's (smx_schur_apply) or the library's own device array
 // (smx_schur_apply_static).
+#include <mutex>
+
 #include "smx_common.h"
 #include "specmc_hip.h"
 
@@ -176,6 +178,67 @@ extern "C" int smx_schur_apply(int prec, int B, int Nk, int N, int nrow, int Q, 
   if (int rc = schur_apply_check(prec, B, Nk, N, nrow, Q)) return rc;
   if (!work || work_elems < smx_schur_apply_work(B, Nk, Q))
     return fail_msg("smx_schur_apply: workspace smaller than smx_schur_apply_work(B, Nk, Q)");
+  return schur_apply_run(prec, B, Nk, N, nrow, Q, v, eta, eta_bs, out, Wz, Mc, kx, So, Ce, Ao,
+                         Ae, V0, V1, work, active, stream);
+}
+
+// ---------------------------------------------------------------------------
+// smx_schur_apply_static: the same launches on the library's own workspace, a
+// file-scope device array (one copy per device, held until the process
+// ends), for callers that cannot allocate.  In host code the array's name is
+// only a stand-in for its symbol; the device address comes from
+// hipGetSymbolAddress, taken once per device on the first call there (with
+// that device current) and kept.  Every call on one device shares the array:
+// calls that overlap on one device race.  Nothing in it has to start at zero:
+// every entry a launch reads for an active run was written earlier in the
+// same call.
+// ---------------------------------------------------------------------------
+#define SCHUR_STATIC_BYTES (64ull << 20)
+#define SCHUR_STATIC_MAXDEV 64
+static __device__ __align__(256) unsigned char scratch[SCHUR_STATIC_BYTES];
+static void *g_scratch_ptr[SCHUR_STATIC_MAXDEV];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail("smx_schur_apply_static: hipGetDevice", e);
+  if (dev < 0 || dev >= SCHUR_STATIC_MAXDEV)
+    return fail_msg("smx_schur_apply_static: device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail("smx_schur_apply_static: hipGetSymbolAddress", e);
+    if (!p) return fail_msg("smx_schur_apply_static: no device address for the workspace");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" long long smx_schur_apply_static_bytes(void) { return (long long)SCHUR_STATIC_BYTES; }
+
+extern "C" int smx_schur_apply_static(int prec, int B, int Nk, int N, int nrow, int Q,
+                                      const void *v, const void *eta, long eta_bs, void *out,
+                                      const void *Wz, const void *Mc, const void *kx,
+                                      const void *So, const void *Ce, const void *Ao,
+                                      const void *Ae, const void *V0, const void *V1,
+                                      const int *active, void *stream) {
+  if (B <= 0) return 0;
+  if (int rc = schur_apply_check(prec, B, Nk, N, nrow, Q)) return rc;
+  const unsigned long long need =
+      (unsigned long long)smx_schur_apply_work(B, Nk, Q) * (prec == 0 ? 8u : 4u);
+  if (need > SCHUR_STATIC_BYTES) {
+    snprintf(g_err, sizeof(g_err),
+             "smx_schur_apply_static: B=%d runs of Nk=%d, Q=%d need %llu bytes of workspace, "
+             "above the library's %llu; split the batch or pass a workspace to "
+             "smx_schur_apply",
+             B, Nk, Q, need, (unsigned long long)SCHUR_STATIC_BYTES);
+    return -1;
+  }
+  void *work = nullptr;
+  if (int rc = scratch_ptr(&work)) return rc;
   return schur_apply_run(prec, B, Nk, N, nrow, Q, v, eta, eta_bs, out, Wz, Mc, kx, So, Ce, Ao,
                          Ae, V0, V1, work, active, stream);
 }

@@ -50,6 +50,8 @@ SIGNATURES = {
     "smx_xprod": [I, I, I, I, I, I, I, P, P, P, L, P, P, P, P, P, P, P],
     "smx_schur_apply_work": [I, I, I],
     "smx_schur_apply": [I, I, I, I, I, I, P, P, L, P, P, P, P, P, P, P, P, P, P, P, LL, P, P],
+    "smx_schur_apply_static_bytes": [],
+    "smx_schur_apply_static": [I, I, I, I, I, I, P, P, L, P, P, P, P, P, P, P, P, P, P, P, P],
     "smx_advect_prod": [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P],
     "smx_axpby": [I, L, I, D, P, P, D, P, P, P, P, P],
     "smx_rowscale": [I, I, I, I, P, P, D, P, I, P, P, P],
@@ -68,7 +70,7 @@ SIGNATURES = {
 }
 
 # entry points that return a value rather than a status
-_VALUE_RETURNING = ("smx_schur_apply_work",)
+_VALUE_RETURNING = ("smx_schur_apply_work", "smx_schur_apply_static_bytes")
 
 _lib = None
 
@@ -125,6 +127,7 @@ def load():
             fn.restype = ctypes.c_int
         cdll.smx_last_error.restype = ctypes.c_char_p
         cdll.smx_schur_apply_work.restype = ctypes.c_longlong
+        cdll.smx_schur_apply_static_bytes.restype = ctypes.c_longlong
         _lib = _Lib(cdll)
     return _lib
 
