@@ -7,30 +7,36 @@ single-run device stepper, with p50/p99 run latency.
 
 Workload (the headline): eta_T viscosity, deta_T = 1e3, Ra = 1e4, Nx = 64,
 Nz = 96, Lx = 2 (the unit box on its mirror-doubled carrier), BFBT Stokes
-route, theta0 = conductive + 3e-2 cos(pi x) sin(pi z), marched to tstop = 5e-3.
+route, theta0 = conductive + amp cos(pi x) sin(pi z), marched to tstop = 5e-3.
 A "step" is one run-step (CFL dt + coupled Stokes solve + BDF2 theta solve).
 
-The cost of a step changes by ~50x along that run (the Stokes FGMRES needs ~10
-iterations at first, then hits its 800-iteration budget and goes through the
-dense rescue route, then settles near 400).  ``--make-states`` marches the
-plain stepper through the whole run and records phase checkpoints and the
-per-step log in ``bench_states_c1_64x96.npz``.  The default run times, from
-each phase checkpoint, ``--steps`` steps of
+``value`` -- the full-run leg: B requests with seeded amp ~ U[2.5e-2, 3.5e-2]
+(one of them at 3e-2, the run the FP64 oracle's full march recorded) arrive
+as a Poisson stream and are served in mixed precision by one bucket of B
+slots, each from t = 0 to its tstop; value = sum of run-steps / wall time from
+the first arrival to the last retirement, p50/p99 = time to solution.  It is
+compared with ``plain_full`` -- one run (amp 3e-2) marched alone to tstop on
+the FP64 device stepper, unbatched, with the same graph settings.
 
-* plain   -- one run on the FP64 device stepper, unbatched, eager launches;
+The cost of a step changes by ~50x along the run (the Stokes FGMRES needs ~10
+iterations at first, then hits its 800-iteration budget and goes through the
+dense rescue route, then settles near 400).  The phase legs break the rate
+down: ``--make-states`` recorded phase checkpoints and the per-step log of the
+whole run in ``bench_states_c1_64x96.npz``, and from each checkpoint
+``--steps`` steps (after ``--warmup``) are timed of
+
+* plain   -- one run on the FP64 device stepper, unbatched;
 * batched -- B runs (the checkpoint with seeded 1e-3 relative perturbations)
-  advancing together in FP64, each with its own dt;
+  advancing together in FP64, each with its own dt, with the launch clock on
+  the per-mode preconditioner apply, the Arnoldi step and the dense solves
+  (the rooflines);
 * mixed   -- the same B runs in mixed precision (FP32 dense rescue factor
   under FP64 FGMRES refinement, reused across steps; after a rejected BFBT
   solve a run goes straight to that factor for up to 16 solves, then probes
-  BFBT again with a 100-iteration budget) with graph-replayed FGMRES rounds:
-  ``value``; its errors
-  against the batched FP64 leg (same inputs) are reported;
-* served  -- a Poisson stream of B requests resumed from the checkpoints,
-  served with continuous admission and retirement: p50/p99.
+  BFBT again with a 100-iteration budget); its errors against the batched
+  FP64 leg (same inputs) are reported;
 
-and weights the phases by their share of the recorded run, giving the
-whole-run aggregate rates.
+weighted by each phase's share of the recorded run.
 """
 import argparse
 import json
@@ -81,9 +87,20 @@ def make_states(path=STATES):
     print(json.dumps(dict(steps=info["steps"], t=info["t"], wall=wall, diag=d)), flush=True)
 
 
-ROOF_KERNEL = "fg_post"      # FGMRES Arnoldi step (Z_j store, MGS, Givens) of both solves
+# the dominant kernel of the step (the per-mode preconditioner apply of both
+# FGMRES solves, k_inv_apply_pairs_pf) is the roofline; the Arnoldi step and
+# the dense solves of the rescue are reported beside it
+ROOF_KERNEL = "pc_apply"
+CLOCKED = ("pc_apply", "fg_post", "dense_solve")
 PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fg_post_c1_b256_traffic.json")
+TRAFFIC_FILES = {
+    "pc_apply": os.path.join(ROOT, "profiles", "inv_apply_c1_b256_traffic.json"),
+    "fg_post": os.path.join(ROOT, "profiles", "fg_post_c1_b256_traffic.json"),
+}
+KERNEL_SYMBOLS = {"pc_apply": "k_inv_apply_pairs_pf", "fg_post": "k_fg_post",
+                  "dense_solve": "k_lud_solve"}
+AMP_RANGE = (2.5e-2, 3.5e-2)   # seeded amplitudes of the full-run requests
+ORACLE_FULL = os.path.join(ROOT, "tests", "golden", "oracle_c1_full_64x96.npz")
 
 
 def legs_of(a):
@@ -142,11 +159,22 @@ def _diag_errors(ens, ref_states, ref_diag):
     return dict(theta_l2=eth, Nu=enu, Vrms=evr)
 
 
-def unit_bytes(eng):
-    """Algorithmic HBM bytes per unit the fg_post kernel counts: one vector of
-    length n = Nk*Nz (an Arnoldi step j reads z, w and V_0..V_j and writes Z_j
-    and V_{j+1}: j + 5 vectors; a restart reads the residual and writes V_0)."""
-    return eng.n * eng.esize
+def unit_bytes(eng, kernel, factor_bytes=8):
+    """Algorithmic HBM bytes per unit each clocked kernel counts.
+
+    * fg_post: one vector of length n = Nk*Nz (an Arnoldi step j reads z, w
+      and V_0..V_j and writes Z_j and V_{j+1}: j + 5 vectors; a restart reads
+      the residual and writes V_0);
+    * pc_apply: one system, the Nz x Nz inverse block read once, its
+      right-hand side read and its solution written: (Nz^2 + 2 Nz) * 8;
+    * dense_solve: one n x n factor read once (``factor_bytes`` per entry),
+      its pivots, and the FP64 right-hand side and solution."""
+    n, N = eng.n, eng.Nz
+    if kernel == "fg_post":
+        return n * eng.esize
+    if kernel == "pc_apply":
+        return (N * N + 2 * N) * eng.esize
+    return n * n * factor_bytes + n * 4 + 2 * n * 8
 
 
 def _timed(ens, steps, warmup, clock=None):
@@ -179,56 +207,116 @@ def phase_weights(log):
     return w / w.sum()
 
 
-def cpu_baseline(state, nsteps=2):
-    """The oracle (numpy/scipy FP64 restatement) on the host cores: ``nsteps``
-    steps of the 64x96 eta_T run from the early checkpoint."""
+def cpu_baseline(z, weights):
+    """The oracle (numpy/scipy FP64 restatement of the reference's algorithm)
+    on the host cores: one step (CFL dt with its Stokes solve, then the step)
+    from each phase checkpoint, combined with the phase weights of the device
+    legs into a whole-run rate."""
     import oracle
-    m = oracle.build_oracle(WORKLOAD)
-    s = np.array(state, dtype=complex)
-    s = m.step(s, oracle.models.policy_dt(m, s))
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
+    per = {}
+    for name, ck in zip(("early", "hard", "late"), CHECKPOINTS):
+        m = oracle.build_oracle(WORKLOAD)
+        s = np.array(z["s%d" % ck], dtype=complex)
+        t0 = time.perf_counter()
         s = m.step(s, oracle.models.policy_dt(m, s))
-    el = time.perf_counter() - t0
+        per[name] = 1.0 / (time.perf_counter() - t0)
+        print("cpu baseline %s: %.3f steps/s (route %s)" % (name, per[name],
+                                                            m.last_stokes_route),
+              file=sys.stderr, flush=True)
+    value = 1.0 / sum(wi / per[p] for wi, p in zip(weights, ("early", "hard", "late")))
     try:
         import threadpoolctl
         cores = max(int(p.get("num_threads", 1)) for p in threadpoolctl.threadpool_info())
     except Exception:
         cores = os.cpu_count() or 1
-    return dict(value=nsteps / el, unit="steps/s", cores=cores, kind="port",
-                sample="%d steps of the 64x96 eta_T run from its step-%d checkpoint (after "
-                       "1 warm-up step) on the oracle; the device rates of that "
-                       "phase are early_plain/early_batched" % (nsteps, CHECKPOINTS[0]))
+    return dict(value=value, unit="steps/s", cores=cores, kind="port", phases=per,
+                sample="one step of the 64x96 eta_T run from each of its step-%d/%d/%d "
+                       "checkpoints on the oracle, weighted like the device legs"
+                       % CHECKPOINTS)
 
 
-def served_leg(z, log, w, B, value, horizon, load, nreq, precision, graph_rounds=0,
-               seed=5):
-    """A Poisson stream of 64x96 eta_T requests resumed from the phase checkpoints
-    (phase drawn with the phase weights, state perturbed by 1e-3 relative),
-    each marching ``horizon`` steps' worth of simulated time, arriving at
-    ``load`` x the completion rate the batched form sustains, served by one
-    bucket of ``B`` slots with continuous admission and retirement."""
+def _theta0(amp):
+    from specmc_amd.workloads import bench_64x96_theta0
+    return np.real(bench_64x96_theta0(amp=amp))
+
+
+def _oracle_full():
+    """The FP64 oracle's march of the amp = 3e-2 run to tstop (final state and
+    diagnostics), if the fixture is present."""
+    if not os.path.exists(ORACLE_FULL):
+        return None
+    with np.load(ORACLE_FULL, allow_pickle=False) as f:
+        return np.real(f["final"]), json.loads(str(f["diag"])), json.loads(str(f["info"]))
+
+
+def _vs_oracle(theta, diag, ref):
+    if ref is None or theta is None or diag is None:
+        return None
+    th, d, _ = ref
+    return dict(theta_l2=float(np.linalg.norm(theta - th) / np.linalg.norm(th)),
+                Nu=abs(diag["Nu_top"] - d["Nu_top"]) / abs(d["Nu_top"]),
+                Nu_energy=abs(diag["Nu_energy"] - d["Nu_energy"]) / abs(d["Nu_energy"]),
+                Vrms=abs(diag["urms"] - d["urms"]) / abs(d["urms"]))
+
+
+def full_leg(B, precision, graph_rounds, arrive_s, seed=8):
+    """B requests of the 64x96 run with seeded amplitudes, Poisson arrivals
+    over about ``arrive_s`` seconds, each marched from t = 0 to tstop by one
+    bucket of B slots (continuous admission and retirement).  A warm-up of B
+    two-step requests first sets kernel attributes, workspaces and graph
+    captures up; the timed window runs from the first arrival to the last
+    retirement."""
     from specmc_amd import ConvectionConfig
     from specmc_amd.serve import Request, Server
+    from specmc_amd import _numerics as NUM
     rng = np.random.default_rng(seed)
-    dt = np.diff(np.r_[0.0, log[:, 1]])
-    rate = load * value / horizon
-    arr = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    ph = rng.choice(len(CHECKPOINTS), size=nreq, p=w)
+    amps = rng.uniform(AMP_RANGE[0], AMP_RANGE[1], B)
+    amps[0] = 3e-2
     cfg = ConvectionConfig(**WORKLOAD)
-    reqs = []
-    for i in range(nreq):
-        ck = CHECKPOINTS[ph[i]]
-        s0 = np.real(z["s%d" % ck]) * (1.0 + 1e-3 * rng.normal())
-        reqs.append(Request(i, cfg, s0, horizon * dt[ck - 1], arr[i]))
-    srv = Server(slots=B, precision=precision, diagnostics=False, graph_rounds=graph_rounds)
-    srv.submit(reqs)
+    srv = Server(slots=B, precision=precision, diagnostics=True, graph_rounds=graph_rounds)
+    srv.submit([Request("warm%d" % i, cfg, _theta0(amps[i]), 2 * NUM.DTMAX, 0.0)
+                for i in range(B)])
     srv.run()
-    st = srv.stats()
-    st.update(rate_per_s=rate, horizon_steps=horizon, load=load, slots=B,
-              phases=[int((ph == k).sum()) for k in range(len(CHECKPOINTS))])
-    st.pop("buckets", None)
-    return st
+    nwarm = len(srv.done)
+    t_first = srv.now() + 0.05
+    arr = t_first + np.concatenate([[0.0], np.cumsum(rng.exponential(arrive_s / B, B - 1))])
+    reqs = [Request(i, cfg, _theta0(amps[i]), TSTOP, arr[i]) for i in range(B)]
+    srv.submit(reqs)
+    srv.run(progress=lambda p: print("full runs: %s" % json.dumps(p), file=sys.stderr,
+                                     flush=True))
+    done = srv.done[nwarm:]
+    wall = max(r.finished for r in done) - t_first
+    steps = int(sum(r.steps for r in done))
+    lat = np.array([r.latency for r in done])
+    r0 = next(r for r in done if r.rid == 0)
+    return dict(requests=len(done), run_steps=steps, wall_s=wall, steps_per_s=steps / wall,
+                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                steps_per_run=[int(min(r.steps for r in done)),
+                               int(max(r.steps for r in done))],
+                arrival_window_s=float(arr[-1] - arr[0]), amp_range=list(AMP_RANGE),
+                slots=B, precision=precision, graph_rounds=graph_rounds,
+                amp3e2_vs_fp64_oracle=_vs_oracle(r0.theta, r0.diag, _oracle_full()))
+
+
+def plain_full_leg(graph_rounds):
+    """One run (amp 3e-2) alone on the FP64 device stepper, unbatched, from
+    t = 0 to tstop (after a two-step warm-up of the same model)."""
+    import torch
+    from specmc_amd import ConvectionConfig, api
+    m = api.build_model(ConvectionConfig(**WORKLOAD))
+    m.eng.graph_rounds = int(graph_rounds)
+    th0 = _theta0(3e-2)
+    api.march(m, th0, 4e-5, 2)
+    m = api.build_model(ConvectionConfig(**WORKLOAD))
+    m.eng.graph_rounds = int(graph_rounds)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s, info = api.march_adaptive(m, th0, TSTOP)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return dict(steps=info["steps"], wall_s=wall, steps_per_s=info["steps"] / wall,
+                graph_rounds=graph_rounds,
+                vs_fp64_oracle=_vs_oracle(s, m.diagnostics(s), _oracle_full()))
 
 
 class _WallLimit(Exception):
@@ -346,20 +434,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--phases", default="early,hard,late")
-    ap.add_argument("--legs", default="plain,batched,mixed,served",
-                    help="plain: one FP64 run; batched: B FP64 runs; mixed: B runs in mixed "
-                         "precision (the reported value); served: Poisson stream")
+    ap.add_argument("--legs", default="full,plainfull,plain,batched,mixed",
+                    help="full: B served full runs in mixed precision (the reported value); "
+                         "plainfull: one FP64 run to tstop; plain / batched / mixed: the "
+                         "phase legs (one FP64 run, B FP64 runs, B mixed runs)")
     ap.add_argument("--pc", default=None, choices=("lu", "inv"),
                     help="preconditioner block application (default: the engine's)")
     ap.add_argument("--ortho", default=None, choices=("mgs", "bgs2"),
                     help="FGMRES orthogonalisation (default: the engine's)")
     ap.add_argument("--graph-rounds", type=int, default=4,
-                    help="FGMRES rounds per HIP graph replay in the mixed and served legs "
-                         "(0: eager launches)")
-    ap.add_argument("--served-horizon", type=int, default=4)
-    ap.add_argument("--served-load", type=float, default=0.8)
-    ap.add_argument("--served-requests", type=int, default=0,
-                    help="requests of the served leg (default: the batch size)")
+                    help="FGMRES rounds per HIP graph replay in every leg but the clocked "
+                         "batched FP64 one (0: eager launches)")
+    ap.add_argument("--arrive-s", type=float, default=4.0,
+                    help="full runs: the Poisson arrivals span about this many seconds")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the GEMM autotuner pass before the timed legs")
@@ -423,12 +510,13 @@ def main():
     log = z["log"]
     w = phase_weights(log)
     names = ("early", "hard", "late")
-    tuning = None if a.no_tune else _tune(z, a, legs_of(a))
+    legs = legs_of(a)
+    tuning = None if a.no_tune else _tune(z, a, legs)
     from specmc_amd.kclock import KernelClock
-    legs = a.legs.split(",")
     rng = np.random.default_rng(1)
-    clock = KernelClock(ROOF_KERNEL)
-    ubytes = None
+    clock = KernelClock(CLOCKED, capacity=40000)
+    clock32 = KernelClock("dense_solve", capacity=4000)
+    ub = {}
     res = {}
     for ph, step in zip(names, CHECKPOINTS):
         if ph not in a.phases.split(","):
@@ -440,7 +528,7 @@ def main():
         ns = max(a.steps, HARD_MIN_STEPS) if ph == "hard" else a.steps
         r = {"steps": ns}
         if "plain" in legs:
-            plain = _ensemble([s0], pc=a.pc, ortho=a.ortho)
+            plain = _ensemble([s0], pc=a.pc, ortho=a.ortho, graph_rounds=a.graph_rounds)
             n1, t1 = _timed(plain, ns, a.warmup)
             r["plain"] = n1 / t1
             del plain
@@ -449,7 +537,8 @@ def main():
         if "batched" in legs:
             ens = _ensemble(sts, pc=a.pc, ortho=a.ortho)
             n2, t2 = _timed(ens, ns, a.warmup, clock)
-            ubytes = unit_bytes(ens.eng)
+            for k in CLOCKED:
+                ub[k] = unit_bytes(ens.eng, k)
             r.update(batched=n2 / t2, ms_batched_step=1e3 * t2 / ns,
                      bfbt_iters=int(ens.eng.last_bfbt_iters[0]),
                      rescues=int(ens.eng.n_rescue.sum()), pc_apply=ens.eng.pc_apply,
@@ -472,18 +561,33 @@ def main():
                      bfbt_rejections=int(ens.eng.n_rescue.sum()))
             if ref is not None:
                 r["mixed_vs_fp64"] = _diag_errors(ens, *ref)
+            if ph == "hard":
+                # two more steps with the clock on the FP32-factor solves of
+                # the carried rescues (eager launches; not part of the rate)
+                _timed(ens, 2, 0, clock32)
+                ub["dense_solve_fp32"] = unit_bytes(ens.eng, "dense_solve", factor_bytes=4)
             del ens
             torch.cuda.empty_cache()
         res[ph] = r
         print("phase %s: %s" % (ph, json.dumps(r)), file=sys.stderr, flush=True)
     ws = np.array([w[names.index(p)] for p in res])
-    ws = ws / ws.sum()
+    ws = ws / ws.sum() if len(ws) else ws
     # whole-run rates: time per step is the weighted harmonic combination
     rate = lambda leg: (1.0 / sum(wi / res[p][leg] for wi, p in zip(ws, res))
                         if res and all(leg in res[p] for p in res) else None)
     plain_rate, fp64_rate, mixed_rate = rate("plain"), rate("batched"), rate("mixed")
-    value = next(v for v in (mixed_rate, fp64_rate, plain_rate, 0.0) if v is not None)
-    dtype = "f64+f32" if mixed_rate is not None else "f64"
+    full = plainfull = None
+    if "full" in legs:
+        full = full_leg(a.batch, "mixed", a.graph_rounds, a.arrive_s)
+        print("full runs: %s" % json.dumps(full), file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    if "plainfull" in legs:
+        plainfull = plain_full_leg(a.graph_rounds)
+        print("plain full run: %s" % json.dumps(plainfull), file=sys.stderr, flush=True)
+    base = plainfull["steps_per_s"] if plainfull else plain_rate
+    value = next(v for v in ((full or {}).get("steps_per_s"), mixed_rate, fp64_rate,
+                             plain_rate, 0.0) if v is not None)
+    dtype = "f64+f32" if (full or mixed_rate is not None) else "f64"
     out = dict(metric="batched steps/s at 64x96 eta_T deta=1e3 vs plain FP64 GPU; "
                       "p50/p99 run latency",
                value=value, unit="run-steps/s", n_gpus=1, steps=a.steps,
@@ -491,41 +595,55 @@ def main():
                higher_is_better=True, scaling="weak", vs_baseline=None, dtype=dtype,
                data="synthetic",
                config=dict(workload="64x96 eta_T deta_T=1e3 Ra=1e4 unit box (Lx=2 carrier), "
-                                    "BFBT route, tstop=5e-3 trajectory phases",
+                                    "BFBT route, theta0 conductive + amp cos(pi x) sin(pi z), "
+                                    "amp ~ U[2.5e-2, 3.5e-2], t=0 to tstop=5e-3",
+                           value=("served full runs: sum of run-steps / wall over B "
+                                  "Poisson-arriving requests, t=0 to tstop, mixed "
+                                  "precision; --steps/--warmup set the phase legs"
+                                  if full else "phase-weighted rate of the phase legs"),
                            batch=a.batch, Nx=64, Nz=96,
                            precision=("mixed: FP64 Krylov/operators/blocks; FP32 dense "
                                       "rescue LU under FP64 FGMRES refinement, reused "
                                       "across steps; rescued runs carried past BFBT "
                                       "with a 100-iteration probe every 17th solve"
-                                      if mixed_rate is not None else "fp64"),
+                                      if dtype != "f64" else "fp64"),
                            phase_weights=dict(zip(names, [float(x) for x in w]))),
-               plain_fp64_steps_per_s=plain_rate, batched_fp64_steps_per_s=fp64_rate,
-               batched_mixed_steps_per_s=mixed_rate,
-               speedup_vs_plain=(value / plain_rate if plain_rate else None),
+               plain_fp64_steps_per_s=base,
+               speedup_vs_plain=(value / base if base else None),
+               full_runs=full, plain_full=plainfull,
+               phase_legs=dict(plain=plain_rate, batched_fp64=fp64_rate,
+                               batched_mixed=mixed_rate,
+                               mixed_vs_plain=(mixed_rate / plain_rate
+                                               if mixed_rate and plain_rate else None)),
                phases=res, autotune=tuning)
+    if full:
+        out["p50_s"], out["p99_s"] = full["p50_s"], full["p99_s"]
     errs = [res[p]["mixed_vs_fp64"] for p in res if "mixed_vs_fp64" in res[p]]
     if errs:
         out["mixed_error_vs_fp64"] = {k: max(e[k] for e in errs) for k in errs[0]}
-    nl, ms, units = clock.summary()
-    if nl:
-        achieved = ubytes * units / nl / (ms * 1e-3)
+    roofs = {}
+    for k, clk, key in (("pc_apply", clock, "pc_apply"), ("fg_post", clock, "fg_post"),
+                        ("dense_solve_fp64", clock, "dense_solve"),
+                        ("dense_solve_fp32", clock32, "dense_solve")):
+        nl, ms, units = clk.summary(key)
+        if not nl or k not in ub and key not in ub:
+            continue
+        b_unit = ub.get(k, ub.get(key))
+        achieved = b_unit * units / nl / (ms * 1e-3)
         traffic = None
-        if os.path.exists(TRAFFIC_FILE):
-            with open(TRAFFIC_FILE) as f:
+        tf = TRAFFIC_FILES.get(k)
+        if tf and os.path.exists(tf):
+            with open(tf) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
-        out["roofline"] = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9,
-                               unit="GB/s", frac=achieved / PEAK_HBM, traffic=traffic,
-                               kernel=ROOF_KERNEL, launches=nl, mean_launch_us=1e3 * ms,
-                               bytes_per_unit=ubytes, units_per_launch=units / nl)
-    if "served" in legs and value:
-        sv = served_leg(z, log, w, a.batch, value, a.served_horizon, a.served_load,
-                        a.served_requests or a.batch,
-                        "mixed" if mixed_rate is not None else "fp64", a.graph_rounds)
-        out["served"] = sv
-        out["p50_s"], out["p99_s"] = sv["p50_s"], sv["p99_s"]
-        print("served: %s" % json.dumps(sv), file=sys.stderr, flush=True)
+        roofs[k] = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9,
+                        unit="GB/s", frac=achieved / PEAK_HBM, traffic=traffic,
+                        kernel=KERNEL_SYMBOLS[key], launches=nl, mean_launch_us=1e3 * ms,
+                        bytes_per_unit=b_unit, units_per_launch=units / nl)
+    if ROOF_KERNEL in roofs:
+        out["roofline"] = roofs.pop(ROOF_KERNEL)
+        out["rooflines_other"] = roofs
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(z["s%d" % CHECKPOINTS[0]])
+        out["cpu_baseline"] = cpu_baseline(z, w)
     print(json.dumps(out), flush=True)
 
 

@@ -34,16 +34,18 @@ int smx_version(void);
 const char *smx_last_error(void);
 
 /* Launch clock (measurement only; no counterpart in the reference): after
- * smx_clock_start(tag, capacity) HIP events are recorded on the launch stream
- * immediately before and after each of the next `capacity` launches of one
- * entry point (tag 1: smx_fg_post, 2: smx_inv_apply), and those launches
- * count the algorithmic units they process on the device (fg_post: vectors of
- * length n read or written, j + 5 per Arnoldi step j, 2 per restart; inv_apply:
- * systems solved).  smx_clock_stop waits for them, returns the summed
- * duration (ms), the launch count and the unit count, and releases the
- * events.  Process-wide, one tag at a time. */
-int smx_clock_start(int tag, int capacity);
-int smx_clock_stop(double *total_ms, int *count, long long *units);
+ * smx_clock_start(tags, capacity), with tags a bit mask of (1 << tag), HIP
+ * events are recorded on the launch stream immediately before and after each
+ * of the next `capacity` launches of the clocked entry points (tag 1:
+ * smx_fg_post, 2: smx_inv_apply, 3: smx_dense_lu_solve), and those launches
+ * count the algorithmic units they process on the device (fg_post: vectors
+ * of length n read or written, j + 5 per Arnoldi step j, 2 per restart;
+ * inv_apply: systems solved; dense_lu_solve: matrices solved).
+ * smx_clock_stop(tag, ...) waits for that tag's launches, returns their
+ * summed duration (ms), the launch count and the unit count, and releases
+ * its events.  Process-wide. */
+int smx_clock_start(int tags, int capacity);
+int smx_clock_stop(int tag, double *total_ms, int *count, long long *units);
 
 /* General strided batched GEMM
  *   C[z](i,j) = alpha * rs(i) * sum_k A[z](i,k) B[z](k,j) + beta * C[z](i,j)

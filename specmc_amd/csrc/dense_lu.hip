@@ -362,12 +362,14 @@ __device__ __forceinline__ void lud_rowdots4(const float *const rows[4], const f
 template <typename T, typename TV>
 __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
                                                     const int *piv, const TV *b, TV *x,
-                                                    const int *active) {
+                                                    const int *active,
+                                                    unsigned long long *units) {
   extern __shared__ unsigned char smem_raw[];
   T *y = (T *)smem_raw;
   T *Db = y + n;                            // [64][65]
   const int q = blockIdx.x;
   if (active && !active[q]) return;
+  if (units && threadIdx.x == 0) atomicAdd(units, 1ull);
   const T *M = A + (long)q * ms;
   // FP32 factors with rows of a multiple of 4 floats (and 16-byte aligned
   // matrices) take the 16-byte row loads
@@ -478,11 +480,13 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
 #define LUDW_CHUNK 2048
 template <typename TV>
 __global__ void __launch_bounds__(1024) k_ludw_perm(int n, const int *piv, const TV *b, TV *x,
-                                                    const int *active) {
+                                                    const int *active,
+                                                    unsigned long long *units) {
   extern __shared__ int perm[];                 // [n] + [LUDW_CHUNK]
   int *pc = perm + n;
   const int q = blockIdx.x;
   if (active && !active[q]) return;
+  if (units && threadIdx.x == 0) atomicAdd(units, 1ull);
   const int *P = piv + (long)q * n;
   for (int i = threadIdx.x; i < n; i += 1024) perm[i] = i;
   for (int c0 = 0; c0 < n; c0 += LUDW_CHUNK) {
@@ -565,7 +569,8 @@ static int ludw_solve(int nmat, int n, const T *LU, long ms, const int *piv, con
                       const int *active, hipStream_t st) {
   const size_t sh = (size_t)(n + LUDW_CHUNK) * sizeof(int);
   if (!lds_optin(k_ludw_perm<TV>, sh)) return fail_msg("smx_dense_lu_solve: n exceeds LDS");
-  hipLaunchKernelGGL(k_ludw_perm<TV>, dim3(nmat), dim3(1024), sh, st, n, piv, b, x, active);
+  hipLaunchKernelGGL(k_ludw_perm<TV>, dim3(nmat), dim3(1024), sh, st, n, piv, b, x, active,
+                     clk_units(3));
   const int nblk = (n + 63) / 64;
   for (int bk = 0; bk < nblk; ++bk) {
     const int i0 = bk * 64, bs = min(64, n - i0), r0 = i0 + bs;
@@ -603,6 +608,9 @@ extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, lon
   const size_t es = prec == 0 ? 8 : 4;
   const size_t sh = (size_t)n * es + 64 * 65 * es;
   hipStream_t st = STREAM(stream);
+  const bool clk = clk_on(3);
+  if (clk) clk_begin(3, st);
+  unsigned long long *units = clk_units(3);
   if (nmat <= LUDW_MAX_MATS && n >= LUDW_MIN_N && b != x) {
     int rc;
     if (prec == 0)
@@ -614,6 +622,7 @@ extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, lon
     else
       rc = ludw_solve<float, double>(nmat, n, (const float *)LU, mstride, piv,
                                      (const double *)b, (double *)x, active, st);
+    if (clk) clk_end(3, st);
     if (rc) return rc;
     CHECK_LAUNCH("smx_dense_lu_solve");
     return 0;
@@ -623,20 +632,21 @@ extern "C" int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, lon
       return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
     hipLaunchKernelGGL((k_lud_solve<double, double>), dim3(nmat), dim3(1024), sh, st,
                        (const double *)LU, mstride, n, piv, (const double *)b, (double *)x,
-                       active);
+                       active, units);
   } else if (prec == 1) {
     if (!lds_optin(k_lud_solve<float, float>, sh))
       return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
     hipLaunchKernelGGL((k_lud_solve<float, float>), dim3(nmat), dim3(1024), sh, st,
                        (const float *)LU, mstride, n, piv, (const float *)b, (float *)x,
-                       active);
+                       active, units);
   } else {
     if (!lds_optin(k_lud_solve<float, double>, sh))
       return fail_msg("smx_dense_lu_solve: vector exceeds LDS");
     hipLaunchKernelGGL((k_lud_solve<float, double>), dim3(nmat), dim3(1024), sh, st,
                        (const float *)LU, mstride, n, piv, (const double *)b, (double *)x,
-                       active);
+                       active, units);
   }
+  if (clk) clk_end(3, st);
   CHECK_LAUNCH("smx_dense_lu_solve");
   return 0;
 }

@@ -24,6 +24,13 @@ static inline int fail_msg(const char *msg) {
   } while (0)
 #define STREAM(s) ((hipStream_t)(s))
 
+// launch clock (specmc_kernels.hip): per-tag HIP events around a launch and
+// the device counter of the units it processes (nullptr when off)
+bool clk_on(int tag);
+void clk_begin(int tag, hipStream_t st);
+void clk_end(int tag, hipStream_t st);
+unsigned long long *clk_units(int tag);
+
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ---------------------------------------------------------------------------

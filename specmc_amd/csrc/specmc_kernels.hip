@@ -387,96 +387,120 @@ static void launch_gemm(const GemmArgs &g, int nz, hipStream_t st) {
 
 // ---------------------------------------------------------------------------
 // launch clock: HIP events recorded on the launch stream immediately around
-// every launch of one chosen entry point (tag 1: smx_fg_post, 2: smx_inv_apply
-// and smx_lu_solve), for measuring a kernel's average duration inside a timed
-// region without host synchronisation
+// every launch of the chosen entry points (tag 1: smx_fg_post, 2:
+// smx_inv_apply and smx_lu_solve, 3: smx_dense_lu_solve), for measuring a
+// kernel's average duration inside a timed region without host
+// synchronisation.  Several tags can run at once, each with its own events
+// and its own device counter of the algorithmic units its launches process
+// (fg_post: vectors of length n moved; inv_apply: systems solved;
+// dense_lu_solve: matrices solved).
 // ---------------------------------------------------------------------------
-static int g_clk_tag = 0, g_clk_cap = 0, g_clk_n = 0;
-static hipEvent_t *g_clk_ev = nullptr;
-// algorithmic units processed by the clocked launches, counted by the kernels
-// themselves (fg_post: vectors of length n moved; inv_apply: systems solved)
-static unsigned long long *g_clk_units = nullptr;
+#define CLK_TAGS 4
+struct ClkTag {
+  int cap = 0, n = 0;
+  hipEvent_t *ev = nullptr;
+};
+static ClkTag g_clk[CLK_TAGS];
+static unsigned long long *g_clk_units = nullptr;   // [CLK_TAGS], device
 
 // the first failure of an event record, reported by smx_clock_stop
 static hipError_t g_clk_err = hipSuccess;
 
-static inline bool clk_on(int tag) { return g_clk_tag == tag && g_clk_n < g_clk_cap; }
+bool clk_on(int tag) {
+  return tag > 0 && tag < CLK_TAGS && g_clk[tag].n < g_clk[tag].cap;
+}
 static inline void clk_rec(hipEvent_t ev, hipStream_t st) {
   const hipError_t e = hipEventRecord(ev, st);
   if (e != hipSuccess && g_clk_err == hipSuccess) g_clk_err = e;
 }
-static inline void clk_begin(hipStream_t st) { clk_rec(g_clk_ev[2 * g_clk_n], st); }
-static inline void clk_end(hipStream_t st) {
-  clk_rec(g_clk_ev[2 * g_clk_n + 1], st);
-  ++g_clk_n;
+void clk_begin(int tag, hipStream_t st) { clk_rec(g_clk[tag].ev[2 * g_clk[tag].n], st); }
+void clk_end(int tag, hipStream_t st) {
+  clk_rec(g_clk[tag].ev[2 * g_clk[tag].n + 1], st);
+  ++g_clk[tag].n;
 }
 
-static inline unsigned long long *clk_units(int tag) {
-  return clk_on(tag) ? g_clk_units : nullptr;
+unsigned long long *clk_units(int tag) {
+  return clk_on(tag) ? g_clk_units + tag : nullptr;
 }
 
-static void clk_free() {
-  // teardown of the clock's own events and counter: nothing to report
-  for (int i = 0; i < 2 * g_clk_cap; ++i) (void)hipEventDestroy(g_clk_ev[i]);
-  free(g_clk_ev);
-  g_clk_ev = nullptr;
+static void clk_free_tag(int tag) {
+  // teardown of the clock's own events: nothing to report
+  ClkTag &c = g_clk[tag];
+  for (int i = 0; i < 2 * c.cap; ++i) (void)hipEventDestroy(c.ev[i]);
+  free(c.ev);
+  c = ClkTag();
+}
+
+static void clk_free_all() {
+  for (int t = 1; t < CLK_TAGS; ++t) clk_free_tag(t);
   if (g_clk_units) (void)hipFree(g_clk_units);
   g_clk_units = nullptr;
-  g_clk_cap = g_clk_n = g_clk_tag = 0;
 }
 
-extern "C" int smx_clock_start(int tag, int capacity) {
-  clk_free();
-  if (tag <= 0 || capacity <= 0) return 0;
-  g_clk_ev = (hipEvent_t *)calloc((size_t)2 * capacity, sizeof(hipEvent_t));
-  if (!g_clk_ev) return fail_msg("smx_clock_start: out of host memory");
-  for (int i = 0; i < 2 * capacity; ++i) {
-    hipError_t e = hipEventCreate(&g_clk_ev[i]);
-    if (e != hipSuccess) {
-      g_clk_cap = i / 2;
-      clk_free();
-      return fail("smx_clock_start", e);
-    }
-  }
-  g_clk_cap = capacity;
-  hipError_t e = hipMalloc((void **)&g_clk_units, sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMemset(g_clk_units, 0, sizeof(unsigned long long));
+extern "C" int smx_clock_start(int tags, int capacity) {
+  clk_free_all();
+  g_clk_err = hipSuccess;
+  if (tags <= 0 || capacity <= 0) return 0;
+  hipError_t e = hipMalloc((void **)&g_clk_units, CLK_TAGS * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(g_clk_units, 0, CLK_TAGS * sizeof(unsigned long long));
   if (e != hipSuccess) {
-    clk_free();
+    clk_free_all();
     return fail("smx_clock_start", e);
   }
-  g_clk_tag = tag;
+  for (int t = 1; t < CLK_TAGS; ++t) {
+    if (!(tags & (1 << t))) continue;
+    ClkTag &c = g_clk[t];
+    c.ev = (hipEvent_t *)calloc((size_t)2 * capacity, sizeof(hipEvent_t));
+    if (!c.ev) {
+      clk_free_all();
+      return fail_msg("smx_clock_start: out of host memory");
+    }
+    for (int i = 0; i < 2 * capacity; ++i) {
+      e = hipEventCreate(&c.ev[i]);
+      if (e != hipSuccess) {
+        c.cap = i / 2;
+        clk_free_all();
+        return fail("smx_clock_start", e);
+      }
+    }
+    c.cap = capacity;
+  }
   return 0;
 }
 
-extern "C" int smx_clock_stop(double *total_ms, int *count, long long *units) {
+extern "C" int smx_clock_stop(int tag, double *total_ms, int *count, long long *units) {
   if (g_clk_err != hipSuccess) {
     const hipError_t e = g_clk_err;
     g_clk_err = hipSuccess;
-    clk_free();
+    clk_free_all();
     return fail("smx_clock_stop (an event record failed)", e);
   }
+  if (tag <= 0 || tag >= CLK_TAGS) return fail_msg("smx_clock_stop: tag must be 1, 2 or 3");
+  ClkTag &c = g_clk[tag];
   double tot = 0.0;
   if (units) {
     *units = 0;
     if (g_clk_units) {
       unsigned long long u = 0;
-      hipError_t e = hipMemcpy(&u, g_clk_units, sizeof(u), hipMemcpyDeviceToHost);
+      hipError_t e = hipMemcpy(&u, g_clk_units + tag, sizeof(u), hipMemcpyDeviceToHost);
       if (e != hipSuccess) return fail("smx_clock_stop", e);
       *units = (long long)u;
     }
   }
-  for (int i = 0; i < g_clk_n; ++i) {
-    hipError_t e = hipEventSynchronize(g_clk_ev[2 * i + 1]);
+  for (int i = 0; i < c.n; ++i) {
+    hipError_t e = hipEventSynchronize(c.ev[2 * i + 1]);
     if (e != hipSuccess) return fail("smx_clock_stop", e);
     float ms = 0.f;
-    e = hipEventElapsedTime(&ms, g_clk_ev[2 * i], g_clk_ev[2 * i + 1]);
+    e = hipEventElapsedTime(&ms, c.ev[2 * i], c.ev[2 * i + 1]);
     if (e != hipSuccess) return fail("smx_clock_stop", e);
     tot += ms;
   }
   if (total_ms) *total_ms = tot;
-  if (count) *count = g_clk_n;
-  clk_free();
+  if (count) *count = c.n;
+  clk_free_tag(tag);
+  bool any = false;
+  for (int t = 1; t < CLK_TAGS; ++t) any |= g_clk[t].cap > 0;
+  if (!any) clk_free_all();
   return 0;
 }
 
@@ -1365,7 +1389,7 @@ extern "C" int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const v
   if (nmat <= 0) return 0;
   if (N <= 0 || N > LUS_MAXN || per_run <= 0) return fail_msg("smx_inv_apply: bad sizes");
   const bool clk = clk_on(2);
-  if (clk) clk_begin(STREAM(stream));
+  if (clk) clk_begin(2, STREAM(stream));
   const bool pairs = prec == 0 && (N % 2) == 0 && N >= 4 && N <= 512 && g_inv_pairs;
   const int cgs = pairs ? 256 / (N / 2) : 1;
   const int iters = pairs && cgs > 0 ? (N + cgs - 1) / cgs : 1 << 30;
@@ -1389,7 +1413,7 @@ extern "C" int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const v
                        (const float *)Inv, (const float *)rhs, (float *)x, N, tau_zero,
                        (float)rhs_scale, (const float *)kscale, (const float *)bscale,
                        per_run, shared, active, clk_units(2));
-  if (clk) clk_end(STREAM(stream));
+  if (clk) clk_end(2, STREAM(stream));
   CHECK_LAUNCH("smx_inv_apply");
   return 0;
 }
@@ -2933,7 +2957,7 @@ extern "C" int smx_fg_post(int prec, int B, long n, int m, const void *b, void *
   if (ortho != 0 && ortho != 1) return fail_msg("smx_fg_post: ortho must be 0 or 1");
   if (m <= 0 || m > FG_MAXM) return fail_msg("smx_fg_post: restart length must be 1..128");
   const bool clk = clk_on(1);
-  if (clk) clk_begin(STREAM(stream));
+  if (clk) clk_begin(1, STREAM(stream));
   unsigned long long *units = clk_units(1);
   if (prec == 0)
     launch_fg_post<double>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho, units,
@@ -2941,7 +2965,7 @@ extern "C" int smx_fg_post(int prec, int B, long n, int m, const void *b, void *
   else
     launch_fg_post<float>(B, n, m, b, w, zin, Zb, Vb, H, cs, sn, g, ctl, sc, ortho, units,
                           STREAM(stream));
-  if (clk) clk_end(STREAM(stream));
+  if (clk) clk_end(1, STREAM(stream));
   CHECK_LAUNCH("smx_fg_post");
   return 0;
 }

@@ -1,11 +1,12 @@
-"""Launch timing of one kernel inside a timed region.
+"""Launch timing of kernels inside a timed region.
 
-``KernelClock(name)`` arms the library's launch clock (``smx_clock_start``):
-HIP events are recorded on the launch stream immediately before and after
-every launch of that entry point, from the C side, and the kernel itself
-counts the algorithmic units it processes in a device counter.  Nothing is
-added on the host per launch and nothing synchronises inside the timed
-region; ``stop`` collects the spans and the count.
+``KernelClock(names)`` arms the library's launch clock (``smx_clock_start``)
+for one or more entry points: HIP events are recorded on the launch stream
+immediately before and after every launch of each, from the C side, and the
+kernels themselves count the algorithmic units they process in a device
+counter per entry point.  Nothing is added on the host per launch and
+nothing synchronises inside the timed region; ``stop`` collects the spans
+and the counts.
 """
 import ctypes
 
@@ -13,7 +14,7 @@ import torch
 
 from . import runtime
 
-TAGS = {"fg_post": 1, "pc_apply": 2}
+TAGS = {"fg_post": 1, "pc_apply": 2, "dense_solve": 3}
 _armed = [False]
 
 
@@ -24,29 +25,42 @@ def armed():
 
 
 class KernelClock:
-    def __init__(self, name, capacity=60000):
-        if name not in TAGS:
-            raise ValueError("no launch clock for %r (have %s)" % (name, sorted(TAGS)))
-        self.name, self.capacity = name, int(capacity)
-        self.launches, self.total_ms, self.total_units = 0, 0.0, 0
+    def __init__(self, names, capacity=60000):
+        names = (names,) if isinstance(names, str) else tuple(names)
+        for n in names:
+            if n not in TAGS:
+                raise ValueError("no launch clock for %r (have %s)" % (n, sorted(TAGS)))
+        self.names, self.capacity = names, int(capacity)
+        self.stats = {n: [0, 0.0, 0] for n in names}   # launches, total ms, units
 
     def start(self):
-        runtime.load().clock_start(TAGS[self.name], self.capacity)
+        mask = 0
+        for n in self.names:
+            mask |= 1 << TAGS[n]
+        runtime.load().clock_start(mask, self.capacity)
         _armed[0] = True
 
     def stop(self):
         torch.cuda.synchronize()
-        ms, n, u = ctypes.c_double(0.0), ctypes.c_int(0), ctypes.c_longlong(0)
-        runtime.load().clock_stop(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(u))
-        _armed[0] = False
-        if n.value >= self.capacity:
-            raise RuntimeError("launch clock full: raise its capacity above %d" % self.capacity)
-        self.launches += n.value
-        self.total_ms += ms.value
-        self.total_units += u.value
+        lib = runtime.load()
+        try:
+            for n in self.names:
+                ms, c, u = ctypes.c_double(0.0), ctypes.c_int(0), ctypes.c_longlong(0)
+                lib.clock_stop(TAGS[n], ctypes.byref(ms), ctypes.byref(c), ctypes.byref(u))
+                if c.value >= self.capacity:
+                    raise RuntimeError("launch clock of %s full: raise its capacity above %d"
+                                       % (n, self.capacity))
+                st = self.stats[n]
+                st[0] += c.value
+                st[1] += ms.value
+                st[2] += u.value
+        finally:
+            _armed[0] = False
 
-    def summary(self):
-        """-> (launches, mean launch ms, total units) over every start/stop."""
-        if not self.launches:
+    def summary(self, name=None):
+        """-> (launches, mean launch ms, total units) of ``name`` (default: the
+        first clocked entry point) over every start/stop."""
+        n, ms, u = self.stats[self.names[0] if name is None else name]
+        if not n:
             return 0, 0.0, 0
-        return self.launches, self.total_ms / self.launches, self.total_units
+        return n, ms / n, u

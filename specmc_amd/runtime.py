@@ -29,7 +29,7 @@ SIGNATURES = {
     "smx_lu_solve": [I, I, I, P, P, P, P, I, D, P, P, I, I, P, P],
     "smx_invert": [I, I, I, P, I, P, P],
     "smx_clock_start": [I, I],
-    "smx_clock_stop": [P, P, P],
+    "smx_clock_stop": [I, P, P, P],
     "smx_set_gemm_mfma": [I],
     "smx_set_xprod_mfma": [I],
     "smx_set_gj_rows": [I],

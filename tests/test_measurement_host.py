@@ -71,4 +71,15 @@ def test_phase_weights_split_the_recorded_run():
     log[17:, 2] = 400
     w = bench.phase_weights(log)
     np.testing.assert_allclose(w, np.array([10, 7, 3]) / 20.0)
-    assert bench.unit_bytes(types.SimpleNamespace(n=3168, esize=8)) == 25344
+
+
+
+def test_algorithmic_bytes_per_unit_of_the_clocked_kernels():
+    eng = types.SimpleNamespace(n=3168, esize=8, Nz=96)
+    # Arnoldi step: one FP64 vector of n = Nk * Nz per unit
+    assert bench.unit_bytes(eng, "fg_post") == 25344
+    # preconditioner apply: a 96 x 96 FP64 inverse block, its right-hand side
+    # and its solution per system
+    assert bench.unit_bytes(eng, "pc_apply") == (96 * 96 + 2 * 96) * 8 == 75264
+    # dense solve: the factor once, pivots, FP64 right-hand side and solution
+    assert bench.unit_bytes(eng, "dense_solve", 4) == 3168 * 3168 * 4 + 3168 * 4 + 2 * 3168 * 8
