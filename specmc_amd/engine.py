@@ -68,6 +68,16 @@ RESCUE_CARRY = 16
 # costs about as much as ~110 BFBT iterations per solve, so a BFBT that needs
 # more is not worth returning to; a probe that misses keeps the run carried
 RESCUE_PROBE_ITERS = 100
+# mixed precision: a run whose BFBT solve converged but needed more than this
+# many iterations is carried onto the factor route as well (with the same
+# probes back): one solve there costs about as much as 110-160 BFBT
+# iterations at 64x96, and BFBT needs 300-800 for the 90 steps before the
+# rescue phase and after it
+RESCUE_SWITCH_ITERS = 150
+# why a run is carried: its last BFBT attempt was rejected (it then holds a
+# reused factor), or its last BFBT solve was accepted but cost more than
+# RESCUE_SWITCH_ITERS (its first carried solve factors)
+CARRY_REJECTED, CARRY_COSTLY = 1, 2
 DENSE_FP64_MAXN = 8192     # largest n of the FP64 dense factor (FP32: 32768)
 ORTHO = {"mgs": 0, "bgs2": 1}
 ORTHO_DEFAULT = "mgs"      # FGMRES orthogonalisation (block variant: "bgs2")
@@ -1094,7 +1104,7 @@ class Engine:
                 self.schur_matvec(zin, wout, mask)
 
             # a run back from carried solves attempts BFBT as a probe
-            probe = bf & (self.rescue_streak > 0) & self.lag_have & self._carries()
+            probe = bf & self._carry_ready() & self._carries()
             budget = None
             if probe.any():
                 self.n_bfbt_probe[probe] += 1
@@ -1112,7 +1122,9 @@ class Engine:
                 self.last_route[r] = "bfbt"
             bad = bf & (info != 0) & ~(res <= sv.rtol)
             self.carry_run[bf] = 0
-            self.rescue_streak[bf] = bad[bf]
+            costly = bf & ~bad & (it > RESCUE_SWITCH_ITERS) & self._carries()
+            self.rescue_streak[bf] = np.where(bad, CARRY_REJECTED,
+                                              np.where(costly, CARRY_COSTLY, 0))[bf]
             if np.any(bad):
                 self.n_rescue[bad] += 1
                 self._rescue(theta, np.where(bad)[0])
@@ -1132,16 +1144,22 @@ class Engine:
     def _carry_select(self, act):
         """Runs of ``act`` whose Stokes solve skips the BFBT attempt and goes
         straight to the mixed-precision rescue on the run's reused FP32
-        factor: the run's last BFBT attempt was rejected, it holds a factor,
-        and fewer than ``rescue_carry`` solves were carried since that
-        attempt.  The attempt after them is a probe with a budget of
-        RESCUE_PROBE_ITERS iterations.  Nothing is carried in FP64, or when
-        the factors do not fit the lagged-factor store (a fresh FP32
-        factorisation per step would then replace the attempt)."""
+        factor: the run's last BFBT attempt was rejected and it holds a
+        factor, or its last BFBT solve was accepted after more than
+        RESCUE_SWITCH_ITERS iterations; and fewer than ``rescue_carry`` solves
+        were carried since that attempt.  The attempt after them is a probe
+        with a budget of RESCUE_PROBE_ITERS iterations.  Nothing is carried in
+        FP64, or when the factors do not fit the lagged-factor store (a fresh
+        FP32 factorisation per step would then replace the attempt)."""
         if not self._carries():
             return np.zeros(self.B, dtype=bool)
-        return (act & (self.rescue_streak > 0) & self.lag_have
-                & (self.carry_run < self.rescue_carry))
+        return act & self._carry_ready() & (self.carry_run < self.rescue_carry)
+
+    def _carry_ready(self):
+        """Runs in a carry: rejected ones holding a reused factor, and costly
+        ones (which factor on their first carried solve)."""
+        return (((self.rescue_streak == CARRY_REJECTED) & self.lag_have)
+                | (self.rescue_streak == CARRY_COSTLY))
 
     def _carries(self):
         """Whether this engine carries rescued runs (mixed precision, the

@@ -178,6 +178,12 @@ CASES = {
         fields=dict(viscosity_model="eta_T", deta_T=1e3, Ra=1.25e4, Nx=32,
                     Nz=48, Lx=4.0, internal_heating=1.5, precond="bfbt"),
         ic=dict(kind="seeded", seed=2), tstop=4e-4),
+    # the heated 2:1 case at its real shape: 64x96, Lx = 4, Ra = 1.25e4,
+    # H = 1.5, eta_T 1e3, conductive + seeded low modes; its first steps
+    "c2_heated_64x96": dict(
+        fields=dict(viscosity_model="eta_T", deta_T=1e3, Ra=1.25e4, Nx=64, Nz=96,
+                    Lx=4.0, internal_heating=1.5, precond="bfbt"),
+        ic=dict(kind="seeded", seed=2), tstop=3e-4),
     # 64x96 eta_T shape, the first steps of the benchmark run
     "c1_short_64x96": dict(
         fields=dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64,

@@ -9,6 +9,7 @@ from specmc_amd.engine import Engine
 class _Runs:
     """The per-run state the policy reads, with the engine's own methods."""
     _carry_select = Engine._carry_select
+    _carry_ready = Engine._carry_ready
     _carries = Engine._carries
     _lag_bytes = Engine._lag_bytes
 
@@ -36,6 +37,20 @@ def test_carry_truth_table():
     act[5] = False
     got = r._carry_select(act)
     assert got.tolist() == [True, False, False, False, True, False]
+
+
+def test_costly_bfbt_runs_are_carried_without_a_factor():
+    # a run whose last BFBT solve converged after more than RESCUE_SWITCH_ITERS
+    # iterations is carried onto the factor route; its first carried solve
+    # factors, so it needs no factor yet
+    P = E.RESCUE_CARRY
+    r = _Runs(4, 3168, P)
+    r.rescue_streak[:] = [E.CARRY_COSTLY, E.CARRY_COSTLY, E.CARRY_REJECTED, 0]
+    r.lag_have[:] = [False, True, False, True]
+    r.carry_run[:] = [0, P, 0, 0]
+    got = r._carry_select(np.ones(4, dtype=bool))
+    assert got.tolist() == [True, False, False, False]
+    assert 0 < E.RESCUE_PROBE_ITERS <= E.RESCUE_SWITCH_ITERS
 
 
 def test_fp64_never_carries():

@@ -63,7 +63,7 @@ def _ic(model, meta):
 
 
 GOLDEN = ["c0_const_32x24", "const_h_di_16x24", "etaz_16x24", "etaT_bfbt_16x24", "etaT_legacy_16x24",
-          "arrh_bfbt_16x24", "etaT_bfbt_32x48_h", "c1_short_64x96"]
+          "arrh_bfbt_16x24", "etaT_bfbt_32x48_h", "c1_short_64x96", "c2_heated_64x96"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)

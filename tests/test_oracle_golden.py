@@ -13,6 +13,9 @@ from golden_io import branch_names, initial_theta, load, names, rel, seeded_thet
 # the 64x96 rescue-phase start marches two dense 3168 x 3168 rescues on the
 # CPU (about a minute); the rest take seconds
 SLOW = {"c1_s300_64x96", "rescue_weak_128x64"}
+# trajectories whose oracle march takes minutes of CPU (the heated 2:1 case at
+# 64x96, 27 steps); they run when SPECMC_SLOW_ORACLE=1 is set
+SLOW_TRAJ = {"c2_heated_64x96"}
 SMALL = [n for n in names() if not n.startswith(("c1_full", "oracle_"))
          and n not in branch_names()]
 
@@ -48,6 +51,9 @@ def _run(name, counters=None, routes=None):
 
 @pytest.mark.parametrize("name", SMALL)
 def test_oracle_matches_reference_trajectory(name):
+    import os
+    if name in SLOW_TRAJ and os.environ.get("SPECMC_SLOW_ORACLE") != "1":
+        pytest.skip("set SPECMC_SLOW_ORACLE=1 for the minutes-long oracle marches")
     g, m, s, info, dts, sits, tits, states = _run(name)
     meta = g["meta"]
     assert info["steps"] == meta["info"]["steps"]
