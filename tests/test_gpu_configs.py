@@ -59,29 +59,56 @@ def _ensemble_subset():
     return out
 
 
+STREAM_STEPS = 3          # steps per stream request (Ra up to 1e7: dt ~ 1e-7)
+
+
 def _stream_subset():
+    """The first request of each of four grids of the stream draw, each cut
+    to STREAM_STEPS steps: its tstop becomes the time its solo march reaches
+    after them (the served run accumulates the same dt sequence, so it
+    retires after the same steps)."""
     from specmc_amd.workloads import stream_requests
     reqs = stream_requests(n=64, seed=4, rate=1.0)
     pick = []
     for want in ((128, 192), (256, 192), (32, 24), (64, 48)):
         r = next(q for q in reqs if (q.cfg.Nx, q.cfg.Nz) == want)
-        r.tstop = TSTOP
+        r.tstop = 1.0
         pick.append(r)
     for i, r in enumerate(pick):
         r.arrival = 0.2 * i
     return pick
 
 
-def _solo(r, precision):
+class _Stop(Exception):
+    pass
+
+
+def _solo(r, precision, max_steps=None):
+    """Solo march of ``r`` to its tstop, or for ``max_steps`` steps (then
+    ``r.tstop`` is set to the time reached)."""
     from specmc_amd import api
     m = api.build_model(r.cfg, precision=precision)
-    s, info = api.march_adaptive(m, r.theta0, r.tstop)
-    return np.real(s), info["steps"], m.diagnostics(s)
+    last = {}
+
+    def cb(n, t, s):
+        last.update(n=n, t=t, s=s)
+        if max_steps is not None and n >= max_steps:
+            raise _Stop()
+
+    try:
+        s, info = api.march_adaptive(m, r.theta0, r.tstop, on_step=cb)
+        steps = info["steps"]
+    except _Stop:
+        s, steps = last["s"], last["n"]
+        r.tstop = last["t"]
+    print("  solo %s %dx%d Ra=%.3g deta=%.3g: %d steps, t=%.3g" % (
+        r.rid, r.cfg.Nx, r.cfg.Nz, r.cfg.Ra, r.cfg.deta_T, steps, r.tstop), flush=True)
+    return np.real(s), steps, m.diagnostics(s)
 
 
-def _serve_and_compare(reqs, precision):
+def _serve_and_compare(reqs, precision, max_steps=None):
     from specmc_amd.serve import Server
-    solo = {r.rid: _solo(r, precision) for r in reqs}
+    solo = {r.rid: _solo(r, precision, max_steps) for r in reqs}
     srv = Server(slots=2, concurrent=True, graph_rounds=2, precision=precision)
     srv.submit(reqs)
     done = srv.run(max_wall=600)
@@ -152,4 +179,4 @@ def test_heated_2to1_mixed_matches_reference(dev):
 
 
 def test_stream_subset_served_equals_solo(dev):
-    _serve_and_compare(_stream_subset(), "mixed")
+    _serve_and_compare(_stream_subset(), "mixed", max_steps=STREAM_STEPS)
