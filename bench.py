@@ -323,26 +323,30 @@ class _WallLimit(Exception):
     pass
 
 
-def run_heated(precisions=("fp64", "mixed"), max_wall=None):
-    """The heated 2:1 case marched to t = 0.03 with the single-run model,
-    diagnostics at the first steps with t >= 0.01, 0.02, 0.03 (stopped early,
-    and reported as such, past ``max_wall`` seconds per precision)."""
+def run_heated(precisions=("fp64", "mixed"), max_wall=None, routes=("legacy",)):
+    """The heated 2:1 case marched to t = 0.03 with the single-run model on
+    each precision and Stokes route, diagnostics at the first steps with
+    t >= 0.01, 0.02, 0.03 (stopped early, and reported as such, past
+    ``max_wall`` seconds per run)."""
     from specmc_amd import api
     from specmc_amd.workloads import heated_2to1_requests, HEATED_LOG_TIMES
     import threading
     import torch
-    r = heated_2to1_requests()[0]
+    reqs = {route: heated_2to1_requests(route)[0] for route in routes}
     out, errors = {}, []
+    combos = [(p, rt) for rt in routes for p in precisions]
 
-    def one(prec):
+    def one(prec, route):
         try:
             with torch.cuda.stream(torch.cuda.Stream()):
-                march_one(prec)
+                march_one(prec, route)
                 torch.cuda.current_stream().synchronize()
         except BaseException as exc:
             errors.append(exc)
 
-    def march_one(prec):
+    def march_one(prec, route):
+        r = reqs[route]
+        label = "%s/%s" % (prec, route)
         m = api.build_model(r.cfg, precision=prec)
         logs, marks = [], list(HEATED_LOG_TIMES)
         t0 = time.perf_counter()
@@ -357,7 +361,7 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
                 th = m.theta
                 print("heated-2to1 %s: step %d t=%.5f dt=%.3e wall %.1fs | theta its %d info %d "
                       "nonconv %d | stokes %s bfbt its %d res %.1e rescue res %.1e" % (
-                          prec, n, t, t - last.get("t_prev", 0.0), time.perf_counter() - t0,
+                          label, n, t, t - last.get("t_prev", 0.0), time.perf_counter() - t0,
                           th.last_iters, th.last_info, th.n_nonconv,
                           dsc.get("last_stokes_route"), dsc.get("last_bfbt_iters", -1),
                           dsc.get("last_bfbt_true_res", float("nan")),
@@ -368,7 +372,8 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
                 logs.append(dict(t_mark=marks.pop(0), step=n, t=t,
                                  Nu_top=d["Nu_top"], Nu_energy=d["Nu_energy"], urms=d["urms"],
                                  wall_s=time.perf_counter() - t0))
-                print("heated-2to1 %s: %s" % (prec, json.dumps(logs[-1])), file=sys.stderr, flush=True)
+                print("heated-2to1 %s: %s" % (label, json.dumps(logs[-1])), file=sys.stderr,
+                      flush=True)
 
         try:
             s, info = api.march_adaptive(m, r.theta0, r.tstop, on_step=cb)
@@ -377,7 +382,7 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
             s, info, complete = last["s"], dict(steps=last["n"], t=last["t"]), False
         wall = time.perf_counter() - t0
         dsc = m.describe()
-        out[prec] = dict(steps=info["steps"], t=info["t"], complete=complete, wall_s=wall,
+        out[label] = dict(steps=info["steps"], t=info["t"], complete=complete, wall_s=wall,
                          steps_per_s=info["steps"] / wall,
                          log=logs, rescues=dsc.get("n_bfbt_rescue"),
                          rescues_fp32=dsc.get("n_rescue_fp32"),
@@ -388,8 +393,8 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
                          last_rescue_res=dsc.get("last_defect_res"),
                          dt_last=(info["t"] - last.get("t_prev", 0.0)) if last else None,
                          theta=np.real(s))
-    # the runs of the two precisions march concurrently, each on its own stream
-    threads = [threading.Thread(target=one, args=(p,)) for p in precisions]
+    # the runs march concurrently, each on its own stream
+    threads = [threading.Thread(target=one, args=c) for c in combos]
     for t in threads:
         t.start()
     for t in threads:
@@ -398,18 +403,19 @@ def run_heated(precisions=("fp64", "mixed"), max_wall=None):
         raise errors[0]
     res = dict(metric="heated 2:1 run (64x96, Ra=1.25e4, H=1.5, eta_T 1e3) to t=0.03",
                unit="steps/s", config=dict(workload="heated-2to1", Nx=64, Nz=96, Lx=4.0))
-    for prec in precisions:
-        o = dict(out[prec])
+    for key in out:
+        o = dict(out[key])
         o.pop("theta")
-        res[prec] = o
-    if "fp64" in out and "mixed" in out and out["fp64"]["log"] and out["mixed"]["log"]:
-        a, b = out["fp64"], out["mixed"]
-        res["mixed_vs_fp64"] = dict(
-            **{k: max(abs(lb[k] - la[k]) / abs(la[k]) for la, lb in zip(a["log"], b["log"]))
-               for k in ("Nu_top", "Nu_energy", "urms")})
-        if a["complete"] and b["complete"]:
-            res["mixed_vs_fp64"]["theta_l2"] = float(np.linalg.norm(b["theta"] - a["theta"])
-                                                     / np.linalg.norm(a["theta"]))
+        res[key] = o
+    for route in routes:
+        a, b = out.get("fp64/" + route), out.get("mixed/" + route)
+        if a and b and a["log"] and b["log"]:
+            e = {k: max(abs(lb[k] - la[k]) / abs(la[k]) for la, lb in zip(a["log"], b["log"]))
+                 for k in ("Nu_top", "Nu_energy", "urms")}
+            if a["complete"] and b["complete"]:
+                e["theta_l2"] = float(np.linalg.norm(b["theta"] - a["theta"])
+                                      / np.linalg.norm(a["theta"]))
+            res["mixed_vs_fp64/" + route] = e
     return res
 
 
@@ -469,6 +475,8 @@ def main():
     ap.add_argument("--sequential", action="store_true",
                     help="ensemble / stream: step the buckets one after another on one stream")
     ap.add_argument("--heated-precision", default="fp64,mixed")
+    ap.add_argument("--heated-route", default="legacy",
+                    help="heated-2to1: Stokes routes, comma-separated (legacy, bfbt)")
     ap.add_argument("--make-states", action="store_true")
     ap.add_argument("--states-out", default=STATES)
     a = ap.parse_args()
@@ -486,7 +494,8 @@ def main():
     if a.workload != "etaT-64x96":
         from specmc_amd import workloads as W
         if a.workload == "heated-2to1":
-            out = run_heated(tuple(a.heated_precision.split(",")), a.max_wall)
+            out = run_heated(tuple(a.heated_precision.split(",")), a.max_wall,
+                             tuple(a.heated_route.split(",")))
         elif a.workload == "ensemble-256":
             out = run_served(W.ensemble_requests(n=a.requests or 256, tstop=a.tstop),
                              a.slots, a.precision, a.graph_rounds, a.max_wall,

@@ -5,7 +5,8 @@
   3e-2 cos(pi x) sin(pi z); tstop = 5e-3.
 * ``heated_2to1_requests``     -- the heated time-dependent case: Lx = 4 (2:1 box),
   64x96, Ra = 1.25e4, H = 1.5, eta_T deta_T = 1e3, theta0 = heated conductive
-  profile + seeded low modes (``default_rng(2)``); march to t = 0.03.
+  profile + seeded low modes (``default_rng(2)``); march to t = 0.03 on the
+  legacy (default) or BFBT Stokes route.
 * ``ensemble_requests`` -- 256 runs from ``default_rng(3)``: grids uniform
   over {32x24, 64x48, 64x96, 128x96}, Ra = 10^U(4,6), deta_T = 10^U(0,4),
   model uniform over {eta_z, eta_T, arrhenius (Ts = 1, E = 2 ln deta_T)},
@@ -30,7 +31,7 @@ BENCH_64x96_FIELDS = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1e4, Nx=64, Nz
                  precond="bfbt")
 C1_TSTOP = 5e-3
 HEATED_2TO1_FIELDS = dict(viscosity_model="eta_T", deta_T=1e3, Ra=1.25e4, Nx=64, Nz=96, Lx=4.0,
-                 internal_heating=1.5, precond="bfbt")
+                 internal_heating=1.5)
 HEATED_LOG_TIMES = (0.01, 0.02, 0.03)
 
 
@@ -60,8 +61,10 @@ def bench_64x96_theta0(amp=3e-2):
     return bench_ic.conductive_theta(g, amp=amp)
 
 
-def heated_2to1_requests():
-    cfg = ConvectionConfig(**HEATED_2TO1_FIELDS)
+def heated_2to1_requests(route="legacy"):
+    """The heated 2:1 run on Stokes route ``route`` ('legacy', the
+    ConvectionConfig default, or 'bfbt')."""
+    cfg = ConvectionConfig(precond=route, **HEATED_2TO1_FIELDS)
     g = _Grid(64, 96, 4.0)
     th = bench_ic.seeded_theta(g, H=1.5, seed=2, sigma=1e-2)
     return [Request("heated-2to1", cfg, th.real, HEATED_LOG_TIMES[-1])]

@@ -49,6 +49,9 @@ def test_workloads_are_seeded_and_well_formed():
         assert 0.0 <= r.cfg.dissipation_number <= 0.5
     heated = W.heated_2to1_requests()[0]
     assert heated.cfg.Lx == 4.0 and heated.cfg.internal_heating == 1.5 and heated.tstop == 0.03
+    # the ConvectionConfig default route unless another is asked for
+    assert heated.cfg.precond == "legacy"
+    assert W.heated_2to1_requests("bfbt")[0].cfg.precond == "bfbt"
 
 
 def test_theta0_vanishes_at_the_walls():
