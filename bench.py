@@ -286,6 +286,17 @@ def full_leg(B, precision, graph_rounds, arrive_s, seed=8):
                                      flush=True))
     done = srv.done[nwarm:]
     wall = max(r.finished for r in done) - t_first
+    # every slot held one timed request from its admission to the end, so the
+    # engine's per-slot counters are those of the timed requests
+    eng = next(iter(srv.buckets.values())).eng
+    counters = dict(bfbt_iters=int(eng.bfbt_iters.sum()),
+                    bfbt_rejections=int(eng.n_rescue.sum()),
+                    rescues_carried=int(eng.n_rescue_carry.sum()),
+                    bfbt_probes=int(eng.n_bfbt_probe.sum()),
+                    rescue_fgmres_iters=int(eng.rescue_iters.sum()),
+                    factor_refreshes=int(eng.n_refresh.sum()),
+                    refresh_batches=int(eng.n_refresh_batches),
+                    rescues_fp64_redone=int(eng.n_rescue_fp32_redo.sum()))
     steps = int(sum(r.steps for r in done))
     lat = np.array([r.latency for r in done])
     r0 = next(r for r in done if r.rid == 0)
@@ -294,7 +305,7 @@ def full_leg(B, precision, graph_rounds, arrive_s, seed=8):
                 steps_per_run=[int(min(r.steps for r in done)),
                                int(max(r.steps for r in done))],
                 arrival_window_s=float(arr[-1] - arr[0]), amp_range=list(AMP_RANGE),
-                slots=B, precision=precision, graph_rounds=graph_rounds,
+                slots=B, precision=precision, graph_rounds=graph_rounds, counters=counters,
                 amp3e2_vs_fp64_oracle=_vs_oracle(r0.theta, r0.diag, _oracle_full()))
 
 

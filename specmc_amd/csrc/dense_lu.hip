@@ -319,9 +319,11 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
   hipLaunchKernelGGL(k_lud_zero, dim3(cdiv(nmat, 256)), dim3(256), 0, s, info, nmat);
   if (prec == 1) {
     float *a = (float *)A;
-    if (n <= 1024) return lud_run<float, 1, 8>(nmat, n, a, mstride, piv, info, s);
-    if (n <= 2048) return lud_run<float, 2, 8>(nmat, n, a, mstride, piv, info, s);
-    if (n <= 4096) return lud_run<float, 4, 8>(nmat, n, a, mstride, piv, info, s);
+    // 16-column sub-panels: half the launches of the latency-bound panel
+    // chain (a batch of a few matrices costs what its launches cost)
+    if (n <= 1024) return lud_run<float, 1, 16>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 2048) return lud_run<float, 2, 16>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 4096) return lud_run<float, 4, 16>(nmat, n, a, mstride, piv, info, s);
     if (n <= 8192) return lud_run<float, 8, 4>(nmat, n, a, mstride, piv, info, s);
     if (n <= 16384) return lud_run<float, 16, 4>(nmat, n, a, mstride, piv, info, s);
     return lud_run<float, 32, 2>(nmat, n, a, mstride, piv, info, s);
@@ -346,17 +348,103 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
 // bytes in flight of a column-per-lane loop
 __device__ __forceinline__ void lud_rowdots4(const float *const rows[4], const float *y, int j0,
                                              int j1, int lane, float acc[4]) {
-  for (int j = j0 + 4 * lane; j < j1; j += 256) {
-    const float y0 = y[j], y1 = y[j + 1], y2 = y[j + 2], y3 = y[j + 3];
+  int j = j0 + 4 * lane;
+  // two column chunks per pass: eight 16-byte loads in flight per lane
+  for (; j + 256 < j1; j += 512) {
+    float4 a[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = *(const float4 *)(rows[u] + j);
+      c[u] = *(const float4 *)(rows[u] + j + 256);
+    }
+    const float4 ya = *(const float4 *)(y + j), yc = *(const float4 *)(y + j + 256);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      acc[u] = fma(a[u].x, ya.x, acc[u]);
+      acc[u] = fma(a[u].y, ya.y, acc[u]);
+      acc[u] = fma(a[u].z, ya.z, acc[u]);
+      acc[u] = fma(a[u].w, ya.w, acc[u]);
+      acc[u] = fma(c[u].x, yc.x, acc[u]);
+      acc[u] = fma(c[u].y, yc.y, acc[u]);
+      acc[u] = fma(c[u].z, yc.z, acc[u]);
+      acc[u] = fma(c[u].w, yc.w, acc[u]);
+    }
+  }
+  if (j < j1) {
+    const float4 ya = *(const float4 *)(y + j);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const float4 a = *(const float4 *)(rows[u] + j);
-      acc[u] = fma(a.x, y0, acc[u]);
-      acc[u] = fma(a.y, y1, acc[u]);
-      acc[u] = fma(a.z, y2, acc[u]);
-      acc[u] = fma(a.w, y3, acc[u]);
+      acc[u] = fma(a.x, ya.x, acc[u]);
+      acc[u] = fma(a.y, ya.y, acc[u]);
+      acc[u] = fma(a.z, ya.z, acc[u]);
+      acc[u] = fma(a.w, ya.w, acc[u]);
     }
   }
+}
+
+// the 64 x 64 diagonal block of rows/columns [i0, i0 + bs): four entries per
+// thread of the 1024, loaded to registers, then stored to LDS (ld 65)
+template <typename T>
+__device__ __forceinline__ void lud_db_load(const T *M, int n, int i0, int bs, int tid,
+                                            T dbr[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = tid + 1024 * k, r = e >> 6, c = e & 63;
+    dbr[k] = (r < bs && c < bs) ? M[(long)(i0 + r) * n + i0 + c] : T(0);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void lud_db_store(T *Db, const T dbr[4], int bs, int tid) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = tid + 1024 * k, r = e >> 6, c = e & 63;
+    if (r < bs && c < bs) Db[r * 65 + c] = dbr[k];
+  }
+}
+
+// one wavefront solves the diagonal block held in LDS (Db, ld 65) for the
+// 64 entries of yb: unit lower (forward) or upper (backward).  Lane l keeps
+// row l of the block in registers, 16 columns at a time, so the serial
+// column loop reads no LDS; lanes at or past bs carry unused values.
+template <typename T>
+__device__ __forceinline__ void lud_diag_lower(const T *Db, T *yb, int bs, int lane) {
+  T yv = lane < bs ? yb[lane] : T(0);
+#pragma unroll
+  for (int t0 = 0; t0 < 64; t0 += 16) {
+    T r[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r[k] = Db[lane * 65 + t0 + k];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int t = t0 + k;
+      if (t < bs) {
+        const T yt = __shfl(yv, t, 64);
+        if (lane > t) yv = fma(-r[k], yt, yv);
+      }
+    }
+  }
+  if (lane < bs) yb[lane] = yv;
+}
+template <typename T>
+__device__ __forceinline__ void lud_diag_upper(const T *Db, T *yb, int bs, int lane) {
+  T yv = lane < bs ? yb[lane] : T(0);
+#pragma unroll
+  for (int t0 = 48; t0 >= 0; t0 -= 16) {
+    T r[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r[k] = Db[lane * 65 + t0 + k];
+#pragma unroll
+    for (int k = 15; k >= 0; --k) {
+      const int t = t0 + k;
+      if (t < bs) {
+        if (lane == t) yv = yv / r[k];
+        const T yt = __shfl(yv, t, 64);
+        if (lane < t) yv = fma(-r[k], yt, yv);
+      }
+    }
+  }
+  if (lane < bs) yb[lane] = yv;
 }
 
 template <typename T, typename TV>
@@ -392,6 +480,10 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
   const int nblk = (n + 63) / 64;
   for (int bk = 0; bk < nblk; ++bk) {
     const int i0 = bk * 64, bs = min(64, n - i0);
+    // the diagonal block's loads go out before the row products, so their
+    // latency hides behind them
+    T dbr[4];
+    lud_db_load(M, n, i0, bs, tid, dbr);
     // y[i0 + t] -= L[i0 + t, 0:i0] . y[0:i0]
     {
       T acc[4] = {T(0), T(0), T(0), T(0)};
@@ -413,23 +505,15 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
         if (lane == 0 && w + 16 * u < bs) y[i0 + w + 16 * u] -= a;
       }
     }
-    for (int e = tid; e < bs * bs; e += 1024) {
-      const int r = e / bs, c = e % bs;
-      Db[r * 65 + c] = M[(long)(i0 + r) * n + i0 + c];
-    }
+    lud_db_store(Db, dbr, bs, tid);
     __syncthreads();
-    if (w == 0) {
-      T yv = lane < bs ? y[i0 + lane] : T(0);
-      for (int t = 0; t < bs; ++t) {
-        const T yt = __shfl(yv, t, 64);
-        if (lane > t && lane < bs) yv = fma(-Db[lane * 65 + t], yt, yv);
-      }
-      if (lane < bs) y[i0 + lane] = yv;
-    }
+    if (w == 0) lud_diag_lower(Db, y + i0, bs, lane);
     __syncthreads();
   }
   for (int bk = nblk - 1; bk >= 0; --bk) {
     const int i0 = bk * 64, bs = min(64, n - i0), j1 = i0 + bs;
+    T dbr[4];
+    lud_db_load(M, n, i0, bs, tid, dbr);
     {
       T acc[4] = {T(0), T(0), T(0), T(0)};
       const T *rows[4];
@@ -450,20 +534,9 @@ __global__ void __launch_bounds__(1024) k_lud_solve(const T *A, long ms, int n,
         if (lane == 0 && w + 16 * u < bs) y[i0 + w + 16 * u] -= a;
       }
     }
-    for (int e = tid; e < bs * bs; e += 1024) {
-      const int r = e / bs, c = e % bs;
-      Db[r * 65 + c] = M[(long)(i0 + r) * n + i0 + c];
-    }
+    lud_db_store(Db, dbr, bs, tid);
     __syncthreads();
-    if (w == 0) {
-      T yv = lane < bs ? y[i0 + lane] : T(0);
-      for (int t = bs - 1; t >= 0; --t) {
-        if (lane == t) yv = yv / Db[t * 65 + t];
-        const T yt = __shfl(yv, t, 64);
-        if (lane < t) yv = fma(-Db[lane * 65 + t], yt, yv);
-      }
-      if (lane < bs) y[i0 + lane] = yv;
-    }
+    if (w == 0) lud_diag_upper(Db, y + i0, bs, lane);
     __syncthreads();
   }
   for (int i = tid; i < n; i += 1024) x[(long)q * n + i] = (TV)y[i];

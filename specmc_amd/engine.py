@@ -470,7 +470,7 @@ class Engine:
                       self.fp32_backoff, self.n_since_refresh, self.n_refresh,
                       self.n_defect, self.n_lag_krylov, self.last_K, self.lag_iters,
                       self.rescue_streak, self.carry_run, self.n_rescue_carry,
-                      self.n_bfbt_probe):
+                      self.n_bfbt_probe, self.n_singular_factor):
                 a[i] = 0
             self.last_bfbt_res[i] = 0.0
             self.last_defect_res[i] = 0.0
@@ -491,7 +491,7 @@ class Engine:
                       "n_rescue_fp32_redo", "rescue_iters", "fp32_backoff", "last_defect_res",
                       "n_since_refresh", "n_refresh", "n_defect", "n_lag_krylov", "last_K",
                       "last_corr", "lag_iters", "force_refresh", "rescue_streak",
-                      "carry_run", "n_rescue_carry", "n_bfbt_probe")
+                      "carry_run", "n_rescue_carry", "n_bfbt_probe", "n_singular_factor")
 
     def snapshot(self, slot):
         """-> the state of the run in ``slot`` as a dict of numpy arrays:
@@ -981,6 +981,8 @@ class Engine:
         self.last_corr = np.zeros(B)
         self.lag_iters = np.zeros(B, dtype=np.int64)
         self.lag_red = torch.zeros(4, B, dtype=torch.float64, device=dev)
+        self.n_refresh_batches = 0
+        self.n_singular_factor = np.zeros(B, dtype=np.int64)
         if self.legacy.any() and self._lag_bytes() > LAG_STORE_BYTES:
             raise ValueError("precond='legacy' keeps one %d x %d factor per slot: %d slots "
                              "need %.1f GB, above the %.0f GB lagged-factor store"
@@ -1217,6 +1219,9 @@ class Engine:
         piv = torch.empty(nb, n, dtype=torch.int32, device=self.dev)
         info = torch.empty(nb, dtype=torch.int32, device=self.dev)
         L.dense_lu(rprec, nb, n, self.p(A), n * n, self.p(piv), self.p(info), st)
+        zero_pivot = info.cpu().numpy() != 0
+        if zero_pivot.any():
+            self.n_singular_factor[np.asarray(bad)[zero_pivot]] += 1
         mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
         mask[bt] = 1
         if rprec != 0:
@@ -1360,6 +1365,12 @@ class Engine:
         sel[bad] = True
         ref = sel & (~self.lag_have | self.force_refresh | (self.n_since_refresh >= self.lag)
                      | (self.lag_iters > RESCUE_REFRESH_ITERS))
+        if ref.any():
+            # a factorisation batch costs about the same for one matrix as for
+            # dozens (its panel chain is latency-bound), so runs past half of
+            # their factor's life are refreshed with it rather than alone later
+            ref |= sel & self.lag_have & ((2 * self.n_since_refresh >= self.lag)
+                                         | (2 * self.lag_iters > RESCUE_REFRESH_ITERS))
         self._lag_refresh(ref)
         self.n_since_refresh[sel] += 1
         ok, it, res = self._lag_krylov(sel)
@@ -1420,6 +1431,7 @@ class Engine:
         self._lag_store()
         L, st, n = self.L, self.stream(), self.n
         self._A64 = None
+        self.n_refresh_batches += 1
         if runs[-1] - runs[0] + 1 == runs.size:
             g0, g1 = int(runs[0]), int(runs[-1]) + 1
             self._assemble_into(range(g0, g1), self.lag_lu[g0:g1])
@@ -1431,13 +1443,15 @@ class Engine:
                 chunk = runs[c0:c0 + cap]
                 A = torch.empty(chunk.size, n, n, dtype=self.lag_lu.dtype, device=self.dev)
                 piv = torch.empty(chunk.size, n, dtype=torch.int32, device=self.dev)
+                info = torch.empty(chunk.size, dtype=torch.int32, device=self.dev)
                 self._assemble_into(chunk, A)
                 L.dense_lu(self.lag_prec, chunk.size, n, self.p(A), n * n, self.p(piv),
-                           self.p(self.lag_info), st)
+                           self.p(info), st)
                 idx = torch.as_tensor(chunk, device=self.dev)
                 self.lag_lu.index_copy_(0, idx, A)
                 self.lag_piv.index_copy_(0, idx, piv)
-                del A, piv
+                self.lag_info.index_copy_(0, idx, info)
+                del A, piv, info
         self._A64 = None
         rt = torch.as_tensor(runs, device=self.dev)
         self.eta_lag[rt] = self.eta_u[rt]
@@ -1445,6 +1459,14 @@ class Engine:
         self.n_since_refresh[runs] = 0
         self.n_refresh[runs] += 1
         self.force_refresh[runs] = False
+        # a zero pivot: the factor of that run is not used; its solves go to
+        # the FP64 factor of the rescue (mixed precision) and it is counted
+        sing = runs[self.lag_info[rt].cpu().numpy() != 0]
+        if sing.size:
+            self.n_singular_factor[sing] += 1
+            self.lag_have[sing] = False
+            if self.rescue_prec == 1 and self.n <= DENSE_FP64_MAXN:
+                self.fp32_backoff[sing] = RESCUE_FP32_BACKOFF
 
     def _assemble_into(self, runs, A):
         """Dense Schur matrices of ``runs`` at their current eta into the

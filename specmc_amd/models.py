@@ -250,6 +250,13 @@ class _DeviceModel:
                      n_rescue_fp32_iters=int(e.rescue_iters[0]),
                      n_rescue_carry=int(e.n_rescue_carry[0]),
                      n_bfbt_probe=int(e.n_bfbt_probe[0]),
+                     # mixed precision: n_bfbt_rescue counts the rescues after
+                     # a rejected full or probe BFBT attempt, n_rescue_carry
+                     # the solves that went to the factor route without one;
+                     # their sum is what the reference's n_bfbt_rescue counts
+                     # when every solve attempts BFBT first
+                     n_rescued_solves=int(e.n_rescue[0] + e.n_rescue_carry[0]),
+                     n_singular_factor=int(e.n_singular_factor[0]),
                      last_defect_res=float(e.last_defect_res[0]),
                      last_stokes_route=e.last_route[0])
         return d
