@@ -5,7 +5,8 @@
 // solves take and return FP64 vectors and the caller's FP64 defect
 // correction against the matrix-free operator refines the result.
 //
-// Blocked right-looking factorisation, block width NB = 64:
+// Blocked right-looking factorisation, block width NB = 64 (FP64) or 128
+// (FP32):
 //   for each block [j0, j0+jb):
 //     for each sub-panel of PB columns [s0, s0+PB):
 //       k_lud_panel        factor the m x PB sub-panel (rows s0..n-1) with
@@ -15,15 +16,14 @@
 //                          U rows (unit-lower PB x PB solve) and the rank-PB
 //                          update of the block columns to its right
 //     k_lud_laswp          the block's swaps on every column outside it
-//     k_lud_trsm           U12 = L11^{-1} A12
-//     smx_gemm             A22 -= L21 U12 (K = 64)
+//     k_lud_trsm           U12 = L11^{-1} A12 (in 64-row halves for NB = 128)
+//     smx_gemm             A22 -= L21 U12 (K = NB)
 // Row swaps are contiguous in row-major storage, so every kernel reads and
 // writes whole row segments.  Pivots are global row indices: at step i rows i
 // and piv[i] were exchanged (LAPACK getrf order, 0-based).
 #include "smx_common.h"
 #include "specmc_hip.h"
 
-#define LUD_NB 64
 #define LUD_MAXN 8192         // FP64 (sub-panel rows held in registers)
 #define LUD_MAXN_F32 32768    // FP32 (narrower sub-panels; solve vector in LDS)
 
@@ -241,26 +241,27 @@ __global__ void __launch_bounds__(256) k_lud_laswp(T *A, long ms, int n, int j0,
   }
 }
 
-// U12 = L11^{-1} A12 for rows j0..j0+jb-1 and columns j0+jb..n-1
+// U12 = L11^{-1} A12: L11 the unit-lower ib x ib block at (i0, i0) (ib <= 64),
+// rows i0..i0+ib-1 of the columns c0..n-1
 template <typename T>
-__global__ void __launch_bounds__(256) k_lud_trsm(T *A, long ms, int n, int j0, int jb) {
+__global__ void __launch_bounds__(256) k_lud_trsm(T *A, long ms, int n, int i0, int ib, int c0) {
   const int q = blockIdx.y;
   T *M = A + (long)q * ms;
-  __shared__ T L[LUD_NB][LUD_NB + 1];
+  __shared__ T L[64][65];
   const int tid = threadIdx.x;
-  for (int e = tid; e < jb * jb; e += 256) {
-    const int i = e / jb, j = e % jb;
-    L[i][j] = M[(long)(j0 + i) * n + j0 + j];
+  for (int e = tid; e < ib * ib; e += 256) {
+    const int i = e / ib, j = e % ib;
+    L[i][j] = M[(long)(i0 + i) * n + i0 + j];
   }
   __syncthreads();
-  const int c = j0 + jb + blockIdx.x * 256 + tid;
+  const int c = c0 + blockIdx.x * 256 + tid;
   if (c >= n) return;
-  T x[LUD_NB];
+  T x[64];
 #pragma unroll
-  for (int i = 0; i < LUD_NB; ++i) x[i] = (i < jb) ? M[(long)(j0 + i) * n + c] : T(0);
+  for (int i = 0; i < 64; ++i) x[i] = (i < ib) ? M[(long)(i0 + i) * n + c] : T(0);
 #pragma unroll
-  for (int i = 1; i < LUD_NB; ++i) {
-    if (i < jb) {
+  for (int i = 1; i < 64; ++i) {
+    if (i < ib) {
       T a = x[i];
 #pragma unroll
       for (int j = 0; j < i; ++j) a = fma(-L[i][j], x[j], a);
@@ -268,8 +269,8 @@ __global__ void __launch_bounds__(256) k_lud_trsm(T *A, long ms, int n, int j0, 
     }
   }
 #pragma unroll
-  for (int i = 0; i < LUD_NB; ++i)
-    if (i < jb) M[(long)(j0 + i) * n + c] = x[i];
+  for (int i = 0; i < 64; ++i)
+    if (i < ib) M[(long)(i0 + i) * n + c] = x[i];
 }
 
 __global__ void k_lud_zero(int *v, int n) {
@@ -277,10 +278,16 @@ __global__ void k_lud_zero(int *v, int n) {
   if (i < n) v[i] = 0;
 }
 
-template <typename T, int R, int PB>
+// NB: the block width, 64 or 128.  A block of 128 solves its U rows in two
+// 64-row halves with a rank-64 update between them, and updates the trailing
+// matrix with K = 128, which halves the trailing-matrix traffic of the
+// factorisation (the update is HBM-bound at K = 64).
+template <typename T, int R, int PB, int NB>
 static int lud_run(int nmat, int n, T *A, long ms, int *piv, int *info, hipStream_t s) {
-  for (int j0 = 0; j0 < n; j0 += LUD_NB) {
-    const int jb = min(LUD_NB, n - j0);
+  const long lda = n;
+  const int prec = sizeof(T) == 8 ? 0 : 1;
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int jb = min(NB, n - j0);
     for (int s0 = j0; s0 < j0 + jb; s0 += PB) {
       hipLaunchKernelGGL((k_lud_panel<T, R, PB>), dim3(nmat), dim3(1024), 0, s, A, ms, n, s0,
                          piv, info);
@@ -290,17 +297,28 @@ static int lud_run(int nmat, int n, T *A, long ms, int *piv, int *info, hipStrea
     if (n - jb > 0)
       hipLaunchKernelGGL(k_lud_laswp<T>, dim3(cdiv(n - jb, 256), nmat), dim3(256), 0, s, A, ms,
                          n, j0, jb, piv);
-    const int nr = n - j0 - jb;
+    const int c0 = j0 + jb, nr = n - c0;
     if (nr > 0) {
+      const int h = min(64, jb);
       hipLaunchKernelGGL(k_lud_trsm<T>, dim3(cdiv(nr, 256), nmat), dim3(256), 0, s, A, ms, n,
-                         j0, jb);
+                         j0, h, c0);
+      if (jb > h) {
+        // rows j0+64.. of U12 -= L21' (the block's lower-left part) U12's first 64 rows
+        int rc = smx_gemm_ex(3, prec, jb - h, nr, h, -1.0, 1.0,
+                             A + (long)(j0 + h) * lda + j0, lda, 1, ms, 0,
+                             A + (long)j0 * lda + c0, lda, 1, ms, 0,
+                             A + (long)(j0 + h) * lda + c0, lda, 1, ms, 0,
+                             nmat, 1, nullptr, 1, nullptr, 0, 1, (void *)s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_lud_trsm<T>, dim3(cdiv(nr, 256), nmat), dim3(256), 0, s, A, ms,
+                           n, j0 + h, jb - h, c0);
+      }
       CHECK_LAUNCH("smx_dense_lu");
-      const long lda = n;
-      int rc = smx_gemm_ex(3, sizeof(T) == 8 ? 0 : 1, nr, nr, jb, -1.0, 1.0,
-                        A + (long)(j0 + jb) * lda + j0, lda, 1, ms, 0,
-                        A + (long)j0 * lda + j0 + jb, lda, 1, ms, 0,
-                        A + (long)(j0 + jb) * lda + j0 + jb, lda, 1, ms, 0,
-                        nmat, 1, nullptr, 1, nullptr, 0, 1, (void *)s);
+      int rc = smx_gemm_ex(3, prec, nr, nr, jb, -1.0, 1.0,
+                           A + (long)c0 * lda + j0, lda, 1, ms, 0,
+                           A + (long)j0 * lda + c0, lda, 1, ms, 0,
+                           A + (long)c0 * lda + c0, lda, 1, ms, 0,
+                           nmat, 1, nullptr, 1, nullptr, 0, 1, (void *)s);
       if (rc) return rc;
     }
   }
@@ -321,18 +339,18 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
     float *a = (float *)A;
     // 16-column sub-panels: half the launches of the latency-bound panel
     // chain (a batch of a few matrices costs what its launches cost)
-    if (n <= 1024) return lud_run<float, 1, 16>(nmat, n, a, mstride, piv, info, s);
-    if (n <= 2048) return lud_run<float, 2, 16>(nmat, n, a, mstride, piv, info, s);
-    if (n <= 4096) return lud_run<float, 4, 16>(nmat, n, a, mstride, piv, info, s);
-    if (n <= 8192) return lud_run<float, 8, 4>(nmat, n, a, mstride, piv, info, s);
-    if (n <= 16384) return lud_run<float, 16, 4>(nmat, n, a, mstride, piv, info, s);
-    return lud_run<float, 32, 2>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 1024) return lud_run<float, 1, 16, 128>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 2048) return lud_run<float, 2, 16, 128>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 4096) return lud_run<float, 4, 16, 128>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 8192) return lud_run<float, 8, 4, 128>(nmat, n, a, mstride, piv, info, s);
+    if (n <= 16384) return lud_run<float, 16, 4, 128>(nmat, n, a, mstride, piv, info, s);
+    return lud_run<float, 32, 2, 128>(nmat, n, a, mstride, piv, info, s);
   }
   double *a = (double *)A;
-  if (n <= 1024) return lud_run<double, 1, 8>(nmat, n, a, mstride, piv, info, s);
-  if (n <= 2048) return lud_run<double, 2, 8>(nmat, n, a, mstride, piv, info, s);
-  if (n <= 4096) return lud_run<double, 4, 8>(nmat, n, a, mstride, piv, info, s);
-  return lud_run<double, 8, 4>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 1024) return lud_run<double, 1, 8, 64>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 2048) return lud_run<double, 2, 8, 64>(nmat, n, a, mstride, piv, info, s);
+  if (n <= 4096) return lud_run<double, 4, 8, 64>(nmat, n, a, mstride, piv, info, s);
+  return lud_run<double, 8, 4, 64>(nmat, n, a, mstride, piv, info, s);
 }
 
 // ---------------------------------------------------------------------------

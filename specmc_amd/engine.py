@@ -56,7 +56,11 @@ RESCUE_FP32_BACKOFF = 8
 LAG_STORE_BYTES = 96 << 30
 # the mixed-precision rescue refactors a run's reused FP32 factor once the
 # FGMRES on it took more than this many iterations (or after ``lag`` uses)
-RESCUE_REFRESH_ITERS = 16
+RESCUE_REFRESH_ITERS = 24
+# when a run of the mixed rescue refreshes, the runs past this fraction of
+# their factor's life (uses since the refresh against ``lag``, FGMRES
+# iterations against RESCUE_REFRESH_ITERS) are refreshed in the same batch
+CO_REFRESH_FRACTION = 0.5
 # mixed precision: a run whose last BFBT solve was rejected and rescued
 # skips the BFBT attempt (800 iterations that end in the rescue anyway) and
 # solves on its reused FP32 factor directly, for at most this many solves in
@@ -1369,8 +1373,9 @@ class Engine:
             # a factorisation batch costs about the same for one matrix as for
             # dozens (its panel chain is latency-bound), so runs past half of
             # their factor's life are refreshed with it rather than alone later
-            ref |= sel & self.lag_have & ((2 * self.n_since_refresh >= self.lag)
-                                         | (2 * self.lag_iters > RESCUE_REFRESH_ITERS))
+            f = CO_REFRESH_FRACTION
+            ref |= sel & self.lag_have & ((self.n_since_refresh >= f * self.lag)
+                                         | (self.lag_iters > f * RESCUE_REFRESH_ITERS))
         self._lag_refresh(ref)
         self.n_since_refresh[sel] += 1
         ok, it, res = self._lag_krylov(sel)
