@@ -15,12 +15,16 @@ import torch
 from . import runtime
 
 TAGS = {"fg_post": 1, "pc_apply": 2, "dense_solve": 3}
+# entry points launched inside the FGMRES rounds that HIP graphs capture; the
+# dense solves run only in eager rescue solves
+_IN_GRAPH_ROUNDS = ("fg_post", "pc_apply")
 _armed = [False]
 
 
 def armed():
-    """True while a clock is running: launches then record events from the C
-    side, which must not be captured into HIP graphs."""
+    """True while a clock runs on an entry point of the graph-captured FGMRES
+    rounds: its launches then record events from the C side, which must not
+    be captured into HIP graphs, so those rounds run eagerly."""
     return _armed[0]
 
 
@@ -38,7 +42,7 @@ class KernelClock:
         for n in self.names:
             mask |= 1 << TAGS[n]
         runtime.load().clock_start(mask, self.capacity)
-        _armed[0] = True
+        _armed[0] = any(n in _IN_GRAPH_ROUNDS for n in self.names)
 
     def stop(self):
         torch.cuda.synchronize()
