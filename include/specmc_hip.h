@@ -171,6 +171,27 @@ int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, int *piv,
 int smx_dense_lu_solve(int prec, int nmat, int n, const void *LU, long mstride,
                        const int *piv, const void *b, void *x, const int *active,
                        void *stream);
+/* The row order of smx_dense_lu's pivots: perm[q][i] is the row of b that
+ * row i of P_q b takes (the swaps of piv applied in order).  Computed once
+ * per factorisation for smx_dense_lu_solve_mw; the reference applies the
+ * pivots inside sla.lu_solve (lagged_stokes.py:302-343). */
+int smx_dense_lu_perm(int nmat, int n, const int *piv, int *perm, const int *active,
+                      void *stream);
+/* Bytes of device workspace smx_dense_lu_solve_mw needs (0 for bad sizes). */
+long smx_dense_lu_solve_mw_bytes(int prec, int nmat, int n);
+/* x_q = A_q^{-1} b_q as smx_dense_lu_solve (same prec codes, same factors),
+ * with each matrix's two triangular sweeps split over workgroups of 64 rows,
+ * so that a few active matrices still occupy the whole device (the
+ * sla.lu_solve of the defect correction and of the frozen-factor FGMRES
+ * preconditioner, coupled_model.py:544-554, lagged_stokes.py:345-360).
+ * perm from smx_dense_lu_perm; b and x must not overlap; ws holds at least
+ * smx_dense_lu_solve_mw_bytes(prec, nmat, n) bytes of device memory, which
+ * the call clears and uses on its stream: concurrent calls need their own
+ * workspaces.  Its 32-bit word 2 is nonzero after a solve whose wait between
+ * workgroups ran past its bound (the solution is then not valid). */
+int smx_dense_lu_solve_mw(int prec, int nmat, int n, const void *LU, long mstride,
+                          const int *perm, const void *b, void *x, const int *active,
+                          void *ws, long ws_bytes, void *stream);
 
 /* Chebyshev-tau operators, column-major, per (run, mode):
  *   kind 0: D2 - (k^2 + lam_b) I, rows N-2, N-1 = (-1)^n, 1

@@ -682,6 +682,437 @@ static int ludw_solve(int nmat, int n, const T *LU, long ms, const int *piv, con
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// multi-workgroup solve (smx_dense_lu_solve_mw).  The batched solve above
+// streams a whole factor through one CU, so when a masked FGMRES round leaves
+// only some of the matrices active, most CUs idle.  Here each matrix's two
+// sweeps are split into tasks of 64 rows: task t < nb is forward block t,
+// task t >= nb backward block 2 nb - 1 - t (nb = ceil(n / 64)).  A task
+// accumulates the products of its rows with the blocks already solved
+// (left-looking), then solves its diagonal block in one wavefront.  It needs
+// results of lower tasks of its own matrix only.
+//
+// Progress: a persistent grid (no more workgroups than are resident at once)
+// takes tickets from a counter, one per task, a new one only after finishing
+// the last, and works on (task, matrix) = (ticket / cnt, list[ticket % cnt])
+// over the cnt active matrices.  Tasks wait only for lower tickets, all of
+// which are held by running workgroups or done, so the lowest unfinished task
+// can always run; every workgroup leaves once the tickets run out.  Each wait
+// is also bounded (LUDM_SPIN polls), after which the solve sets the error
+// word and goes on.
+//
+// Results pass between workgroups as 64-bit words (value bits | stage << 32)
+// stored and loaded with agent-scope atomics, so no fences are needed: stage
+// 1 is the forward result y, stage 2 the solution x, which overwrites y once
+// every forward task has finished (the last forward task reads every y and
+// the first backward task waits for it).  The workspace is cleared before
+// each solve.  FP64 work types take two words per entry (low and high half).
+// Arithmetic in the factor's type T, as in k_lud_solve.
+// ---------------------------------------------------------------------------
+#define LUDM_SPIN (1 << 22)
+#define LUDM_HDR 64            // bytes: ticket, count, error word
+#define LUDM_UNROLL 2          // column blocks loaded per wait in the products
+
+static inline long ludm_list_bytes(int nmat) { return ((long)nmat * 4 + 255) / 256 * 256; }
+static inline long ludm_ws_bytes(int prec, int nmat, int n) {
+  return LUDM_HDR + ludm_list_bytes(nmat) + (long)nmat * n * (prec == 0 ? 16 : 8);
+}
+
+// perm_q[i] = the row of b that row i of P_q b takes: the swaps of piv
+// replayed in order on an index array in LDS (one workgroup per matrix)
+__global__ void __launch_bounds__(1024) k_lud_perm(int n, const int *piv, int *perm,
+                                                   const int *active) {
+  extern __shared__ int pidx[];                 // [n] + [LUDW_CHUNK]
+  int *pc = pidx + n;
+  const int q = blockIdx.x;
+  if (active && !active[q]) return;
+  const int *P = piv + (long)q * n;
+  for (int i = threadIdx.x; i < n; i += 1024) pidx[i] = i;
+  for (int c0 = 0; c0 < n; c0 += LUDW_CHUNK) {
+    const int c1 = min(n, c0 + LUDW_CHUNK);
+    __syncthreads();
+    for (int i = c0 + threadIdx.x; i < c1; i += 1024) pc[i - c0] = P[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = c0; i < c1; ++i) {
+        const int p = pc[i - c0];
+        if (p != i && p >= 0 && p < n) {
+          const int t = pidx[i];
+          pidx[i] = pidx[p];
+          pidx[p] = t;
+        }
+      }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 1024) perm[(long)q * n + i] = pidx[i];
+}
+
+// clear the workspace: header, the compact list of active matrices (block 0,
+// one wavefront, in matrix order) and every stage word
+__global__ void __launch_bounds__(256) k_ludm_prep(int nmat, long nwords, const int *active,
+                                                   unsigned *hdr, int *list,
+                                                   unsigned long long *tags,
+                                                   unsigned long long *units) {
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int base = 0;
+    for (int q0 = 0; q0 < nmat; q0 += 64) {
+      const int q = q0 + lane;
+      const bool on = q < nmat && (!active || active[q]);
+      const unsigned long long m = __ballot(on);
+      const int before = __popcll(m & ((1ull << lane) - 1ull));
+      if (on) list[base + before] = q;
+      base += __popcll(m);
+    }
+    if (lane == 0) {
+      hdr[0] = 0u;
+      hdr[1] = (unsigned)base;
+      hdr[2] = 0u;
+      if (units) atomicAdd(units, (unsigned long long)base);
+    }
+  }
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < nwords; e += (long)gridDim.x * 256)
+    tags[e] = 0ull;
+}
+
+template <typename T> struct LudmWord;
+template <> struct LudmWord<float> {
+  static constexpr int W = 1;
+  static __device__ __forceinline__ void put(unsigned long long *tg, int i, float v,
+                                             unsigned stage) {
+    __hip_atomic_store(tg + i, ((unsigned long long)stage << 32) | __float_as_uint(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  static __device__ __forceinline__ bool get(unsigned long long *tg, int i, unsigned stage,
+                                             float &v) {
+    const unsigned long long w =
+        __hip_atomic_load(tg + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = __uint_as_float((unsigned)w);
+    return (unsigned)(w >> 32) == stage;
+  }
+  static __device__ __forceinline__ float bcast(float v, int t) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t));
+  }
+};
+template <> struct LudmWord<double> {
+  static constexpr int W = 2;
+  static __device__ __forceinline__ void put(unsigned long long *tg, int i, double v,
+                                             unsigned stage) {
+    const unsigned long long s = (unsigned long long)stage << 32;
+    __hip_atomic_store(tg + 2 * i, s | (unsigned)__double2loint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(tg + 2 * i + 1, s | (unsigned)__double2hiint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  static __device__ __forceinline__ bool get(unsigned long long *tg, int i, unsigned stage,
+                                             double &v) {
+    const unsigned long long lo =
+        __hip_atomic_load(tg + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long hi =
+        __hip_atomic_load(tg + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = __hiloint2double((int)(unsigned)hi, (int)(unsigned)lo);
+    return (unsigned)(lo >> 32) == stage && (unsigned)(hi >> 32) == stage;
+  }
+  static __device__ __forceinline__ double bcast(double v, int t) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), t);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), t);
+    return __hiloint2double(hi, lo);
+  }
+};
+
+// entry i of the stage words once it holds ``stage`` (lanes with need = false
+// take 0 and do not wait); one bounded wait per wavefront
+template <typename T>
+__device__ __forceinline__ T ludm_wait(unsigned long long *tg, int i, bool need,
+                                       unsigned stage, unsigned *err) {
+  T v = T(0);
+  bool ok = !need;
+  for (int it = 0;; ++it) {
+    if (!ok) ok = LudmWord<T>::get(tg, i, stage, v);
+    if (__all(ok)) break;
+    if (it >= LUDM_SPIN) {
+      if ((threadIdx.x & 63) == 0) atomicOr(err, 1u);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return ok ? v : T(0);
+}
+
+// wavefront sum by DPP: row_shr 1, 2, 4, 8 sum each row of 16 lanes into its
+// lane 15, row_bcast 15 and 31 add the rows into lane 63, which is read out
+// (a fixed order; no LDS traffic, unlike the shuffles of wave_sum)
+template <int CTRL, int RMASK>
+__device__ __forceinline__ float ludm_dpp_add(float v) {
+  return v + __int_as_float(
+                 __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RMASK, 0xf, false));
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double ludm_dpp_add(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, RMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, RMASK, 0xf, false);
+  return v + __hiloint2double(hi, lo);
+}
+template <typename T>
+__device__ __forceinline__ T ludm_wave_sum(T v) {
+  v = ludm_dpp_add<0x111, 0xf>(v);   // row_shr:1
+  v = ludm_dpp_add<0x112, 0xf>(v);   // row_shr:2
+  v = ludm_dpp_add<0x114, 0xf>(v);   // row_shr:4
+  v = ludm_dpp_add<0x118, 0xf>(v);   // row_shr:8
+  v = ludm_dpp_add<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+  v = ludm_dpp_add<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+  return LudmWord<T>::bcast(v, 63);
+}
+
+// one task: ticket tk of the cnt active matrices.  LDS: Db the diagonal
+// block, Dl the block of the last dependency (ld 65 each), sred the row sums.
+// The four wavefronts multiply the rows with every solved block but the one
+// solved last, which is usually ready long before; wavefront 0 alone then
+// waits for that last block, multiplies it (a row per lane, from LDS) and
+// solves the diagonal block, so one wavefront sits on the dependency chain.
+template <typename T, typename TV>
+__device__ __forceinline__ void ludm_task(const T *A, long ms, int n, const int *perm,
+                                          const TV *b, TV *x, unsigned *hdr, const int *list,
+                                          unsigned long long *tags, int cnt, int nb, int tk,
+                                          T *Db, T *Dl, T *sred) {
+  typedef LudmWord<T> Wd;
+  // the wavefront index is uniform: row pointers below live in scalar registers
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int task = tk / cnt, q = list[tk - task * cnt];
+  const bool fwd = task < nb;
+  const int blk = fwd ? task : 2 * nb - 1 - task;
+  const int i0 = blk * 64, bs = min(64, n - i0);
+  const T *M = A + (long)q * ms;
+  unsigned long long *tg = tags + (long)q * n * Wd::W;
+  unsigned *err = hdr + 2;
+  // solved column blocks this task multiplies: forward 0 .. blk-1 (y, stage
+  // 1), backward nb-1 down to blk+1 (x, stage 2); the last of them is klast
+  const int nk = fwd ? blk : nb - 1 - blk;
+  const unsigned stage = fwd ? 1u : 2u;
+  const int klast = fwd ? blk - 1 : blk + 1;
+
+  // wavefront 0's right-hand side: (P b) rows of the block (forward)
+  T rhs = T(0);
+  if (fwd && w == 0 && lane < bs) rhs = (T)b[(long)q * n + perm[(long)q * n + i0 + lane]];
+  // the diagonal block and the last dependency's block into LDS (a few rows
+  // in flight per thread: the task's products follow and hide the rest)
+#pragma unroll 2
+  for (int j = 0; j < 16; ++j) {
+    const int e = tid + 256 * j, r = e >> 6, c = e & 63;
+    Db[r * 65 + c] = (r < bs && c < bs) ? M[(long)(i0 + r) * n + i0 + c] : T(0);
+    const int cl = klast * 64 + c;
+    Dl[r * 65 + c] = (nk > 0 && r < bs && cl < n) ? M[(long)(i0 + r) * n + cl] : T(0);
+  }
+  // products of rows i0 + 16 w + u with the solved blocks before klast
+  T acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) acc[u] = T(0);
+  const T *rows[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) rows[u] = M + (long)(i0 + min(16 * w + u, bs - 1)) * n;
+  const int nkb = nk > 0 ? nk - 1 : 0;
+  for (int s = 0; s < nkb; s += LUDM_UNROLL) {
+    T a[LUDM_UNROLL][16];
+    int col[LUDM_UNROLL];
+    bool cok[LUDM_UNROLL];
+#pragma unroll
+    for (int v = 0; v < LUDM_UNROLL; ++v) {
+      const int k = fwd ? s + v : nb - 1 - (s + v);
+      col[v] = k * 64 + lane;
+      cok[v] = (s + v < nkb) && col[v] < n;
+      const int cc = cok[v] ? col[v] : 0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) a[v][u] = cok[v] ? rows[u][cc] : T(0);
+    }
+#pragma unroll
+    for (int v = 0; v < LUDM_UNROLL; ++v) {
+      if (s + v >= nkb) break;
+      const T yk = ludm_wait<T>(tg, cok[v] ? col[v] : 0, cok[v], stage, err);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc[u] = fma(a[v][u], yk, acc[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const T sv = ludm_wave_sum(acc[u]);
+    if (lane == 0) sred[16 * w + u] = sv;
+  }
+  __syncthreads();
+  if (w != 0) return;     // wavefront 0 alone finishes the block
+  if (!fwd) rhs = ludm_wait<T>(tg, i0 + lane, lane < bs, 1u, err);
+  T yv = lane < bs ? rhs - sred[lane] : T(0);
+  if (nk > 0) {
+    // the last dependency: y (or x) of block klast, then row lane of Dl
+    // against it (four partial sums)
+    const int cl = klast * 64 + lane;
+    const T yl = ludm_wait<T>(tg, cl < n ? cl : 0, cl < n, stage, err);
+    T p[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll 1
+    for (int c0 = 0; c0 < 64; c0 += 16) {
+      T d[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) d[k] = Dl[lane * 65 + c0 + k];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) p[k & 3] = fma(d[k], Wd::bcast(yl, c0 + k), p[k & 3]);
+    }
+    yv -= (p[0] + p[1]) + (p[2] + p[3]);
+  }
+  // the serial column steps carry no branches: every multiplier is loaded
+  // unconditionally and masked (backward: scaled by the reciprocal diagonal)
+  // by arithmetic beforehand
+  if (fwd) {
+    // unit lower diagonal block
+#pragma unroll 1
+    for (int t0 = 0; t0 < 64; t0 += 16) {
+      T r[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const T dv = Db[lane * 65 + t0 + k];
+        r[k] = dv * (lane > t0 + k ? T(1) : T(0));
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int t = t0 + k;
+        if (t < bs) yv = fma(-r[k], Wd::bcast(yv, t), yv);
+      }
+    }
+    if (lane < bs) Wd::put(tg, i0 + lane, yv, 1u);
+  } else {
+    // upper diagonal block: row l divided by its diagonal, then a unit sweep
+    const T dd = Db[lane * 65 + lane];
+    const T rd = lane < bs ? T(1) / dd : T(0);
+    yv *= rd;
+#pragma unroll 1
+    for (int t0 = 48; t0 >= 0; t0 -= 16) {
+      T r[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const T dv = Db[lane * 65 + t0 + k];
+        r[k] = dv * (lane < t0 + k ? rd : T(0));
+      }
+#pragma unroll
+      for (int k = 15; k >= 0; --k) {
+        const int t = t0 + k;
+        if (t < bs) yv = fma(-r[k], Wd::bcast(yv, t), yv);
+      }
+    }
+    if (lane < bs) {
+      Wd::put(tg, i0 + lane, yv, 2u);
+      x[(long)q * n + i0 + lane] = (TV)yv;
+    }
+  }
+}
+
+// persistent: at most as many workgroups as are resident at once, each
+// taking tickets until none are left (a workgroup takes its next ticket only
+// after finishing its task, so every ticket below the lowest unfinished one is
+// done and that task can run)
+template <typename T, typename TV>
+__global__ void __launch_bounds__(256) k_ludm_solve(const T *A, long ms, int n,
+                                                    const int *perm, const TV *b, TV *x,
+                                                    unsigned *hdr, const int *list,
+                                                    unsigned long long *tags) {
+  __shared__ T Db[64 * 65], Dl[64 * 65];
+  __shared__ T sred[64];
+  __shared__ int s_tk;
+  const int cnt = (int)hdr[1];
+  const int nb = (n + 63) >> 6;
+  const int total = cnt * 2 * nb;
+  for (;;) {
+    __syncthreads();        // the previous task's LDS reads are done
+    if (threadIdx.x == 0) s_tk = (int)atomicAdd(&hdr[0], 1u);
+    __syncthreads();
+    const int tk = s_tk;
+    if (tk >= total) break;
+    ludm_task<T, TV>(A, ms, n, perm, b, x, hdr, list, tags, cnt, nb, tk, Db, Dl, sred);
+  }
+}
+
+// workgroups of k_ludm_solve<T, TV> resident on the device at once
+template <typename T, typename TV>
+static int ludm_resident() {
+  static int cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ludm_solve<T, TV>, 256, 0) !=
+            hipSuccess ||
+        cus <= 0 || per <= 0)
+      return 1024;
+    cached = cus * per;
+  }
+  return cached;
+}
+
+extern "C" long smx_dense_lu_solve_mw_bytes(int prec, int nmat, int n) {
+  if (nmat <= 0 || n <= 0 || prec < 0 || prec > 2) return 0;
+  return ludm_ws_bytes(prec, nmat, n);
+}
+
+extern "C" int smx_dense_lu_perm(int nmat, int n, const int *piv, int *perm, const int *active,
+                                 void *stream) {
+  if (nmat <= 0) return 0;
+  if (n <= 0 || n > LUD_MAXN_F32) return fail_msg("smx_dense_lu_perm: bad order");
+  const size_t sh = (size_t)(n + LUDW_CHUNK) * sizeof(int);
+  if (!lds_optin(k_lud_perm, sh)) return fail_msg("smx_dense_lu_perm: n exceeds LDS");
+  hipLaunchKernelGGL(k_lud_perm, dim3(nmat), dim3(1024), sh, STREAM(stream), n, piv, perm,
+                     active);
+  CHECK_LAUNCH("smx_dense_lu_perm");
+  return 0;
+}
+
+extern "C" int smx_dense_lu_solve_mw(int prec, int nmat, int n, const void *LU, long mstride,
+                                     const int *perm, const void *b, void *x,
+                                     const int *active, void *ws, long ws_bytes,
+                                     void *stream) {
+  if (nmat <= 0) return 0;
+  if (prec < 0 || prec > 2)
+    return fail_msg("smx_dense_lu_solve_mw: prec must be 0 (FP64), 1 (FP32) or 2 (FP32 "
+                    "factor, FP64 vectors)");
+  if (n <= 0 || n > (prec ? LUD_MAXN_F32 : LUD_MAXN) || mstride < (long)n * n)
+    return fail_msg("smx_dense_lu_solve_mw: bad sizes (1 <= n <= 8192 FP64 / 32768 FP32, "
+                    "mstride >= n*n)");
+  if (!ws || ws_bytes < ludm_ws_bytes(prec, nmat, n))
+    return fail_msg("smx_dense_lu_solve_mw: workspace smaller than "
+                    "smx_dense_lu_solve_mw_bytes(prec, nmat, n)");
+  const size_t vs = prec == 1 ? 4 : 8;
+  const char *bb = (const char *)b, *xb = (const char *)x;
+  if (bb < xb + (size_t)nmat * n * vs && xb < bb + (size_t)nmat * n * vs)
+    return fail_msg("smx_dense_lu_solve_mw: b and x overlap");
+  const int nb = (n + 63) / 64;
+  if ((long)nmat * 2 * nb > 0x7fffffffL) return fail_msg("smx_dense_lu_solve_mw: too many tasks");
+  hipStream_t st = STREAM(stream);
+  unsigned *hdr = (unsigned *)ws;
+  int *list = (int *)((char *)ws + LUDM_HDR);
+  unsigned long long *tags =
+      (unsigned long long *)((char *)ws + LUDM_HDR + ludm_list_bytes(nmat));
+  const long nwords = (long)nmat * n * (prec == 0 ? 2 : 1);
+  const bool clk = clk_on(3);
+  if (clk) clk_begin(3, st);
+  hipLaunchKernelGGL(k_ludm_prep, dim3(min(1024, cdiv(nwords, 256 * 8))), dim3(256), 0, st,
+                     nmat, nwords, active, hdr, list, tags, clk_units(3));
+  const int tasks = nmat * 2 * nb;
+  if (prec == 0)
+    hipLaunchKernelGGL((k_ludm_solve<double, double>),
+                       dim3(min(tasks, ludm_resident<double, double>())), dim3(256), 0, st,
+                       (const double *)LU, mstride, n, perm, (const double *)b, (double *)x,
+                       hdr, list, tags);
+  else if (prec == 1)
+    hipLaunchKernelGGL((k_ludm_solve<float, float>),
+                       dim3(min(tasks, ludm_resident<float, float>())), dim3(256), 0, st,
+                       (const float *)LU, mstride, n, perm, (const float *)b, (float *)x, hdr,
+                       list, tags);
+  else
+    hipLaunchKernelGGL((k_ludm_solve<float, double>),
+                       dim3(min(tasks, ludm_resident<float, double>())), dim3(256), 0, st,
+                       (const float *)LU, mstride, n, perm, (const double *)b, (double *)x,
+                       hdr, list, tags);
+  if (clk) clk_end(3, st);
+  CHECK_LAUNCH("smx_dense_lu_solve_mw");
+  return 0;
+}
+
 // matrices at most this many and order at least LUDW_MIN_N take the wide solve
 #define LUDW_MAX_MATS 16
 #define LUDW_MIN_N 1024

@@ -557,6 +557,8 @@ class Engine:
                     raise ValueError("the snapshot's lagged factor does not fit this engine")
                 self.lag_lu[i] = d(snap["lag_lu"])
                 self.lag_piv[i] = d(snap["lag_piv"])
+                self.L.dense_lu_perm(1, self.n, self.p(self.lag_piv[i]), self.p(self.lag_perm[i]),
+                                     None, self.stream())
                 self.eta_lag[i] = d(snap["eta_lag"]).to(self.dtype)
                 self.lag_have[i] = True
 
@@ -974,7 +976,8 @@ class Engine:
         self.lag = np.array([max(1, int(p.get("lag", 16))) for p in self.params])
         self.eta_drift_tol = float(NUM.ETA_DRIFT_TOL)
         self.lag_prec = 1 if (self.rescue_prec == 1 or self.n > DENSE_FP64_MAXN) else 0
-        self.lag_lu = self.lag_piv = self.lag_info = self.eta_lag = None
+        self.lag_lu = self.lag_piv = self.lag_perm = self.lag_info = self.eta_lag = None
+        self._ludm_ws = None
         self.lag_have = np.zeros(B, dtype=bool)
         self.force_refresh = np.zeros(B, dtype=bool)
         self.n_since_refresh = np.zeros(B, dtype=np.int64)
@@ -1172,6 +1175,21 @@ class Engine:
         reused factors fit the lagged-factor store)."""
         return self.rescue_carry > 0 and self._lag_bytes() <= LAG_STORE_BYTES
 
+    def _dense_solve(self, prec, nmat, LU, perm, b, x, mask):
+        """x[q] = A_q^-1 b[q] for the dense factors LU [nmat][n][n] (row
+        order ``perm`` from smx_dense_lu_perm) of the matrices whose ``mask``
+        entry is set (all when None): each matrix's triangular sweeps spread
+        over workgroups of 64 rows (smx_dense_lu_solve_mw), on this engine's
+        stream with its own workspace."""
+        n = self.n
+        need = self.L.dense_lu_solve_mw_bytes(prec, nmat, n)
+        if self._ludm_ws is None or self._ludm_ws.numel() < need:
+            self._ludm_ws = torch.empty(need, dtype=torch.uint8, device=self.dev)
+        ws = self._ludm_ws
+        self.L.dense_lu_solve_mw(prec, nmat, n, self.p(LU), n * n, self.p(perm), self.p(b),
+                                 self.p(x), None if mask is None else self.p(mask),
+                                 self.p(ws), ws.numel(), self.stream())
+
     def _rescue(self, theta, idx, rprec=None):
         """Recompute rejected BFBT solves with the robust route
         (coupled_model.py:613-654) for every listed run at once: dense Schur LU
@@ -1223,20 +1241,21 @@ class Engine:
         piv = torch.empty(nb, n, dtype=torch.int32, device=self.dev)
         info = torch.empty(nb, dtype=torch.int32, device=self.dev)
         L.dense_lu(rprec, nb, n, self.p(A), n * n, self.p(piv), self.p(info), st)
+        perm = torch.empty(nb, n, dtype=torch.int32, device=self.dev)
+        L.dense_lu_perm(nb, n, self.p(piv), self.p(perm), None, st)
         zero_pivot = info.cpu().numpy() != 0
         if zero_pivot.any():
             self.n_singular_factor[np.asarray(bad)[zero_pivot]] += 1
         mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
         mask[bt] = 1
         if rprec != 0:
-            self._rescue_mixed(theta, bad, bt, A, piv, mask)
+            self._rescue_mixed(theta, bad, bt, A, perm, mask)
             return
 
         def solve(R):
             R = R.contiguous()
             X = torch.empty_like(R)
-            L.dense_lu_solve(0, nb, n, self.p(A), n * n, self.p(piv), self.p(R),
-                             self.p(X), None, st)
+            self._dense_solve(0, nb, A, perm, R, X, None)
             return X
 
         def afree(X):
@@ -1295,7 +1314,7 @@ class Engine:
                 self.last_route[r] = "bfbt->direct"
         sv.x[bt] = x
 
-    def _rescue_mixed(self, theta, bad, bt, A32, piv, mask):
+    def _rescue_mixed(self, theta, bad, bt, A32, perm, mask):
         """Mixed-precision rescue: FGMRES in FP64 on the matrix-free Schur
         operator, right-preconditioned by the FP32 dense LU of the same
         operator (its solves take and return FP64 vectors).  The outer FP64
@@ -1316,8 +1335,7 @@ class Engine:
             # rows whose mask is off this round keep zout: in a residual
             # round it holds the iterate the operator is applied to
             R.copy_(vin[bt])
-            L.dense_lu_solve(2, nb, n, self.p(A32), n * n, self.p(piv), self.p(R),
-                             self.p(X), None, st)
+            self._dense_solve(2, nb, A32, perm, R, X, None)
             on = m[bt].bool()[:, None]
             zout[bt] = torch.where(on, X, zout[bt])
 
@@ -1418,6 +1436,7 @@ class Engine:
             dt = torch.float32 if self.lag_prec else torch.float64
             self.lag_lu = torch.empty(B, n, n, dtype=dt, device=self.dev)
             self.lag_piv = torch.zeros(B, n, dtype=torch.int32, device=self.dev)
+            self.lag_perm = torch.zeros(B, n, dtype=torch.int32, device=self.dev)
             self.lag_info = torch.zeros(B, dtype=torch.int32, device=self.dev)
             self.eta_lag = torch.zeros_like(self.eta_u)
         return self.lag_lu
@@ -1458,6 +1477,8 @@ class Engine:
                 self.lag_info.index_copy_(0, idx, info)
                 del A, piv, info
         self._A64 = None
+        L.dense_lu_perm(self.B, n, self.p(self.lag_piv), self.p(self.lag_perm),
+                        self.p(self._mask(sel)), st)
         rt = torch.as_tensor(runs, device=self.dev)
         self.eta_lag[rt] = self.eta_u[rt]
         self.lag_have[runs] = True
@@ -1509,10 +1530,9 @@ class Engine:
         -> the runs whose correction missed the tolerance."""
         L, st, B, n = self.L, self.stream(), self.B, self.n
         sv = self.ssol
-        LU, piv = self.p(self.lag_lu), self.p(self.lag_piv)
         m = self._mask(sel)
         bn, xn, cn = self.lag_red[1], self.lag_red[2], self.lag_red[3]
-        L.dense_lu_solve(0, B, n, LU, n * n, piv, self.p(sv.b), self.p(sv.x), self.p(m), st)
+        self._dense_solve(0, B, self.lag_lu, self.lag_perm, sv.b, sv.x, m)
         L.rownorm(self.prec, B, n, n, self.p(sv.b), self.p(bn), self.p(m), st)
         L.rownorm(self.prec, B, n, n, self.p(sv.x), self.p(xn), self.p(m), st)
         bnh = np.maximum(bn.cpu().numpy(), 1e-300)
@@ -1526,8 +1546,7 @@ class Engine:
             self.schur_matvec(sv.x, self.s2, lm)
             L.axpby(self.prec, n, B, 1.0, None, self.p(sv.b), -1.0, None, self.p(self.s2),
                     self.p(self.s1), self.p(lm), st)
-            L.dense_lu_solve(0, B, n, LU, n * n, piv, self.p(self.s1), self.p(self.s3),
-                             self.p(lm), st)
+            self._dense_solve(0, B, self.lag_lu, self.lag_perm, self.s1, self.s3, lm)
             L.rownorm(self.prec, B, n, n, self.p(self.s3), self.p(cn), self.p(lm), st)
             L.axpby(self.prec, n, B, 1.0, None, self.p(sv.x), 1.0, None, self.p(self.s3),
                     self.p(sv.x), self.p(lm), st)
@@ -1566,11 +1585,9 @@ class Engine:
         if rs is None:
             rs = self.rsol = Solver(self, n, RESCUE_RESTART, RESCUE_MAXITER, RESCUE_RTOL)
         rs.b.copy_(sv.b)
-        LU, piv = self.p(self.lag_lu), self.p(self.lag_piv)
-
         def precond(vin, zout, mm):
             # masked rows keep zout (a residual round holds the iterate there)
-            L.dense_lu_solve(2, B, n, LU, n * n, piv, self.p(vin), self.p(zout), self.p(mm), st)
+            self._dense_solve(2, B, self.lag_lu, self.lag_perm, vin, zout, mm)
 
         def op(zin, wout, mm):
             self.schur_matvec(zin, wout, mm)

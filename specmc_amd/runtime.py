@@ -37,6 +37,9 @@ SIGNATURES = {
     "smx_set_inv_pairs": [I],
     "smx_dense_lu": [I, I, I, P, L, P, P, P],
     "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P, P],
+    "smx_dense_lu_perm": [I, I, P, P, P, P],
+    "smx_dense_lu_solve_mw_bytes": [I, I, I],
+    "smx_dense_lu_solve_mw": [I, I, I, P, L, P, P, P, P, P, L, P],
     "smx_eta_drift": [I, I, L, P, P, P, P, P],
     "smx_rownorm": [I, I, L, L, P, P, P, P],
     "smx_wdot": [I, I, L, P, L, P, L, P, P, L, P, P],
@@ -70,7 +73,8 @@ SIGNATURES = {
 }
 
 # entry points that return a value rather than a status
-_VALUE_RETURNING = ("smx_schur_apply_work", "smx_schur_apply_static_bytes")
+_VALUE_RETURNING = ("smx_schur_apply_work", "smx_schur_apply_static_bytes",
+                    "smx_dense_lu_solve_mw_bytes")
 
 _lib = None
 
@@ -128,6 +132,7 @@ def load():
         cdll.smx_last_error.restype = ctypes.c_char_p
         cdll.smx_schur_apply_work.restype = ctypes.c_longlong
         cdll.smx_schur_apply_static_bytes.restype = ctypes.c_longlong
+        cdll.smx_dense_lu_solve_mw_bytes.restype = ctypes.c_long
         _lib = _Lib(cdll)
     return _lib
 

@@ -15,7 +15,7 @@ def _declarations():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     out = {}
-    for m in re.finditer(r"^(?:int|long long|const char \*)\s*(smx_\w+)\(([^)]*)\);", text, re.M | re.S):
+    for m in re.finditer(r"^(?:int|long long|long|const char \*)\s*(smx_\w+)\(([^)]*)\);", text, re.M | re.S):
         args = " ".join(m.group(2).split())
         out[m.group(1)] = [] if args in ("", "void") else [a.strip() for a in args.split(",")]
     return out
@@ -49,3 +49,29 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     lib.smx_version.restype = ctypes.c_int
     assert lib.smx_version() >= 1
+
+
+def test_dense_solve_mw_checks_its_arguments():
+    """The multi-workgroup dense solve sizes its workspace and rejects a
+    missing or short workspace and overlapping vectors before any launch."""
+    from specmc_amd import runtime
+    if not os.path.exists(runtime.LIB_PATH):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    lib = runtime.load()
+    n, nm = 3168, 5
+    # header 64 B, the active list rounded to 256 B, one 8-byte word per
+    # entry (FP32 work) or two (FP64 work)
+    assert lib.dense_lu_solve_mw_bytes(2, nm, n) == 64 + 256 + nm * n * 8
+    assert lib.dense_lu_solve_mw_bytes(0, nm, n) == 64 + 256 + nm * n * 16
+    assert lib.dense_lu_solve_mw_bytes(3, nm, n) == 0
+    need = lib.dense_lu_solve_mw_bytes(2, nm, n)
+    fake = 1 << 40           # never dereferenced: the checks come first
+    with pytest.raises(RuntimeError, match="workspace"):
+        lib.dense_lu_solve_mw(2, nm, n, fake, n * n, fake, fake, fake + (1 << 30), None,
+                              None, need, None)
+    with pytest.raises(RuntimeError, match="workspace"):
+        lib.dense_lu_solve_mw(2, nm, n, fake, n * n, fake, fake, fake + (1 << 30), None,
+                              fake, need - 8, None)
+    with pytest.raises(RuntimeError, match="overlap"):
+        lib.dense_lu_solve_mw(2, nm, n, fake, n * n, fake, fake, fake + 8, None, fake,
+                              need, None)
