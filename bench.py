@@ -95,12 +95,12 @@ ROOF_KERNEL = "pc_apply"
 CLOCKED = ("pc_apply", "fg_post", "dense_solve")
 PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
 TRAFFIC_FILES = {
-    "dense_solve_full": os.path.join(ROOT, "profiles", "lud_solve_c1_b256_traffic.json"),
+    "dense_solve_full": os.path.join(ROOT, "profiles", "ludm_solve_c1_b256_traffic.json"),
     "pc_apply": os.path.join(ROOT, "profiles", "inv_apply_c1_b256_traffic.json"),
     "fg_post": os.path.join(ROOT, "profiles", "fg_post_c1_b256_traffic.json"),
 }
 KERNEL_SYMBOLS = {"pc_apply": "k_inv_apply_pairs_pf", "fg_post": "k_fg_post",
-                  "dense_solve": "k_lud_solve"}
+                  "dense_solve": "k_ludm_solve"}
 AMP_RANGE = (2.5e-2, 3.5e-2)   # seeded amplitudes of the full-run requests
 ORACLE_FULL = os.path.join(ROOT, "tests", "golden", "oracle_c1_full_64x96.npz")
 
@@ -170,7 +170,7 @@ def unit_bytes(eng, kernel, factor_bytes=8):
     * pc_apply: one system, the Nz x Nz inverse block read once, its
       right-hand side read and its solution written: (Nz^2 + 2 Nz) * 8;
     * dense_solve: one n x n factor read once (``factor_bytes`` per entry),
-      its pivots, and the FP64 right-hand side and solution."""
+      its row order, and the FP64 right-hand side and solution."""
     n, N = eng.n, eng.Nz
     if kernel == "fg_post":
         return n * eng.esize

@@ -20,7 +20,12 @@ counters.  Its ``that``/``t``/``n``/``fac`` fields are the reference's, so
 the reference's reader and ``load`` read it as a plain state.  A snapshot
 restores into any slot of an engine of the same model kind, shape and
 precision, which is how a served run is preempted and resumed
-(``serve.Server.preempt``).
+(``serve.Server.preempt``).  In FP64 the resumed march equals the uninterrupted
+one bit for bit in any such engine.  In mixed precision it does too when the
+runs around it are the same; otherwise its reused FP32 rescue factor may be
+refreshed at other steps (runs past half of their factor's life join another
+run's refresh), which changes its answers within the rescue's 1e-12 residual,
+not beyond.
 """
 from __future__ import annotations
 

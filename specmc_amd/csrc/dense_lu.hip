@@ -977,7 +977,7 @@ __device__ __forceinline__ void ludm_task(const T *A, long ms, int n, const int 
       }
     }
     if (lane < bs) Wd::put(tg, i0 + lane, yv, 1u);
-  } else {
+  } else if constexpr (sizeof(T) == 4) {
     // upper diagonal block: row l divided by its diagonal, then a unit sweep
     const T dd = Db[lane * 65 + lane];
     const T rd = lane < bs ? T(1) / dd : T(0);
@@ -994,6 +994,28 @@ __device__ __forceinline__ void ludm_task(const T *A, long ms, int n, const int 
       for (int k = 15; k >= 0; --k) {
         const int t = t0 + k;
         if (t < bs) yv = fma(-r[k], Wd::bcast(yv, t), yv);
+      }
+    }
+    if (lane < bs) {
+      Wd::put(tg, i0 + lane, yv, 2u);
+      x[(long)q * n + i0 + lane] = (TV)yv;
+    }
+  } else {
+    // FP64: each entry divided by its diagonal when its column is reached
+    // (the arithmetic of the batched solve; FP64 solves serve the defect
+    // correction, whose stopping point follows the solve's rounding)
+#pragma unroll 1
+    for (int t0 = 48; t0 >= 0; t0 -= 16) {
+      T r[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) r[k] = Db[lane * 65 + t0 + k];
+#pragma unroll
+      for (int k = 15; k >= 0; --k) {
+        const int t = t0 + k;
+        if (t < bs) {
+          const T xt = Wd::bcast(yv, t) / Wd::bcast(r[k], t);
+          yv = lane == t ? xt : (lane < t ? fma(-r[k], xt, yv) : yv);
+        }
       }
     }
     if (lane < bs) {

@@ -128,6 +128,29 @@ def inverse_fits_lds(N, esize):
 # generator state when a capture begins, which it refuses while another
 # thread's capture is open.  Launches and replays from other threads go on.
 _CAPTURE_LOCK = threading.Lock()
+# the transient dense matrices of a rescue or of a scattered lagged-factor
+# refresh (at most RESCUE_BYTES, plus one FP64 assembly matrix) exist for one
+# engine at a time in the process: engines stepping concurrently (the
+# server's bucket workers) take turns, so the server's one RESCUE_BYTES of
+# headroom covers them.  Re-entrant: a rescue may run inside a refresh.
+_RESCUE_GATE = threading.RLock()
+_TRANSIENT_STREAMS = {}
+
+
+def _transient(shape, dtype, dev):
+    """A transient dense rescue buffer.  The caching allocator keeps freed
+    blocks per allocating stream, so buffers of engines on different streams
+    would each keep their own cache; these come from one side stream per
+    device (one pool), marked as used on the caller's stream so a block is
+    not handed out again before the caller's kernels on it are done."""
+    side = _TRANSIENT_STREAMS.get(dev)
+    if side is None:
+        side = _TRANSIENT_STREAMS.setdefault(dev, torch.cuda.Stream(device=dev))
+    cur = torch.cuda.current_stream(dev)
+    with torch.cuda.stream(side):
+        t = torch.empty(*shape, dtype=dtype, device=dev)
+    t.record_stream(cur)
+    return t
 
 
 class Solver:
@@ -1175,13 +1198,22 @@ class Engine:
         reused factors fit the lagged-factor store)."""
         return self.rescue_carry > 0 and self._lag_bytes() <= LAG_STORE_BYTES
 
-    def _dense_solve(self, prec, nmat, LU, perm, b, x, mask):
+    # dense LU solves: "mw" spreads each matrix's sweeps over workgroups of
+    # 64 rows (smx_dense_lu_solve_mw); "batched" runs one workgroup per
+    # matrix (smx_dense_lu_solve)
+    dense_solve_kind = "mw"
+
+    def _dense_solve(self, prec, nmat, LU, perm, b, x, mask, piv=None):
         """x[q] = A_q^-1 b[q] for the dense factors LU [nmat][n][n] (row
-        order ``perm`` from smx_dense_lu_perm) of the matrices whose ``mask``
-        entry is set (all when None): each matrix's triangular sweeps spread
-        over workgroups of 64 rows (smx_dense_lu_solve_mw), on this engine's
-        stream with its own workspace."""
+        order ``perm`` from smx_dense_lu_perm, pivots ``piv``) of the
+        matrices whose ``mask`` entry is set (all when None), on this
+        engine's stream (the multi-workgroup form with its own workspace)."""
         n = self.n
+        if self.dense_solve_kind == "batched":
+            self.L.dense_lu_solve(prec, nmat, n, self.p(LU), n * n, self.p(piv), self.p(b),
+                                  self.p(x), None if mask is None else self.p(mask),
+                                  self.stream())
+            return
         need = self.L.dense_lu_solve_mw_bytes(prec, nmat, n)
         if self._ludm_ws is None or self._ludm_ws.numel() < need:
             self._ludm_ws = torch.empty(need, dtype=torch.uint8, device=self.dev)
@@ -1222,18 +1254,32 @@ class Engine:
             for c0 in range(0, len(bad), cap):
                 self._rescue(theta, bad[c0:c0 + cap], rprec)
             return
+        with _RESCUE_GATE:
+            redo = self._rescue_dense(theta, bad, rprec)
+        if redo:
+            # runs the FP32 factor did not bring to the acceptance test, with
+            # the FP64 factor (after the FP32 matrices are released)
+            self.n_rescue_fp32_redo[redo] += 1
+            self.fp32_backoff[redo] = RESCUE_FP32_BACKOFF
+            self._rescue(theta, redo, rprec=0)
+
+    def _rescue_dense(self, theta, bad, rprec):
+        """The rescue of ``bad`` (at most RESCUE_BYTES of dense matrices) in
+        factor precision ``rprec``.  -> the runs an FP32 factor left above
+        the acceptance test, for the caller to redo in FP64."""
+        sv, n, B = self.ssol, self.n, self.B
         nb = len(bad)
         tol = self.defect_tol
         bt = torch.as_tensor(bad, device=self.dev)
         L, st = self.L, self.stream()
         if rprec == 0:
-            A = torch.empty(nb, n, n, dtype=self.dtype, device=self.dev)
+            A = _transient((nb, n, n), self.dtype, self.dev)
             for q, r in enumerate(bad):
                 self.assemble_schur(r, out=A[q])
         else:
             # assemble each matrix in FP64, keep its FP32 rounding
-            A = torch.empty(nb, n, n, dtype=torch.float32, device=self.dev)
-            A64 = torch.empty(n, n, dtype=self.dtype, device=self.dev)
+            A = _transient((nb, n, n), torch.float32, self.dev)
+            A64 = _transient((n, n), self.dtype, self.dev)
             for q, r in enumerate(bad):
                 self.assemble_schur(r, out=A64)
                 A[q].copy_(A64)
@@ -1249,13 +1295,12 @@ class Engine:
         mask = torch.zeros(B, dtype=torch.int32, device=self.dev)
         mask[bt] = 1
         if rprec != 0:
-            self._rescue_mixed(theta, bad, bt, A, perm, mask)
-            return
+            return self._rescue_mixed(theta, bad, bt, A, perm, mask, piv)
 
         def solve(R):
             R = R.contiguous()
             X = torch.empty_like(R)
-            self._dense_solve(0, nb, A, perm, R, X, None)
+            self._dense_solve(0, nb, A, perm, R, X, None, piv)
             return X
 
         def afree(X):
@@ -1313,16 +1358,17 @@ class Engine:
                 self.last_defect_res[r] = res_direct[q]
                 self.last_route[r] = "bfbt->direct"
         sv.x[bt] = x
+        return []
 
-    def _rescue_mixed(self, theta, bad, bt, A32, perm, mask):
+    def _rescue_mixed(self, theta, bad, bt, A32, perm, mask, piv):
         """Mixed-precision rescue: FGMRES in FP64 on the matrix-free Schur
         operator, right-preconditioned by the FP32 dense LU of the same
         operator (its solves take and return FP64 vectors).  The outer FP64
         residual refines the FP32 factor's answer (GMRES-based iterative
         refinement), and the Krylov acceleration removes the few directions
         in which the FP32 factor is poor, which a plain defect iteration
-        would leave at the FP32 level.  Runs whose true residual misses the
-        acceptance test are redone with the FP64 factor."""
+        would leave at the FP32 level.  -> the runs whose true residual
+        misses the acceptance test (for the FP64 factor)."""
         sv, n, nb, L, st = self.ssol, self.n, len(bad), self.L, self.stream()
         rs = getattr(self, "rsol", None)
         if rs is None:
@@ -1335,7 +1381,7 @@ class Engine:
             # rows whose mask is off this round keep zout: in a residual
             # round it holds the iterate the operator is applied to
             R.copy_(vin[bt])
-            self._dense_solve(2, nb, A32, perm, R, X, None)
+            self._dense_solve(2, nb, A32, perm, R, X, None, piv)
             on = m[bt].bool()[:, None]
             zout[bt] = torch.where(on, X, zout[bt])
 
@@ -1366,10 +1412,8 @@ class Engine:
                 self.last_route[r] = "bfbt->legacy(weak)"
             g = torch.as_tensor(redo, device=self.dev)
             sv.x[g] = rs.x[g]
-        elif redo:
-            self.n_rescue_fp32_redo[redo] += 1
-            self.fp32_backoff[redo] = RESCUE_FP32_BACKOFF
-            self._rescue(theta, redo, rprec=0)
+            return []
+        return redo
 
     def _rescue_lagged(self, theta, bad):
         """Mixed-precision rescue on the reused FP32 factor.  FGMRES on the
@@ -1452,6 +1496,10 @@ class Engine:
         runs = np.where(sel)[0]
         if runs.size == 0:
             return
+        with _RESCUE_GATE:
+            self._lag_refresh_runs(sel, runs)
+
+    def _lag_refresh_runs(self, sel, runs):
         self._lag_store()
         L, st, n = self.L, self.stream(), self.n
         self._A64 = None
@@ -1465,7 +1513,7 @@ class Engine:
             cap = max(1, int(RESCUE_BYTES // (n * n * self.lag_lu.element_size())))
             for c0 in range(0, runs.size, cap):
                 chunk = runs[c0:c0 + cap]
-                A = torch.empty(chunk.size, n, n, dtype=self.lag_lu.dtype, device=self.dev)
+                A = _transient((chunk.size, n, n), self.lag_lu.dtype, self.dev)
                 piv = torch.empty(chunk.size, n, dtype=torch.int32, device=self.dev)
                 info = torch.empty(chunk.size, dtype=torch.int32, device=self.dev)
                 self._assemble_into(chunk, A)
@@ -1503,7 +1551,7 @@ class Engine:
                 self.assemble_schur(int(r), out=A[q])
             else:
                 if self._A64 is None:
-                    self._A64 = torch.empty(self.n, self.n, dtype=self.dtype, device=self.dev)
+                    self._A64 = _transient((self.n, self.n), self.dtype, self.dev)
                 self.assemble_schur(int(r), out=self._A64)
                 A[q].copy_(self._A64)
 
@@ -1532,7 +1580,7 @@ class Engine:
         sv = self.ssol
         m = self._mask(sel)
         bn, xn, cn = self.lag_red[1], self.lag_red[2], self.lag_red[3]
-        self._dense_solve(0, B, self.lag_lu, self.lag_perm, sv.b, sv.x, m)
+        self._dense_solve(0, B, self.lag_lu, self.lag_perm, sv.b, sv.x, m, self.lag_piv)
         L.rownorm(self.prec, B, n, n, self.p(sv.b), self.p(bn), self.p(m), st)
         L.rownorm(self.prec, B, n, n, self.p(sv.x), self.p(xn), self.p(m), st)
         bnh = np.maximum(bn.cpu().numpy(), 1e-300)
@@ -1546,7 +1594,8 @@ class Engine:
             self.schur_matvec(sv.x, self.s2, lm)
             L.axpby(self.prec, n, B, 1.0, None, self.p(sv.b), -1.0, None, self.p(self.s2),
                     self.p(self.s1), self.p(lm), st)
-            self._dense_solve(0, B, self.lag_lu, self.lag_perm, self.s1, self.s3, lm)
+            self._dense_solve(0, B, self.lag_lu, self.lag_perm, self.s1, self.s3, lm,
+                              self.lag_piv)
             L.rownorm(self.prec, B, n, n, self.p(self.s3), self.p(cn), self.p(lm), st)
             L.axpby(self.prec, n, B, 1.0, None, self.p(sv.x), 1.0, None, self.p(self.s3),
                     self.p(sv.x), self.p(lm), st)
@@ -1587,7 +1636,7 @@ class Engine:
         rs.b.copy_(sv.b)
         def precond(vin, zout, mm):
             # masked rows keep zout (a residual round holds the iterate there)
-            self._dense_solve(2, B, self.lag_lu, self.lag_perm, vin, zout, mm)
+            self._dense_solve(2, B, self.lag_lu, self.lag_perm, vin, zout, mm, self.lag_piv)
 
         def op(zin, wout, mm):
             self.schur_matvec(zin, wout, mm)

@@ -153,3 +153,43 @@ def test_mw_solve_large_fp32_factor(eng):
     print("  residuals %s" % ["%.1e" % h for h in hist], flush=True)
     assert int(info.cpu()[0]) == 0
     assert hist[0] < 1e-4 and hist[-1] < 1e-12
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_mw_solve_is_backward_stable(eng, prec):
+    """On matrices of condition ~1e9 (the Schur operators of the rescue reach
+    1e8-1e9) the multi-workgroup solve's normwise backward error
+    |A x - b| / (|A| |x| + |b|) stays within a small factor of the batched
+    solve's and LAPACK's (FP64 factor), or of the FP32 rounding level (FP32
+    factor); its forward error, cond * backward error, is then as good."""
+    import scipy.linalg as sla
+    n, nm = 600, 4
+    rng = np.random.default_rng(11 + prec)
+    Ms = []
+    for _ in range(nm):
+        U, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        V, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        Ms.append((U * np.logspace(0, -9, n)) @ V.T)
+    M = np.stack(Ms)
+    A, piv, perm, _ = _factor(eng, M, prec)
+    b = torch.tensor(rng.normal(size=(nm, n)), device=eng.dev)
+    xb, xm = torch.zeros_like(b), torch.zeros_like(b)
+    ws = _ws(eng, prec, nm, n)
+    eng.L.dense_lu_solve(prec, nm, n, A.data_ptr(), n * n, piv.data_ptr(), b.data_ptr(),
+                         xb.data_ptr(), None, eng.stream())
+    eng.L.dense_lu_solve_mw(prec, nm, n, A.data_ptr(), n * n, perm.data_ptr(), b.data_ptr(),
+                            xm.data_ptr(), None, ws.data_ptr(), ws.numel(), eng.stream())
+    torch.cuda.synchronize()
+    bh, xb, xm = b.cpu().numpy(), xb.cpu().numpy(), xm.cpu().numpy()
+    Mf = M if prec == 0 else A.double().cpu().numpy()
+
+    def berr(q, x):
+        return (np.linalg.norm(Mf[q] @ x - bh[q])
+                / (np.linalg.norm(Mf[q], 2) * np.linalg.norm(x) + np.linalg.norm(bh[q])))
+    for q in range(nm):
+        xl = sla.lu_solve(sla.lu_factor(Mf[q]), bh[q])
+        e_mw, e_b, e_l = berr(q, xm[q]), berr(q, xb[q]), berr(q, xl)
+        print("  q=%d backward error mw %.1e batched %.1e lapack %.1e" % (q, e_mw, e_b, e_l),
+              flush=True)
+        floor = 1e-15 if prec == 0 else 1e-6
+        assert e_mw <= 4.0 * max(e_b, e_l, floor)

@@ -107,9 +107,14 @@ def _check_branch(name):
         assert abs(d[k] - ref) <= 1e-10 * max(abs(ref), 1.0), (k, d[k], ref)
 
 
-# a Stokes solve of the guard state: its operator is conditioned near 1e9, so
-# two FP64 implementations of the same solve agree to about cond * eps
-GUARD_PSI_TOL = 1e-8
+# a Stokes solve of the guard state: its operator is conditioned near 1e9 and
+# the legacy route stops after one correction at defect_tol = 1e-6, so the
+# reference's own answer has a true residual of 5e-7 in its operator.  Two
+# FP64 implementations of the solve agree on psi to about cond * eps ~ 1e-7:
+# LAPACK with two pivot orders (sharing numpy's operator rounding) agree to
+# 2e-10, the device's two dense triangular-solve orders land at 1.9e-9 and
+# 1.7e-8 (true residuals 1.1e-6 and 2.6e-6, at the level of the reference's)
+GUARD_PSI_TOL = 1e-7
 
 
 def _check_probe(name):

@@ -83,3 +83,26 @@ def test_lag_slots_cap_only_the_largest_coupled_grids():
     for Nx, Nz in ((32, 24), (64, 48), (64, 96), (128, 96), (128, 192)):
         for prec in ("fp64", "mixed"):
             assert lag_slots(Nx, Nz, prec) >= 64, (Nx, Nz, prec)
+
+
+def test_rescue_gate_serialises_transient_rescue_memory():
+    """The transient dense matrices of rescues and scattered refreshes exist
+    for one engine at a time: the gate is re-entrant in one thread (a rescue
+    inside a refresh) and excludes a second thread (another bucket worker),
+    so the server's single RESCUE_BYTES of headroom covers them."""
+    import threading
+    from specmc_amd import engine
+    gate = engine._RESCUE_GATE
+    with gate:
+        with gate:
+            pass
+        got = []
+        t = threading.Thread(target=lambda: got.append(gate.acquire(timeout=0.2)))
+        t.start()
+        t.join()
+        assert got == [False]
+    got = []
+    t = threading.Thread(target=lambda: (got.append(gate.acquire(timeout=0.2)), gate.release()))
+    t.start()
+    t.join()
+    assert got == [True]
