@@ -23,6 +23,10 @@ FLAGS = [
     # packed FP32 instructions are disabled for every kernel of this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast",
+    # the register-resident LU sub-panel (k_lud_panel, 4 rows x 16 columns per
+    # thread) is a fully unrolled column loop; at the default limit it stays
+    # rolled and its array goes to scratch memory
+    "-mllvm", "-pragma-unroll-threshold=200000",
 ]
 
 

@@ -49,10 +49,12 @@ __global__ void __launch_bounds__(1024) k_lud_panel(T *A, long ms, int n, int s0
     for (int c = 0; c < PB; ++c)
       v[k][c] = (r < n && s0 + c < n) ? M[(long)r * n + s0 + c] : T(0);
   }
+  // fully unrolled (no early exit), so every v[k][c] is a register: a loop
+  // left rolled indexes v dynamically and puts it in scratch memory
 #pragma unroll
   for (int c = 0; c < PB; ++c) {
     const int col = s0 + c;
-    if (col >= n) break;
+    if (col >= n) continue;   // uniform: the barriers below are skipped by all
     // pivot: first row of largest |a| at or below the diagonal
     T best = T(-1);
     int bi = n;
@@ -65,30 +67,22 @@ __global__ void __launch_bounds__(1024) k_lud_panel(T *A, long ms, int n, int s0
         bi = r;
       }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      const T ob = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
+    wave_argmax_first(best, bi);
     if (lane == 0) {
       rv[w] = best;
       ri[w] = bi;
     }
     __syncthreads();
-    if (tid == 0) {
-      T b = rv[0];
-      int i = ri[0];
-      for (int u = 1; u < 16; ++u)
-        if (rv[u] > b || (rv[u] == b && ri[u] < i)) {
-          b = rv[u];
-          i = ri[u];
-        }
+    if (w == 0) {
+      // the 16 wavefronts' choices (the same total order)
+      T b = lane < 16 ? rv[lane] : T(-1);
+      int i = lane < 16 ? ri[lane] : 0x7fffffff;
+      wave_argmax_first(b, i);
       if (i < col || i >= n) i = col;
-      s_p = i;
-      P[col] = i;
+      if (lane == 0) {
+        s_p = i;
+        P[col] = i;
+      }
     }
     __syncthreads();
     const int p = s_p;

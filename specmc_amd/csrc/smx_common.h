@@ -100,6 +100,54 @@ __device__ T block_min(T v, T *red) {
   return s;
 }
 
+// (best, bi) <- the first largest value over the wavefront: the largest
+// value, the smallest index among equal values.  That is a total order, so
+// any reduction order gives the same pair: DPP row shifts 1, 2, 4, 8 carry
+// each row's choice into its lane 15, row broadcasts 15 and 31 carry them
+// into lane 63, which is read out to every lane (no LDS traffic, unlike
+// shuffles).  Lanes a DPP move leaves out see (-1, INT_MAX), which never
+// wins, so candidates must be >= -1 (pivot searches use |a| or -1).
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void argmax_step(double &best, int &bi) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(best), CTRL, RMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)0xbff00000, __double2hiint(best), CTRL, RMASK,
+                                             0xf, false);
+  const int oi = __builtin_amdgcn_update_dpp(0x7fffffff, bi, CTRL, RMASK, 0xf, false);
+  const double ov = __hiloint2double(hi, lo);
+  if (ov > best || (ov == best && oi < bi)) {
+    best = ov;
+    bi = oi;
+  }
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void argmax_step(float &best, int &bi) {
+  const float ov = __int_as_float(__builtin_amdgcn_update_dpp(
+      (int)0xbf800000, __float_as_int(best), CTRL, RMASK, 0xf, false));
+  const int oi = __builtin_amdgcn_update_dpp(0x7fffffff, bi, CTRL, RMASK, 0xf, false);
+  if (ov > best || (ov == best && oi < bi)) {
+    best = ov;
+    bi = oi;
+  }
+}
+__device__ __forceinline__ double readlane_t(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ float readlane_t(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <typename T>
+__device__ __forceinline__ void wave_argmax_first(T &best, int &bi) {
+  argmax_step<0x111, 0xf>(best, bi);   // row_shr:1
+  argmax_step<0x112, 0xf>(best, bi);   // row_shr:2
+  argmax_step<0x114, 0xf>(best, bi);   // row_shr:4
+  argmax_step<0x118, 0xf>(best, bi);   // row_shr:8
+  argmax_step<0x142, 0xa>(best, bi);   // row_bcast:15 into rows 1 and 3
+  argmax_step<0x143, 0xc>(best, bi);   // row_bcast:31 into rows 2 and 3
+  best = readlane_t(best, 63);
+  bi = __builtin_amdgcn_readlane(bi, 63);
+}
+
 // Tile of this workgroup in the (N tiles x M tiles) plane of one batch entry.
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2: the launch order would put the N tiles of one row block on different

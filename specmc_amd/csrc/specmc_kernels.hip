@@ -903,11 +903,7 @@ __global__ void __launch_bounds__(512) k_gj_invert_reg(double *Am, int N, int pe
       const double v = fabs(cv[rr]);
       if (v > best) { best = v; bi = rr; }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-    }
+    wave_argmax_first(best, bi);
     const int p = bi >= N ? c : bi;
     if (tid == 0) sw[c] = p;
     // a_pc (the pivot after the swap) is cv[p]: every thread forms 1 / a_cc
@@ -2770,7 +2766,8 @@ __global__ void __launch_bounds__(ORTHO ? 512 : 1024) k_fg_post(int B, long n, i
       // modified Gram-Schmidt in the reference's order, with the next PF-1
       // basis vectors already loading while h_i is reduced (a register ring
       // of PF vectors: one step's reduction is short next to an HBM load)
-      constexpr int PF = E <= 4 ? 6 : 3;
+      // (E = 8 with 1024 threads has 128 registers: a ring of 3 spills)
+      constexpr int PF = E <= 4 ? 6 : (E <= 8 ? 2 : 3);
       __shared__ double red2[32];
       __syncthreads();
       T vb[PF][E];

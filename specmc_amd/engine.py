@@ -55,8 +55,11 @@ RESCUE_FP32_BACKOFF = 8
 # reused FP32 factor
 LAG_STORE_BYTES = 96 << 30
 # the mixed-precision rescue refactors a run's reused FP32 factor once the
-# FGMRES on it took more than this many iterations (or after ``lag`` uses)
-RESCUE_REFRESH_ITERS = 24
+# FGMRES on it took more than this many iterations, or after RESCUE_REUSE
+# uses (the BFBT route has no lagged factor of the reference's, so its
+# ``lag`` field does not apply; the legacy route keeps ``lag``)
+RESCUE_REFRESH_ITERS = 32
+RESCUE_REUSE = 24
 # when a run of the mixed rescue refreshes, the runs past this fraction of
 # their factor's life (uses since the refresh against ``lag``, FGMRES
 # iterations against RESCUE_REFRESH_ITERS) are refreshed in the same batch
@@ -74,10 +77,10 @@ RESCUE_CARRY = 16
 RESCUE_PROBE_ITERS = 100
 # mixed precision: a run whose BFBT solve converged but needed more than this
 # many iterations is carried onto the factor route as well (with the same
-# probes back): one solve there costs about as much as 110-160 BFBT
-# iterations at 64x96, and BFBT needs 300-800 for the 90 steps before the
-# rescue phase and after it
-RESCUE_SWITCH_ITERS = 150
+# probes back): with the reused FP32 factor a solve there costs about as much
+# as ~100 BFBT iterations at 64x96, and BFBT needs 100-800 from step ~60 of
+# the 64x96 run on
+RESCUE_SWITCH_ITERS = 100
 # why a run is carried: its last BFBT attempt was rejected (it then holds a
 # reused factor), or its last BFBT solve was accepted but cost more than
 # RESCUE_SWITCH_ITERS (its first carried solve factors)
@@ -1421,7 +1424,7 @@ class Engine:
         factor of an earlier rescue (``_lag_krylov``); the true residual
         decides acceptance, so the factor's age moves only the iteration
         count.  A run refactors at this eta when it has no factor, after
-        ``lag`` uses, or when its last solve took more than
+        RESCUE_REUSE uses, or when its last solve took more than
         RESCUE_REFRESH_ITERS iterations; a miss with an older factor
         refactors and solves again, and a miss with a fresh one is redone
         with the FP64 factor (the weak acceptance above the FP64 factor's
@@ -1429,14 +1432,15 @@ class Engine:
         B, n = self.B, self.n
         sel = np.zeros(B, dtype=bool)
         sel[bad] = True
-        ref = sel & (~self.lag_have | self.force_refresh | (self.n_since_refresh >= self.lag)
+        life = np.where(self.legacy, self.lag, RESCUE_REUSE)
+        ref = sel & (~self.lag_have | self.force_refresh | (self.n_since_refresh >= life)
                      | (self.lag_iters > RESCUE_REFRESH_ITERS))
         if ref.any():
             # a factorisation batch costs about the same for one matrix as for
             # dozens (its panel chain is latency-bound), so runs past half of
             # their factor's life are refreshed with it rather than alone later
             f = CO_REFRESH_FRACTION
-            ref |= sel & self.lag_have & ((self.n_since_refresh >= f * self.lag)
+            ref |= sel & self.lag_have & ((self.n_since_refresh >= f * life)
                                          | (self.lag_iters > f * RESCUE_REFRESH_ITERS))
         self._lag_refresh(ref)
         self.n_since_refresh[sel] += 1

@@ -78,12 +78,14 @@ GROUPS = (
     ("d", "Stokes operator, Krylov and preconditioner",
      lambda k: _role(k) == 2 or k.startswith(("k_fg_", "k_inv_apply", "k_lu_solve",
                                               "k_lu_factor", "k_gj_invert",
-                                              "k_stokes_combine", "k_tau_assemble",
-                                              "k_rowscale", "k_cheb_mult"))),
+                                              "k_stokes_combine", "k_schur_combine",
+                                              "k_tau_assemble", "k_rowscale",
+                                              "k_cheb_mult"))),
     ("e", "theta BDF2 pieces", lambda k: k.startswith(("k_theta_", "k_axpby"))),
     ("f", "CFL and diagnostics", lambda k: k.startswith(("k_maxabs", "k_cfl", "k_diag"))),
     ("rescue", "dense rescue (assembly, LU, solves)",
-     lambda k: _role(k) == 3 or k.startswith(("k_lud_", "k_schur_"))),
+     lambda k: _role(k) == 3 or k.startswith(("k_lud_", "k_ludm_", "k_schur_blocks",
+                                               "k_schur_tau"))),
 )
 
 
