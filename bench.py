@@ -96,6 +96,7 @@ CLOCKED = ("pc_apply", "fg_post", "dense_solve")
 PEAK_HBM = 8.0e12            # MI355X HBM3E, bytes/s
 TRAFFIC_FILES = {
     "dense_solve_full": os.path.join(ROOT, "profiles", "ludm_solve_c1_b256_traffic.json"),
+    "dense_solve_fixed": os.path.join(ROOT, "profiles", "ludm_solve_c1_b256_traffic.json"),
     "pc_apply": os.path.join(ROOT, "profiles", "inv_apply_c1_b256_traffic.json"),
     "fg_post": os.path.join(ROOT, "profiles", "fg_post_c1_b256_traffic.json"),
 }
@@ -319,6 +320,58 @@ def full_leg(B, precision, graph_rounds, arrive_s, seed=8, clock=None):
                 amp3e2_vs_fp64_oracle=_vs_oracle(r0.theta, r0.diag, _oracle_full()))
 
 
+DENSE_ACTIVE = 71   # matrices per launch of the full runs' FP32-factor solves (their mean)
+
+
+def dense_leg(nmat, active=DENSE_ACTIVE, launches=20, clock=None):
+    """The FP32-factor dense solve alone (smx_dense_lu_solve_mw prec 2) on
+    ``nmat`` factors of the 64x96 Schur size (n = 3168, random matrices:
+    bytes and time do not depend on the values), ``active`` of them solved per
+    launch (a seeded draw), ``launches`` launches; with ``clock`` armed, each
+    launch is timed.  The kernel's traffic is measured on this leg."""
+    import torch
+    from specmc_amd import runtime
+    L = runtime.load()
+    n = (WORKLOAD["Nx"] // 2 + 1) * WORKLOAD["Nz"]
+    st = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator(device="cuda").manual_seed(7)
+    A = torch.randn(nmat, n, n, device="cuda", dtype=torch.float32, generator=g)
+    A.diagonal(dim1=1, dim2=2).add_(4.0 * n ** 0.5)
+    piv = torch.zeros(nmat, n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(nmat, dtype=torch.int32, device="cuda")
+    L.dense_lu(1, nmat, n, A.data_ptr(), n * n, piv.data_ptr(), info.data_ptr(), st)
+    perm = torch.zeros(nmat, n, dtype=torch.int32, device="cuda")
+    L.dense_lu_perm(nmat, n, piv.data_ptr(), perm.data_ptr(), None, st)
+    b = torch.randn(nmat, n, device="cuda", dtype=torch.float64, generator=g)
+    x = torch.zeros_like(b)
+    ws = torch.zeros(L.dense_lu_solve_mw_bytes(2, nmat, n), dtype=torch.uint8, device="cuda")
+    rng = np.random.default_rng(8)
+    m = np.zeros(nmat, dtype=np.int32)
+    m[rng.choice(nmat, size=min(active, nmat), replace=False)] = 1
+    mask = torch.as_tensor(m, device="cuda")
+
+    def run():
+        L.dense_lu_solve_mw(2, nmat, n, A.data_ptr(), n * n, perm.data_ptr(), b.data_ptr(),
+                            x.data_ptr(), mask.data_ptr(), ws.data_ptr(), ws.numel(), st)
+    run()
+    torch.cuda.synchronize()
+    if clock is not None:
+        clock.start()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        run()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if clock is not None:
+        clock.stop()
+    err = int(ws[8:12].view(torch.int32).cpu()[0])
+    out = dict(nmat=nmat, active=int(m.sum()), launches=launches, n=n,
+               us_per_launch=1e6 * wall / launches, error_word=err)
+    del A, piv, perm, b, x, ws
+    torch.cuda.empty_cache()
+    return out
+
+
 def plain_full_leg(graph_rounds):
     """One run (amp 3e-2) alone on the FP64 device stepper, unbatched, from
     t = 0 to tstop (after a two-step warm-up of the same model)."""
@@ -461,10 +514,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--phases", default="early,hard,late")
-    ap.add_argument("--legs", default="full,plainfull,plain,batched,mixed",
+    ap.add_argument("--legs", default="full,plainfull,plain,batched,mixed,dense",
                     help="full: B served full runs in mixed precision (the reported value); "
                          "plainfull: one FP64 run to tstop; plain / batched / mixed: the "
-                         "phase legs (one FP64 run, B FP64 runs, B mixed runs)")
+                         "phase legs (one FP64 run, B FP64 runs, B mixed runs); dense: the "
+                         "FP32-factor dense solve alone on B factors at the full runs' "
+                         "mean active count")
     ap.add_argument("--pc", default=None, choices=("lu", "inv"),
                     help="preconditioner block application (default: the engine's)")
     ap.add_argument("--ortho", default=None, choices=("mgs", "bgs2"),
@@ -612,6 +667,11 @@ def main():
         full = full_leg(a.batch, "mixed", a.graph_rounds, a.arrive_s, clock=clock_full)
         print("full runs: %s" % json.dumps(full), file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
+    dense = None
+    clock_dense = KernelClock("dense_solve", capacity=1000)
+    if "dense" in legs:
+        dense = dense_leg(a.batch, clock=clock_dense)
+        print("dense solve: %s" % json.dumps(dense), file=sys.stderr, flush=True)
     if "plainfull" in legs:
         plainfull = plain_full_leg(a.graph_rounds)
         print("plain full run: %s" % json.dumps(plainfull), file=sys.stderr, flush=True)
@@ -641,7 +701,7 @@ def main():
                            phase_weights=dict(zip(names, [float(x) for x in w]))),
                plain_fp64_steps_per_s=base,
                speedup_vs_plain=(value / base if base else None),
-               full_runs=full, plain_full=plainfull,
+               full_runs=full, plain_full=plainfull, dense_solve=dense,
                phase_legs=dict(plain=plain_rate, batched_fp64=fp64_rate,
                                batched_mixed=mixed_rate,
                                mixed_vs_plain=(mixed_rate / plain_rate
@@ -653,12 +713,14 @@ def main():
     if errs:
         out["mixed_error_vs_fp64"] = {k: max(e[k] for e in errs) for k in errs[0]}
     roofs = {}
-    if full:
+    if full or dense:
         import types
         shape = types.SimpleNamespace(n=(WORKLOAD["Nx"] // 2 + 1) * WORKLOAD["Nz"],
                                       Nz=WORKLOAD["Nz"], esize=8)
-        ub["dense_solve_full"] = unit_bytes(shape, "dense_solve", factor_bytes=4)
+        ub["dense_solve_full"] = ub["dense_solve_fixed"] = unit_bytes(shape, "dense_solve",
+                                                                      factor_bytes=4)
     for k, clk, key in (("dense_solve_full", clock_full, "dense_solve"),
+                        ("dense_solve_fixed", clock_dense, "dense_solve"),
                         ("pc_apply", clock, "pc_apply"), ("fg_post", clock, "fg_post"),
                         ("dense_solve_fp64", clock, "dense_solve"),
                         ("dense_solve_fp32", clock32, "dense_solve")):
@@ -671,7 +733,10 @@ def main():
         tf = TRAFFIC_FILES.get(k)
         if tf and os.path.exists(tf):
             with open(tf) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                tj = json.load(f)
+            # a traffic file measured per unit scales to this leg's units per launch
+            traffic = (tj["hbm_bytes_per_unit"] * units / nl if "hbm_bytes_per_unit" in tj
+                       else tj.get("hbm_bytes_per_launch"))
         roofs[k] = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9,
                         unit="GB/s", frac=achieved / PEAK_HBM, traffic=traffic,
                         kernel=KERNEL_SYMBOLS[key], launches=nl, mean_launch_us=1e3 * ms,
@@ -679,9 +744,11 @@ def main():
     # the roofline is the dominant kernel of the value leg: the FP32-factor
     # dense solve of the rescue route in the full runs (the preconditioner
     # apply when the full runs are not measured)
-    top = "dense_solve_full" if "dense_solve_full" in roofs else ROOF_KERNEL
-    if top in roofs:
+    order = ("dense_solve_full", ROOF_KERNEL, "dense_solve_fixed")
+    top = next((k for k in order if k in roofs), next(iter(roofs), None))
+    if top is not None:
         out["roofline"] = roofs.pop(top)
+        out["roofline"]["leg"] = top
         out["rooflines_other"] = roofs
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(z, w)

@@ -1,7 +1,7 @@
 """Summaries of rocprofv3 output for the profiles kept with the sources.
 
     python -m specmc_amd.profsum stats  RUN_kernel_stats.csv  "title" "command"
-    python -m specmc_amd.profsum traffic FETCH.csv WRITE.csv KERNEL PROBE_MIB "command"
+    python -m specmc_amd.profsum traffic FETCH.csv WRITE.csv KERNEL PROBE_MIB "command" [UNITS]
     python -m specmc_amd.profsum letters STATS.csv FETCH.csv WRITE.csv PROBE_MIB "title" \
         "command" [MFMA.csv]
 
@@ -10,7 +10,9 @@
 dispatch) and prints JSON with the mean HBM bytes per dispatch of ``KERNEL``:
 each counter is scaled by the factor that makes it read the known size of
 the device-to-device copy of PROBE_MIB MiB in the same run (its first
-``__amd_rocclr_copyBuffer`` dispatch of that size).
+``__amd_rocclr_copyBuffer`` dispatch of that size).  With UNITS (the
+algorithmic units every dispatch of the kernel processes) it also gives the
+bytes per unit.
 ``letters`` groups the kernels by the part of the time step they serve --
 (a) x-transforms, (b) z-transforms, (c) viscosity law and pseudo-spectral
 products, (d) Stokes / Krylov / preconditioner, (e) theta BDF2 pieces,
@@ -48,7 +50,7 @@ def _per_kernel(path, kernel):
     return vals, probe
 
 
-def traffic(fetch_csv, write_csv, kernel, probe_mib, command=""):
+def traffic(fetch_csv, write_csv, kernel, probe_mib, command="", units=None):
     res = dict(kernel=kernel, command=command, probe_mib=probe_mib)
     total = 0.0
     for name, path in (("fetch", fetch_csv), ("write", write_csv)):
@@ -64,6 +66,9 @@ def traffic(fetch_csv, write_csv, kernel, probe_mib, command=""):
         res[name + "_dispatches"] = len(vals)
         total += mean
     res["hbm_bytes_per_launch"] = total
+    if units:
+        res["units_per_launch"] = units
+        res["hbm_bytes_per_unit"] = total / units
     return res
 
 
@@ -186,7 +191,8 @@ def main(argv):
                          argv[7] if len(argv) > 7 else None))
     elif argv[:1] == ["traffic"]:
         print(json.dumps(traffic(argv[1], argv[2], argv[3], float(argv[4]),
-                                 argv[5] if len(argv) > 5 else ""), indent=1))
+                                 argv[5] if len(argv) > 5 else "",
+                                 float(argv[6]) if len(argv) > 6 else None), indent=1))
     else:
         raise SystemExit(__doc__)
 
