@@ -101,6 +101,12 @@ int smx_set_gj_rows(int on);
 /* smx_set_gj_reg(1) (default) inverts FP64 blocks of order 16..128 with the
  * matrix held in registers (4N threads), 0 with the kernels above. */
 int smx_set_gj_reg(int on);
+/* smx_set_gj_mfma(1) (default) inverts FP64 blocks of order 16..128 by
+ * Gauss-Jordan blocked by 16 columns: each 16-column panel is factored in one
+ * wavefront and applied to the other columns as a K = 16 product on the
+ * matrix cores (v_mfma_f64_16x16x4f64); 0 uses the kernels above.  The same
+ * pivot choice; the inverses agree to roundoff. */
+int smx_set_gj_mfma(int on);
 int smx_set_inv_pairs(int on);
 
 /* Autotuner table (no counterpart in the reference; the choices are measured

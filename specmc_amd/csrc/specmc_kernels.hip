@@ -249,7 +249,16 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   }
 }
 
-// FP64 inversion in registers (default for 16 <= N <= 128)
+// FP64 inversion blocked by 16 columns on the matrix cores (default for
+// 16 <= N <= 128; gj_mfma.hip)
+int gj_invert_mfma(int nmat, int N, double *A, int per_run, const int *active, hipStream_t st);
+static int g_gj_mfma = 1;
+extern "C" int smx_set_gj_mfma(int on) {
+  g_gj_mfma = on ? 1 : 0;
+  return 0;
+}
+
+// FP64 inversion in registers (for 16 <= N <= 128 when the matrix-core form is off)
 static int g_gj_reg = 1;
 extern "C" int smx_set_gj_reg(int on) {
   g_gj_reg = on ? 1 : 0;
@@ -1323,6 +1332,12 @@ extern "C" int smx_invert(int prec, int nmat, int N, void *A, int per_run, const
                           void *stream) {
   if (nmat <= 0) return 0;
   if (N <= 0 || per_run <= 0) return fail_msg("smx_invert: bad sizes");
+  if (prec == 0 && N >= 16 && N <= 128 && g_gj_mfma) {
+    int rc = gj_invert_mfma(nmat, N, (double *)A, per_run, active, STREAM(stream));
+    if (rc) return rc;
+    CHECK_LAUNCH("smx_invert");
+    return 0;
+  }
   if (prec == 0 && N >= 16 && N <= GJR_MAXN && g_gj_reg) {
     // register-resident kernel: 4N threads, ceil(N / 4) columns per thread
     const int cpt = (N + 3) / 4;

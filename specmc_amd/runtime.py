@@ -34,6 +34,7 @@ SIGNATURES = {
     "smx_set_xprod_mfma": [I],
     "smx_set_gj_rows": [I],
     "smx_set_gj_reg": [I],
+    "smx_set_gj_mfma": [I],
     "smx_set_inv_pairs": [I],
     "smx_dense_lu": [I, I, I, P, L, P, P, P],
     "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P, P],

@@ -1,7 +1,8 @@
 """Kernel variants on the device (MI355X) against each other and float64 host
 arithmetic: the matrix-core pseudo-spectral product against the vector
 kernel and a numpy restatement; the row-owner Gauss-Jordan inversion against
-numpy.linalg.inv and the first kernel.
+numpy.linalg.inv and the first kernel; the matrix-core Gauss-Jordan
+inversion against both.
 
 Tolerance: 1e-13 relative (the variants reorder floating-point sums only).
 """
@@ -63,19 +64,24 @@ def test_xprod_mfma_matches_vector_kernel_and_numpy(lib, shape):
     assert e_mfma < 1e-13 and e_vec < 1e-13
 
 
-@pytest.mark.parametrize("N", [16, 24, 48, 96, 128])
+@pytest.mark.parametrize("N", [16, 24, 40, 48, 96, 128])
 def test_gj_inverse_variants(lib, N):
     """The register-resident, row-owner and first Gauss-Jordan kernels give
-    the same inverse (the same arithmetic per entry) and match numpy."""
+    the same inverse (the same arithmetic per entry) and match numpy; the
+    matrix-core kernel (16-column panels, K = 16 products) matches them to
+    roundoff."""
     rng = np.random.default_rng(N)
     nm = 7
     M = rng.normal(size=(nm, N, N)) + np.sqrt(N) * np.eye(N)
     M[0] = rng.normal(size=(N, N))           # pivoting on every column
+    M[1] = M[1][rng.permutation(N)]          # a row permutation of a dominant matrix
     dev = torch.device("cuda")
     s = torch.cuda.current_stream().cuda_stream
     got = {}
     try:
-        for name, reg, rows in (("reg", 1, 0), ("rows", 0, 1), ("first", 0, 0)):
+        for name, mf, reg, rows in (("mfma", 1, 1, 0), ("reg", 0, 1, 0), ("rows", 0, 0, 1),
+                                    ("first", 0, 0, 0)):
+            lib.set_gj_mfma(mf)
             lib.set_gj_reg(reg)
             lib.set_gj_rows(rows)
             A = torch.tensor(np.ascontiguousarray(M.transpose(0, 2, 1)), device=dev)
@@ -83,16 +89,42 @@ def test_gj_inverse_variants(lib, N):
             torch.cuda.synchronize()
             got[name] = A.cpu().numpy().transpose(0, 2, 1)
     finally:
+        lib.set_gj_mfma(1)
         lib.set_gj_reg(1)
         lib.set_gj_rows(0)
     want = np.linalg.inv(M)
-    print("gj N=%d: reg %.2e rows %.2e first %.2e; reg vs first %.2e" % (
-        N, rel(got["reg"], want), rel(got["rows"], want), rel(got["first"], want),
-        rel(got["reg"], got["first"])), flush=True)
+    print("gj N=%d: mfma %.2e reg %.2e rows %.2e first %.2e; reg vs first %.2e, "
+          "mfma vs reg %.2e" % (
+              N, rel(got["mfma"], want), rel(got["reg"], want), rel(got["rows"], want),
+              rel(got["first"], want), rel(got["reg"], got["first"]),
+              rel(got["mfma"], got["reg"])), flush=True)
     for k in got:
         assert rel(got[k], want) < 1e-11, k
     assert rel(got["reg"], got["first"]) < 1e-14
     assert rel(got["rows"], got["first"]) < 1e-14
+    assert rel(got["mfma"], got["reg"]) < 1e-12
+
+
+@pytest.mark.parametrize("N", [24, 96])
+def test_gj_mfma_active_mask(lib, N):
+    """The matrix-core inversion leaves the blocks of inactive runs untouched
+    and inverts the others (per_run blocks per run)."""
+    rng = np.random.default_rng(100 + N)
+    per_run, nrun = 3, 4
+    M = rng.normal(size=(per_run * nrun, N, N)) + np.sqrt(N) * np.eye(N)
+    act = np.array([1, 0, 1, 0], dtype=np.int32)
+    dev = torch.device("cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    A = torch.tensor(np.ascontiguousarray(M.transpose(0, 2, 1)), device=dev)
+    da = torch.tensor(act, device=dev)
+    lib.invert(0, per_run * nrun, N, A.data_ptr(), per_run, da.data_ptr(), s)
+    torch.cuda.synchronize()
+    got = A.cpu().numpy().transpose(0, 2, 1)
+    for q in range(per_run * nrun):
+        if act[q // per_run]:
+            assert rel(got[q], np.linalg.inv(M[q])) < 1e-11, q
+        else:
+            assert np.array_equal(got[q], M[q]), q
 
 
 @pytest.mark.parametrize("N", [24, 48, 96, 192])
