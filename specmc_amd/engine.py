@@ -23,6 +23,7 @@ Layouts (row-major, run-major)
     LU factors           [B][Nk][Nz*Nz] column-major, pivots [B][Nk][Nz]
 """
 import json
+import contextlib
 import threading
 
 import numpy as np
@@ -131,12 +132,49 @@ def inverse_fits_lds(N, esize):
 # generator state when a capture begins, which it refuses while another
 # thread's capture is open.  Launches and replays from other threads go on.
 _CAPTURE_LOCK = threading.Lock()
-# the transient dense matrices of a rescue or of a scattered lagged-factor
-# refresh (at most RESCUE_BYTES, plus one FP64 assembly matrix) exist for one
-# engine at a time in the process: engines stepping concurrently (the
-# server's bucket workers) take turns, so the server's one RESCUE_BYTES of
-# headroom covers them.  Re-entrant: a rescue may run inside a refresh.
-_RESCUE_GATE = threading.RLock()
+class _ByteGate:
+    """Admission of the transient dense matrices of rescues and lagged-factor
+    refreshes across the engines of a process (the server's bucket workers):
+    a thread asks for the bytes it will hold and waits while the bytes other
+    threads hold plus its own would exceed ``budget``, so the server's one
+    RESCUE_BYTES of headroom covers them all, and small grids' rescues are not
+    queued behind large ones.  A request larger than the budget goes ahead
+    alone.  A thread's nested requests (a rescue inside a refresh) never wait,
+    so a waiting thread holds nothing and no two threads wait on each other."""
+
+    def __init__(self, budget):
+        self.budget = int(budget)
+        self.used = 0
+        self.cv = threading.Condition()
+        self.held = {}       # thread ident -> [depth, bytes]
+
+    @contextlib.contextmanager
+    def hold(self, nbytes):
+        nbytes = max(0, int(nbytes))
+        me = threading.get_ident()
+        with self.cv:
+            h = self.held.get(me)
+            if h is None:
+                while self.used > 0 and self.used + nbytes > self.budget:
+                    self.cv.wait()
+                h = self.held[me] = [0, 0]
+            h[0] += 1
+            h[1] += nbytes
+            self.used += nbytes
+        try:
+            yield
+        finally:
+            with self.cv:
+                h = self.held[me]
+                h[0] -= 1
+                h[1] -= nbytes
+                self.used -= nbytes
+                if h[0] == 0:
+                    del self.held[me]
+                self.cv.notify_all()
+
+
+_RESCUE_GATE = _ByteGate(RESCUE_BYTES)
 _TRANSIENT_STREAMS = {}
 
 
@@ -1257,7 +1295,10 @@ class Engine:
             for c0 in range(0, len(bad), cap):
                 self._rescue(theta, bad[c0:c0 + cap], rprec)
             return
-        with _RESCUE_GATE:
+        # the rescue's transient matrices: nb factors, and the FP64 assembly
+        # matrix of an FP32 rescue
+        es = 4 if rprec else 8
+        with _RESCUE_GATE.hold(len(bad) * n * n * es + (n * n * 8 if rprec else 0)):
             redo = self._rescue_dense(theta, bad, rprec)
         if redo:
             # runs the FP32 factor did not bring to the acceptance test, with
@@ -1500,7 +1541,14 @@ class Engine:
         runs = np.where(sel)[0]
         if runs.size == 0:
             return
-        with _RESCUE_GATE:
+        # transient matrices of a scattered refresh (at most RESCUE_BYTES) and
+        # the FP64 assembly matrix of an FP32 store; a refresh of consecutive
+        # slots factors in the store itself
+        n = self.n
+        es = 4 if self.lag_prec else 8
+        scattered = runs[-1] - runs[0] + 1 != runs.size
+        nb = min(runs.size, max(1, int(RESCUE_BYTES // (n * n * es)))) if scattered else 0
+        with _RESCUE_GATE.hold(nb * n * n * es + (n * n * 8 if self.lag_prec else 0)):
             self._lag_refresh_runs(sel, runs)
 
     def _lag_refresh_runs(self, sel, runs):

@@ -85,24 +85,60 @@ def test_lag_slots_cap_only_the_largest_coupled_grids():
             assert lag_slots(Nx, Nz, prec) >= 64, (Nx, Nz, prec)
 
 
-def test_rescue_gate_serialises_transient_rescue_memory():
-    """The transient dense matrices of rescues and scattered refreshes exist
-    for one engine at a time: the gate is re-entrant in one thread (a rescue
-    inside a refresh) and excludes a second thread (another bucket worker),
-    so the server's single RESCUE_BYTES of headroom covers them."""
+def test_rescue_gate_admits_transient_bytes_up_to_its_budget():
+    """The transient dense matrices of rescues and refreshes are admitted by
+    bytes: concurrent holders share the budget (small requests go ahead
+    beside a large one), a request that would pass it waits for a release, a
+    request above the whole budget goes ahead alone, and a thread's nested
+    request (a rescue inside a refresh) never waits."""
     import threading
+    import time
     from specmc_amd import engine
-    gate = engine._RESCUE_GATE
-    with gate:
-        with gate:
-            pass
-        got = []
-        t = threading.Thread(target=lambda: got.append(gate.acquire(timeout=0.2)))
-        t.start()
-        t.join()
-        assert got == [False]
+    gate = engine._ByteGate(100)
+
+    def take(nbytes, out, hold_s=0.0):
+        with gate.hold(nbytes):
+            out.append((nbytes, time.perf_counter()))
+            time.sleep(hold_s)
+
     got = []
-    t = threading.Thread(target=lambda: (got.append(gate.acquire(timeout=0.2)), gate.release()))
+    with gate.hold(60):
+        t = threading.Thread(target=take, args=(30, got))     # fits beside 60
+        t.start()
+        t.join(timeout=5)
+        assert not t.is_alive() and got and got[0][0] == 30
+        got.clear()
+        t = threading.Thread(target=take, args=(50, got))     # 60 + 50 > 100
+        t.start()
+        time.sleep(0.2)
+        assert got == [] and t.is_alive()
+        with gate.hold(500):                                 # nested: never waits
+            assert gate.used == 560
+        t0 = time.perf_counter()
+    t.join(timeout=5)
+    assert not t.is_alive() and got and got[0][1] >= t0
+    assert gate.used == 0 and gate.held == {}
+    got.clear()
+    t = threading.Thread(target=take, args=(1000, got))       # above the budget, alone
     t.start()
-    t.join()
-    assert got == [True]
+    t.join(timeout=5)
+    assert not t.is_alive() and got[0][0] == 1000 and gate.used == 0
+    # many threads of mixed sizes finish, and the budget held at once stays within
+    peak = [0]
+    lock = threading.Lock()
+
+    def work(nbytes):
+        with gate.hold(nbytes):
+            with lock:
+                peak[0] = max(peak[0], gate.used)
+            time.sleep(0.01)
+
+    ts = [threading.Thread(target=work, args=(b,)) for b in (10, 90, 40, 70, 20, 60, 30, 80)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=10)
+    assert all(not t.is_alive() for t in ts)
+    assert gate.used == 0 and peak[0] <= 100
+    assert isinstance(engine._RESCUE_GATE, engine._ByteGate)
+    assert engine._RESCUE_GATE.budget == engine.RESCUE_BYTES
