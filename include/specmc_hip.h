@@ -289,10 +289,17 @@ int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
  *               (Mc [3Q][Nz] = [(D2 Tp)^T; Tp^T; (D Tp)^T]), and the tau rows
  *               out[Nz-2] = V0 . v[b][k], out[Nz-1] = V1 . v[b][k].
  * v and out are [B][Nk][Nz]; eta of run b at eta + b*eta_bs, [nrow][Q];
- * So/Ce [nrow][Nk], Ao/Ae [Nk][nrow] as smx_xprod; kx [Nk].  The workspace
- * holds yz and tz: at least smx_schur_apply_work(B, Nk, Q) elements of the
- * precision's type. */
+ * So/Ce [nrow][Nk], Ao/Ae [Nk][nrow] as smx_xprod; kx [Nk].  With a mask,
+ * the z-synthesis runs over the active runs only, listed on the device.  The
+ * workspace holds yz, tz and that list: at least smx_schur_apply_work(B, Nk,
+ * Q) elements of the precision's type, (8 Nk Q + 2) per run. */
 long long smx_schur_apply_work(int B, int Nk, int Q);
+/* smx_set_schur_compact(1) (default): with a mask, the z-synthesis of
+ * smx_schur_apply runs over the active runs listed on the device (a 64-run
+ * tile is no longer computed whole for one active run in it); 0 runs it over
+ * every run and skips only tiles without an active run.  No counterpart in
+ * the reference (a kernel variant; the same outputs bit for bit). */
+int smx_set_schur_compact(int on);
 int smx_schur_apply(int prec, int B, int Nk, int N, int nrow, int Q, const void *v,
                     const void *eta, long eta_bs, void *out, const void *Wz,
                     const void *Mc, const void *kx, const void *So, const void *Ce,

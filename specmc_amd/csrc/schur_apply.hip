@@ -1,7 +1,8 @@
 // schur_apply.hip -- the coupled Schur operator apply of the BFBT route
 // (FreeSchurOperator.matvec, lagged_stokes.py:202-220) in three launches:
 //
-//   1. z-synthesis   yz[b][k] = W_k v[b][k]           per-mode GEMM, K = Nz
+//   1. z-synthesis   yz[b][k] = W_k v[b][k]           per-mode GEMM, K = Nz, over
+//                                                    the active runs (listed)
 //      W_k = [Vp; 2k^2 Vp G_k; Vp D2 G_k; k Vp D G_k] (4Q x Nz): the Green
 //      apply G_k and the synthesis of v, w = G_k v, D2 w, D w on the Q padded
 //      Chebyshev nodes, with the x-convolution factors 2k^2 and k folded in
@@ -119,8 +120,37 @@ __global__ void __launch_bounds__(256) k_schur_combine(int Nk, int N, int Q, int
   }
 }
 
+// the active runs as a list (ascending) and its length, one wavefront
+__global__ void __launch_bounds__(64) k_active_list(int B, const int *active, int *list,
+                                                    int *cnt) {
+  const int lane = threadIdx.x;
+  int base = 0;
+  for (int b0 = 0; b0 < B; b0 += 64) {
+    const int b = b0 + lane;
+    const bool on = b < B && active[b] != 0;
+    const unsigned long long m = __ballot(on);
+    if (on) list[base + __popcll(m & ((1ull << lane) - 1ull))] = b;
+    base += __popcll(m);
+  }
+  if (lane == 0) *cnt = base;
+}
+
+int gemm_rows(int role, int prec, int M, int N, int K, double alpha, double beta,
+              const void *A, long ars, long acs, long sa1, const void *B, long brs, long bcs,
+              long sb1, void *C, long crs, long ccs, long sc1, int nz1, const int *rowlist,
+              const int *rowcnt, hipStream_t st);
+// z-synthesis over the listed active runs: 1 (default), or over all runs: 0
+static int g_schur_compact = 1;
+extern "C" int smx_set_schur_compact(int on) {
+  g_schur_compact = on ? 1 : 0;
+  return 0;
+}
+
+// yz and tz, then the active-run list and its length (B + 1 ints), in
+// elements of at least 4 bytes; proportional to B, so that the capacity of a
+// fixed workspace in runs is its size over the work of one run
 extern "C" long long smx_schur_apply_work(int B, int Nk, int Q) {
-  return 2LL * B * Nk * 4 * Q;
+  return (2LL * Nk * 4 * Q + 2) * B;
 }
 
 static int schur_apply_run(int prec, int B, int Nk, int N, int nrow, int Q, const void *v,
@@ -133,10 +163,21 @@ static int schur_apply_run(int prec, int B, int Nk, int N, int nrow, int Q, cons
   char *yz = (char *)work;
   char *tz = yz + (size_t)B * Nk * F * es;
   // 1. yz[b][k][:] = W_k v[b][k]: M = B runs, N = 4Q, K = Nz, batch over k;
-  //    A(b, n) = v[b][k][n], B(n, c) = Wz[k][n][c], C(b, c) = yz[b][k][c]
-  int rc = smx_gemm_ex(2, prec, B, (int)F, N, 1.0, 0.0, v, (long)Nk * N, 1, N, 0, Wz, F, 1,
-                       (long)N * F, 0, yz, (long)Nk * F, 1, F, 0, Nk, 1, nullptr, 1, active,
-                       active ? 1 : 0, 1, stream);
+  //    A(b, n) = v[b][k][n], B(n, c) = Wz[k][n][c], C(b, c) = yz[b][k][c].
+  //    With a mask, the rows are the active runs, compacted: a tile of 64 runs
+  //    would otherwise be computed whole for one active run in it
+  int rc;
+  if (active && g_schur_compact) {
+    int *list = (int *)(tz + (size_t)B * Nk * F * es), *cnt = list + B;
+    hipLaunchKernelGGL(k_active_list, dim3(1), dim3(64), 0, STREAM(stream), B, active, list,
+                       cnt);
+    rc = gemm_rows(2, prec, B, (int)F, N, 1.0, 0.0, v, (long)Nk * N, 1, N, Wz, F, 1,
+                   (long)N * F, yz, (long)Nk * F, 1, F, Nk, list, cnt, STREAM(stream));
+  } else {
+    rc = smx_gemm_ex(2, prec, B, (int)F, N, 1.0, 0.0, v, (long)Nk * N, 1, N, 0, Wz, F, 1,
+                     (long)N * F, 0, yz, (long)Nk * F, 1, F, 0, Nk, 1, nullptr, 1, active,
+                     active ? 1 : 0, 1, stream);
+  }
   if (rc) return rc;
   // 2. the four x-products with eta on the padded grid (factors folded in W)
   rc = smx_xprod(prec, B, 4, 3, Nk, nrow, Q, yz, nullptr, eta, eta_bs, tz, So, Ce, Ao, Ae,

@@ -23,6 +23,10 @@ struct GemmArgs {
   int nz2;
   const void *rscale; int rperiod;
   const int *active; int mask_mode, rows_per_run;
+  // compacted rows: when rowlist is set, row i < *rowcnt of the product is
+  // row rowlist[i] of A and C (the active runs, listed on the device), and
+  // the mask is not consulted
+  const int *rowlist, *rowcnt;
 };
 
 // R: role of the product, part of the kernel symbol only (so kernel traces
@@ -38,7 +42,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   int ti, tj;
   xcd_tile(ti, tj);
   const int i0 = ti * TM, j0 = tj * TN;
-  if (g.active) {
+  const int Mv = g.rowlist ? min(g.M, *g.rowcnt) : g.M;
+  if (i0 >= Mv) return;
+  if (g.active && !g.rowlist) {
     if (g.mask_mode == 2 && !g.active[z1]) return;
     if (g.mask_mode == 3 && !g.active[z2]) return;
     if (g.mask_mode == 1) {
@@ -70,8 +76,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
         int e = u * 256 + tid, ii, kk;
         if (a_kfast) { ii = e >> 4; kk = e & 15; } else { kk = e / TM; ii = e % TM; }
         const int i = i0 + ii, k = k0 + kk;
-        const bool ok = (i < g.M && k < g.K);
-        ra[u] = A[ok ? (long)i * g.ars + (long)k * g.acs : 0];
+        const bool ok = (i < Mv && k < g.K);
+        const long ir = ok && g.rowlist ? g.rowlist[i] : i;
+        ra[u] = A[ok ? ir * g.ars + (long)k * g.acs : 0];
         ra[u] = ok ? ra[u] : T(0);
       }
 #pragma unroll
@@ -116,8 +123,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 #pragma unroll
   for (int p = 0; p < PM; ++p) {
     int i = i0 + ty + 16 * p;
-    if (i >= g.M) continue;
-    if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
+    if (i >= Mv) continue;
+    if (g.rowlist) i = g.rowlist[i];
+    else if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
     T al = rs ? alpha * rs[i % g.rperiod] : alpha;
 #pragma unroll
     for (int q = 0; q < PN; ++q) {
@@ -148,7 +156,9 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   int ti, tj;
   xcd_tile(ti, tj);
   const int i0 = ti * TM, j0 = tj * TN;
-  if (g.active) {
+  const int Mv = g.rowlist ? min(g.M, *g.rowcnt) : g.M;
+  if (i0 >= Mv) return;
+  if (g.active && !g.rowlist) {
     if (g.mask_mode == 2 && !g.active[z1]) return;
     if (g.mask_mode == 3 && !g.active[z2]) return;
     if (g.mask_mode == 1) {
@@ -175,14 +185,20 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   // flight while the matrix cores work on chunk k0 from LDS
   constexpr int NA = KC * TM / 256, NB = KC * TN / 256;
   T ra[NA], rb[NB];
+  // the A row each thread loads is the same in every K chunk: its offset
+  // (through the row list, when there is one) is found once; -1 past the rows
+  long arow[NA];
+#pragma unroll
+  for (int u = 0; u < NA; ++u) {
+    const int e = u * 256 + tid, i = i0 + (a_kfast ? e / KC : e % TM);
+    arow[u] = i < Mv ? (long)(g.rowlist ? g.rowlist[i] : i) * g.ars : -1;
+  }
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
-      int e = u * 256 + tid, ii, kk;
-      if (a_kfast) { ii = e / KC; kk = e % KC; } else { kk = e / TM; ii = e % TM; }
-      const int i = i0 + ii, k = k0 + kk;
-      const bool ok = (i < g.M && k < g.K);
-      ra[u] = A[ok ? (long)i * g.ars + (long)k * g.acs : 0];
+      const int e = u * 256 + tid, k = k0 + (a_kfast ? e % KC : e / TM);
+      const bool ok = (arow[u] >= 0 && k < g.K);
+      ra[u] = A[ok ? arow[u] + (long)k * g.acs : 0];
       ra[u] = ok ? ra[u] : T(0);
     }
 #pragma unroll
@@ -232,9 +248,10 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(GemmArgs g) {
   for (int p = 0; p < BM; ++p) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = i0 + wr + 16 * p + MfmaAcc<T>::row(lk, r);
-      if (i >= g.M) continue;
-      if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
+      int i = i0 + wr + 16 * p + MfmaAcc<T>::row(lk, r);
+      if (i >= Mv) continue;
+      if (g.rowlist) i = g.rowlist[i];
+      else if (g.active && g.mask_mode == 1 && !g.active[i / g.rows_per_run]) continue;
       const T al = rs ? alpha * rs[i % g.rperiod] : alpha;
 #pragma unroll
       for (int q = 0; q < BN; ++q) {
@@ -549,6 +566,22 @@ extern "C" int smx_gemm_ex(int role, int prec, int M, int N, int K, double alpha
              rows_per_run};
   if (prec == 0) launch_gemm_role<double>(role, g, nz1 * nz2, STREAM(stream));
   else launch_gemm_role<float>(role, g, nz1 * nz2, STREAM(stream));
+  CHECK_LAUNCH("smx_gemm");
+  return 0;
+}
+
+// smx_gemm_ex over a compacted list of rows (rowlist / rowcnt on the device,
+// at most M rows); used by the Schur apply for the active runs of a batch
+int gemm_rows(int role, int prec, int M, int N, int K, double alpha, double beta,
+              const void *A, long ars, long acs, long sa1, const void *B, long brs, long bcs,
+              long sb1, void *C, long crs, long ccs, long sc1, int nz1, const int *rowlist,
+              const int *rowcnt, hipStream_t st) {
+  if (M <= 0 || N <= 0 || nz1 <= 0) return 0;
+  if (nz1 > 65535) return fail_msg("smx_gemm: batch > 65535");
+  GemmArgs g{M, N, K, alpha, beta, A, ars, acs, sa1, 0, B, brs, bcs, sb1, 0,
+             C, crs, ccs, sc1, 0, 1, nullptr, 1, nullptr, 0, 1, rowlist, rowcnt};
+  if (prec == 0) launch_gemm_role<double>(role, g, nz1, st);
+  else launch_gemm_role<float>(role, g, nz1, st);
   CHECK_LAUNCH("smx_gemm");
   return 0;
 }

@@ -54,6 +54,7 @@ SIGNATURES = {
     "smx_eta_guard_profile": [I, I, I, I, P, P, P, I, P, P, P],
     "smx_xprod": [I, I, I, I, I, I, I, P, P, P, L, P, P, P, P, P, P, P],
     "smx_schur_apply_work": [I, I, I],
+    "smx_set_schur_compact": [I],
     "smx_schur_apply": [I, I, I, I, I, I, P, P, L, P, P, P, P, P, P, P, P, P, P, P, LL, P, P],
     "smx_schur_apply_static_bytes": [],
     "smx_schur_apply_static": [I, I, I, I, I, I, P, P, L, P, P, P, P, P, P, P, P, P, P, P, P],

@@ -69,6 +69,9 @@ def test_static_workspace_rejects_oversized_batches():
     cap = int(L.schur_apply_static_bytes())
     assert cap == 64 << 20
     nb = cap // (8 * int(L.schur_apply_work(1, e.Nk, e.Q))) + 1
+    # the call below must be rejected before any launch (its buffers hold
+    # one run): check on the host that this batch exceeds the workspace
+    assert 8 * int(L.schur_apply_work(nb, e.Nk, e.Q)) > cap
     v = torch.zeros(1, e.n, dtype=torch.float64, device=e.dev)
     out = torch.zeros_like(v)
     # rejected on the host before any launch, so the small buffers are never read
