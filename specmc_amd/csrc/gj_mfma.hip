@@ -4,8 +4,8 @@
 // 16..128: the per-mode Helmholtz and BFBT blocks whose inverses the FGMRES
 // preconditioners apply (inv(A) r instead of two triangular sweeps).
 //
-// Column-major N x N matrices, one workgroup of NT wavefronts per matrix,
-// padded to NP = 16 NT with an identity block.  Wavefront w holds column
+// Column-major N x N matrices, NT wavefronts per matrix (two matrices per
+// workgroup, below), padded to NP = 16 NT with an identity block.  Wavefront w holds column
 // block w (NP x 16) in the accumulator layout of v_mfma_f64_16x16x4f64:
 // element i of tile t of lane l is row 16 t + l / 16 + 4 i, column
 // 16 w + l % 16 (the FP64 layout measured on gfx950).
@@ -52,22 +52,37 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long b)
   return ((unsigned long long)hi << 32) | lo;
 }
 
+// Two matrices per workgroup (2 NT wavefronts, NT per matrix): wavefronts k
+// and NT + k hold the panels of step k, and a workgroup's wavefronts go to
+// the four SIMDs in turn, so for NT = 6 the two panels, the serial part of
+// every step, run on different SIMDs (two one-matrix workgroups on a CU
+// often share one: 2.13 ms for 8448 blocks of 96 against 3.16 ms with one
+// such workgroup per CU, not half).  Each matrix has its own LDS; a half
+// without a matrix (the last workgroup of an odd count, or an inactive run)
+// skips the work but takes every barrier.
+#define GJM_MPW 2
 template <int NT>
-__global__ void __launch_bounds__(64 * NT) k_gj_invert_mfma(double *Am, int N, int per_run,
-                                                            const int *active) {
+__global__ void __launch_bounds__(64 * NT * GJM_MPW) k_gj_invert_mfma(double *Am, int nmat,
+                                                                      int N, int per_run,
+                                                                      const int *active) {
   constexpr int NP = 16 * NT, RPL = (NP + 63) / 64, PS = 17;
   static_assert(RPL <= 2, "panel rows: at most two per lane");
+  constexpr int LDS_D = 2 * NP * PS + NT * 256;        // doubles of one matrix's region
+  constexpr int LDS_I = 2 * NP + (3 * NP + 3) / 4;     // ints after them
   extern __shared__ double gjm_smem[];
-  double *pn = gjm_smem;                               // [2][NP * PS] panels, row-major
+  const int tid = threadIdx.x, lane = tid & 63, wg = tid >> 6;
+  const int half = wg / NT, w = wg - half * NT;        // matrix slot, wavefront in it
+  double *base = gjm_smem + half * (LDS_D + (LDS_I + 1) / 2);
+  double *pn = base;                                   // [2][NP * PS] panels, row-major
   double *xs = pn + 2 * NP * PS;                       // [NT][16 * 16] pivot rows
   int *pv = (int *)(xs + NT * 256);                    // [NP] pivot row of each column
   int *pinv = pv + NP;                                 // [NP] column of each pivot row
   signed char *pos = (signed char *)(pinv + NP);       // [2][NP] place among the step's pivots
   unsigned char *used = (unsigned char *)(pos + 2 * NP);   // [NP] row already pivotal
-  const int m = blockIdx.x;
-  if (active && !active[m / per_run]) return;
-  double *Gm = Am + (long)m * N * N;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m = blockIdx.x * GJM_MPW + half;
+  const bool live = m < nmat && !(active && !active[m / per_run]);
+  double *Gm = Am + (long)(live ? m : 0) * N * N;
+  const int ltid = tid - half * 64 * NT;               // thread index within the matrix
   const int lr = lane & 15, lg = lane >> 4;
   const int Q = 16 * w + lr;
   v4f64 acc[NT];
@@ -76,14 +91,14 @@ __global__ void __launch_bounds__(64 * NT) k_gj_invert_mfma(double *Am, int N, i
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int R = 16 * t + lg + 4 * i;
-      acc[t][i] = (R < N && Q < N) ? Gm[(long)Q * N + R] : (R == Q ? 1.0 : 0.0);
+      acc[t][i] = (live && R < N && Q < N) ? Gm[(long)Q * N + R] : (R == Q ? 1.0 : 0.0);
     }
-  for (int e = tid; e < NP; e += 64 * NT) used[e] = 0;
+  for (int e = ltid; e < NP; e += 64 * NT) used[e] = 0;
   __syncthreads();
   for (int k = 0; k < NT; ++k) {
     double *P = pn + (k & 1) * NP * PS;
     signed char *ps = pos + (k & 1) * NP;
-    if (w == k) {
+    if (live && w == k) {
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -170,7 +185,7 @@ __global__ void __launch_bounds__(64 * NT) k_gj_invert_mfma(double *Am, int N, i
         for (int i = 0; i < 4; ++i) acc[t][i] = P[(16 * t + lg + 4 * i) * PS + lr];
     }
     __syncthreads();
-    if (w != k) {
+    if (live && w != k) {
       // this block's pivot rows to X (zeroed in place), then + P X
       double *X = xs + w * 256;
 #pragma unroll
@@ -196,9 +211,10 @@ __global__ void __launch_bounds__(64 * NT) k_gj_invert_mfma(double *Am, int N, i
     }
   }
   __syncthreads();
-  for (int e = tid; e < NP; e += 64 * NT) pinv[pv[e]] = e;
+  if (live)
+    for (int e = ltid; e < NP; e += 64 * NT) pinv[pv[e]] = e;
   __syncthreads();
-  if (Q < N) {
+  if (live && Q < N) {
     const long oq = (long)pv[Q] * N;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -213,7 +229,8 @@ __global__ void __launch_bounds__(64 * NT) k_gj_invert_mfma(double *Am, int N, i
 template <int NT>
 static size_t gjm_lds() {
   constexpr int NP = 16 * NT;
-  return (size_t)(2 * NP * 17 + NT * 256) * 8 + 2 * NP * 4 + 2 * NP + NP;
+  constexpr int LDS_D = 2 * NP * 17 + NT * 256, LDS_I = 2 * NP + (3 * NP + 3) / 4;
+  return (size_t)GJM_MPW * (LDS_D + (LDS_I + 1) / 2) * 8;
 }
 
 template <int NT>
@@ -221,8 +238,8 @@ static int gjm_launch(int nmat, int N, double *A, int per_run, const int *active
                       hipStream_t st) {
   const size_t sh = gjm_lds<NT>();
   if (!lds_optin(k_gj_invert_mfma<NT>, sh)) return fail_msg("smx_invert: LDS opt-in failed");
-  hipLaunchKernelGGL(k_gj_invert_mfma<NT>, dim3(nmat), dim3(64 * NT), sh, st, A, N, per_run,
-                     active);
+  hipLaunchKernelGGL(k_gj_invert_mfma<NT>, dim3(cdiv(nmat, GJM_MPW)), dim3(64 * NT * GJM_MPW),
+                     sh, st, A, nmat, N, per_run, active);
   return 0;
 }
 
