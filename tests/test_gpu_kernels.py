@@ -159,3 +159,41 @@ def test_inv_apply_pairs(lib, N):
     assert rel(got[0], want) < 1e-13 and rel(got[1], want) < 1e-13
     # the prefetching row-pair kernel accumulates in the same order
     assert np.array_equal(got[0], got[2])
+
+
+@pytest.mark.parametrize("shape,nact", [((32, 48), 5), ((64, 96), 70)])
+def test_schur_apply_over_listed_active_runs(lib, shape, nact):
+    """The Schur apply with a run mask runs its z-synthesis over the active
+    runs listed on the device: the active runs' outputs equal those of the
+    unmasked apply and of the apply without the list, bit for bit, and the
+    other runs' outputs are left as they were."""
+    from specmc_amd.engine import Engine
+    Nx, Nz = shape
+    B = 96 if Nx == 64 else 12
+    e = Engine("coupled", Nx, Nz, 2.0, [dict(Ra=1e4 * (1 + 0.01 * i), deta_T=1e3,
+                                              precond="bfbt") for i in range(B)])
+    rng = np.random.default_rng(B + nact)
+    e.set_theta(rng.normal(size=(B, e.Nk, e.Nz)) * 1e-2 * np.exp(-np.arange(e.Nz) / 8.0))
+    e._eta_of(e.theta, None)
+    v = torch.tensor(rng.normal(size=(B, e.n)), dtype=torch.float64, device=e.dev)
+    act = np.zeros(B, dtype=np.int32)
+    act[rng.choice(B, min(nact, B), replace=False)] = 1
+    mask = torch.tensor(act, device=e.dev)
+    ref = torch.empty_like(v)
+    e.schur_matvec(v, ref, None)
+    outs = []
+    try:
+        for compact in (1, 0):
+            lib.set_schur_compact(compact)
+            out = torch.full_like(v, np.nan)
+            e.schur_matvec(v, out, mask)
+            torch.cuda.synchronize()
+            outs.append(out.cpu().numpy())
+    finally:
+        lib.set_schur_compact(1)
+    on = act.astype(bool)
+    r = ref.cpu().numpy()
+    assert np.isfinite(r[on]).all()
+    assert np.array_equal(outs[0][on], r[on])
+    assert np.array_equal(outs[1][on], r[on])
+    assert np.isnan(outs[0][~on]).all() and np.isnan(outs[1][~on]).all()
