@@ -351,6 +351,8 @@ __global__ void k_lud_zero(int *v, int n) {
 // factorisation (the update is HBM-bound at K = 64).
 template <typename T, int R, int PB, int NB, bool FUSE = false>
 static int lud_run(int nmat, int n, T *A, long ms, int *piv, int *info, hipStream_t s) {
+  // the fused sub-panel kernel gathers the moved rows of at most 128 block columns
+  static_assert(!FUSE || NB <= 128, "fused sub-panels take blocks of at most 128 columns");
   const long lda = n;
   const int prec = sizeof(T) == 8 ? 0 : 1;
   for (int j0 = 0; j0 < n; j0 += NB) {
