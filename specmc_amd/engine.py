@@ -23,6 +23,7 @@ Layouts (row-major, run-major)
     LU factors           [B][Nk][Nz*Nz] column-major, pivots [B][Nk][Nz]
 """
 import json
+import collections
 import contextlib
 import threading
 
@@ -139,14 +140,19 @@ class _ByteGate:
     threads hold plus its own would exceed ``budget``, so the server's one
     RESCUE_BYTES of headroom covers them all, and small grids' rescues are not
     queued behind large ones.  A request larger than the budget goes ahead
-    alone.  A thread's nested requests (a rescue inside a refresh) never wait,
-    so a waiting thread holds nothing and no two threads wait on each other."""
+    alone.  Waiting requests are admitted in arrival order (a later one does
+    not pass an earlier one that still waits), so a large request is not
+    starved by a stream of small ones.  A thread's nested requests (a rescue
+    inside a refresh) never wait, so a waiting thread holds nothing and no two
+    threads wait on each other."""
 
     def __init__(self, budget):
         self.budget = int(budget)
         self.used = 0
         self.cv = threading.Condition()
         self.held = {}       # thread ident -> [depth, bytes]
+        self.queue = collections.deque()   # tickets of the waiting requests
+        self.tickets = 0
 
     @contextlib.contextmanager
     def hold(self, nbytes):
@@ -155,8 +161,14 @@ class _ByteGate:
         with self.cv:
             h = self.held.get(me)
             if h is None:
-                while self.used > 0 and self.used + nbytes > self.budget:
+                ticket = self.tickets
+                self.tickets += 1
+                self.queue.append(ticket)
+                while (self.queue[0] != ticket
+                       or (self.used > 0 and self.used + nbytes > self.budget)):
                     self.cv.wait()
+                self.queue.popleft()
+                self.cv.notify_all()          # the next waiting request may fit too
                 h = self.held[me] = [0, 0]
             h[0] += 1
             h[1] += nbytes

@@ -142,3 +142,40 @@ def test_rescue_gate_admits_transient_bytes_up_to_its_budget():
     assert gate.used == 0 and peak[0] <= 100
     assert isinstance(engine._RESCUE_GATE, engine._ByteGate)
     assert engine._RESCUE_GATE.budget == engine.RESCUE_BYTES
+
+
+def test_rescue_gate_admits_waiting_requests_in_order():
+    """A request above the budget that waits for the gate to empty is not
+    overtaken by later small requests: they wait behind it, it runs alone,
+    and they follow."""
+    import threading
+    import time
+    from specmc_amd import engine
+    gate = engine._ByteGate(100)
+    order = []
+    lock = threading.Lock()
+
+    def take(name, nbytes, hold_s):
+        with gate.hold(nbytes):
+            with lock:
+                order.append((name, gate.used))
+            time.sleep(hold_s)
+
+    small0 = threading.Thread(target=take, args=("small0", 10, 0.3))
+    small0.start()
+    time.sleep(0.05)
+    big = threading.Thread(target=take, args=("big", 500, 0.1))
+    big.start()
+    time.sleep(0.05)
+    later = [threading.Thread(target=take, args=("small%d" % i, 10, 0.01)) for i in (1, 2, 3)]
+    for t in later:
+        t.start()
+        time.sleep(0.02)
+    for t in [small0, big] + later:
+        t.join(timeout=10)
+    assert all(not t.is_alive() for t in [small0, big] + later)
+    names = [n for n, _ in order]
+    assert names[:2] == ["small0", "big"]
+    assert dict(order)["big"] == 500            # alone
+    assert sorted(names[2:]) == ["small1", "small2", "small3"]
+    assert gate.used == 0 and not gate.queue
