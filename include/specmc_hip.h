@@ -404,6 +404,38 @@ int smx_diag(int prec, int B, int Nx, int N, const void *ux, const void *uz,
 int smx_maxabs(int prec, int B, long n_per_run, const void *x, double *out,
                const int *active, void *stream);
 
+/* States with an x-antisymmetric part (isoviscous model only).  RBC.step,
+ * RBC.cfl_dt and RBC.diagnostics (rbc_galerkin.py:396-471) take any complex
+ * coefficient array; the device carries it as two real channels, a = Re
+ * that (x-even fields) and b = Im that (x-odd fields).
+ * advect_prod_general: the explicit right-hand side of RBC.explicit_rhs
+ *   (rbc_galerkin.py:396-423) and its truncation (_trunc_pad,
+ *   rbc_galerkin.py:338-339) for both channels: ya / yb [B][Nk][2][N] hold
+ *   z-values of (theta, D theta) of channel a / b; ux_e..uz_o the parts of
+ *   the velocity on the padded half grid [B][nrow][N]; out_e = Ae N_e,
+ *   out_o = Ao N_o (h, e2_e, e2_o may be NULL).
+ * phi_grid_general: even and odd parts of phi_grid (eta = 1) from the parts
+ *   of gxx and gxz (dissipation.py:336-362).
+ * diag_general: diag with the odd-parity fields of the general state added
+ *   (ux_o, uz_e, th_e are the mirror-case inputs of smx_diag).
+ * maxabs_pair: per-run max |x| + |y|, the full-grid max |f| of f = x +- y. */
+int smx_advect_prod_general(int prec, int B, int Nk, int nrow, int N, const void *ya,
+                            const void *yb, const void *ks, const void *ux_e,
+                            const void *ux_o, const void *uz_e, const void *uz_o,
+                            const void *h, const void *e2_e, const void *e2_o,
+                            void *out_e, void *out_o, const void *So, const void *Ce,
+                            const void *Ae, const void *Ao, const int *active, void *stream);
+int smx_phi_grid_general(int prec, int B, long np, const void *gxx_e, const void *gxz_o,
+                         const void *gxx_o, const void *gxz_e, const void *scale,
+                         void *out_e, void *out_o, const int *active, void *stream);
+int smx_diag_general(int prec, int B, int Nx, int N, const void *ux_o, const void *uz_e,
+                     const void *th_e, const void *ux_e, const void *uz_o,
+                     const void *th_o, const void *dth0, long ldd, const void *V0,
+                     const void *V1, const void *wq, double *out, const int *active,
+                     void *stream);
+int smx_maxabs_pair(int prec, int B, long n_per_run, const void *x, const void *y,
+                    double *out, const int *active, void *stream);
+
 /* FGMRES state machine (krylov_fgmres.py:60-179), one solver instance per
  * run.  The per-run control block ctl (int[B][8]) and scalars sc
  * (double[B][8]) live in device memory; see the implementation for the

@@ -11,7 +11,7 @@ import torch
 from . import _numerics as NUM
 from .api import model_spec
 from .core.cfl_policy import clamp_safety
-from .engine import Engine, KIND_CONST, KIND_DEPTH, KIND_COUPLED
+from .engine import ANTISYM_TOL, Engine, KIND_CONST, KIND_DEPTH, KIND_COUPLED, antisym_part
 
 _KIND = {"const": KIND_CONST, "eta_z": KIND_DEPTH, "eta_T": KIND_COUPLED,
          "arrhenius": KIND_COUPLED}
@@ -45,7 +45,10 @@ class Ensemble:
         self.eng = Engine(self.kind, Nx, Nz, Lx, [run_params(c) for c in cfgs],
                           prec=prec, **engine_kw)
         self.B = len(cfgs)
-        self.eng.set_theta(np.stack([np.real(t) for t in thetas]))
+        if self.kind != KIND_CONST and any(antisym_part(t) > ANTISYM_TOL for t in thetas):
+            raise ValueError("a %s ensemble needs mirror-symmetric initial states (real "
+                             "x-mode coefficients)" % self.kind)
+        self.eng.set_theta(np.stack([np.asarray(t) for t in thetas]))
         self.tstop = np.broadcast_to(np.asarray(tstops, dtype=np.float64), (self.B,)).copy()
         self.t = np.zeros(self.B)
         self.steps = np.zeros(self.B, dtype=np.int64)
@@ -79,7 +82,7 @@ class Ensemble:
         return n
 
     def states(self):
-        return self.eng.get(self.eng.theta)
+        return self.eng.get_theta()
 
     def budgets(self):
         """Energy budget of every run's last step, on the device (see

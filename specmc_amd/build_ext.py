@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = [os.path.join(HERE, "csrc", f)
-       for f in ("specmc_kernels.hip", "dense_lu.hip", "schur_apply.hip", "gj_mfma.hip")]
+       for f in ("specmc_kernels.hip", "dense_lu.hip", "schur_apply.hip", "gj_mfma.hip",
+                          "general_state.hip")]
 OUT_DIR = os.path.join(HERE, "_lib")
 OUT = os.path.join(OUT_DIR, "libspecmc_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

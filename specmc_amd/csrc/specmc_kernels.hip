@@ -2216,6 +2216,7 @@ extern "C" int smx_phi_grid(int prec, int B, long np, const void *gxx, const voi
 // out[b] = Nu_top, Nu_bot, Nu_energy, urms, umax, amp
 template <typename T>
 __global__ void __launch_bounds__(256) k_diag(int Nx, int N, const T *ux, const T *uz, const T *th,
+                                              const T *uxo, const T *uzo, const T *tho,
                                               const T *dth0, long ldd, const T *V0, const T *V1,
                                               const T *wq, double *out, const int *active) {
   const int b = blockIdx.x;
@@ -2230,6 +2231,17 @@ __global__ void __launch_bounds__(256) k_diag(int Nx, int N, const T *ux, const 
       double w = (j == 0 || j == nrow - 1) ? 1.0 : 2.0;
       double a = (double)ux[o + (long)j * N + q], c = (double)uz[o + (long)j * N + q];
       double t = (double)th[o + (long)j * N + q];
+      if (uxo) {
+        // x-odd parts: squares and products of matching parity survive the
+        // x-mean; x_j and x_{-j} carry e + o and e - o
+        const double ao = (double)uxo[o + (long)j * N + q], co = (double)uzo[o + (long)j * N + q];
+        const double to = (double)tho[o + (long)j * N + q];
+        sk += w * ((a * a + c * c) + (ao * ao + co * co));
+        sn += w * (c * t + co * to);
+        um = fmax(um, fmax(fabs(a) + fabs(ao), fabs(c) + fabs(co)));
+        am = fmax(am, fabs(t) + fabs(to));
+        continue;
+      }
       sk += w * (a * a + c * c);
       sn += w * (c * t);
       um = fmax(um, fmax(fabs(a), fabs(c)));
@@ -2260,23 +2272,41 @@ __global__ void __launch_bounds__(256) k_diag(int Nx, int N, const T *ux, const 
     r[5] = am;
   }
 }
-extern "C" int smx_diag(int prec, int B, int Nx, int N, const void *ux, const void *uz,
-                        const void *th, const void *dth0, long ldd, const void *V0,
-                        const void *V1,
-                        const void *wq, double *out, const int *active, void *stream) {
+static int diag_launch(int prec, int B, int Nx, int N, const void *ux, const void *uz,
+                       const void *th, const void *uxo, const void *uzo, const void *tho,
+                       const void *dth0, long ldd, const void *V0, const void *V1,
+                       const void *wq, double *out, const int *active, void *stream) {
   if (B <= 0) return 0;
   if (prec == 0)
     hipLaunchKernelGGL(k_diag<double>, dim3(B), dim3(256), 0, STREAM(stream), Nx, N,
                        (const double *)ux, (const double *)uz, (const double *)th,
+                       (const double *)uxo, (const double *)uzo, (const double *)tho,
                        (const double *)dth0, ldd, (const double *)V0, (const double *)V1,
                        (const double *)wq, out, active);
   else
     hipLaunchKernelGGL(k_diag<float>, dim3(B), dim3(256), 0, STREAM(stream), Nx, N,
                        (const float *)ux, (const float *)uz, (const float *)th,
+                       (const float *)uxo, (const float *)uzo, (const float *)tho,
                        (const float *)dth0, ldd, (const float *)V0, (const float *)V1,
                        (const float *)wq, out, active);
   CHECK_LAUNCH("smx_diag");
   return 0;
+}
+extern "C" int smx_diag(int prec, int B, int Nx, int N, const void *ux, const void *uz,
+                        const void *th, const void *dth0, long ldd, const void *V0,
+                        const void *V1,
+                        const void *wq, double *out, const int *active, void *stream) {
+  return diag_launch(prec, B, Nx, N, ux, uz, th, nullptr, nullptr, nullptr, dth0, ldd, V0, V1,
+                     wq, out, active, stream);
+}
+extern "C" int smx_diag_general(int prec, int B, int Nx, int N, const void *ux_o,
+                                const void *uz_e, const void *th_e, const void *ux_e,
+                                const void *uz_o, const void *th_o, const void *dth0, long ldd,
+                                const void *V0, const void *V1, const void *wq, double *out,
+                                const int *active, void *stream) {
+  if (!ux_e || !uz_o || !th_o) return fail_msg("smx_diag_general: null odd-part operand");
+  return diag_launch(prec, B, Nx, N, ux_o, uz_e, th_e, ux_e, uz_o, th_o, dth0, ldd, V0, V1, wq,
+                     out, active, stream);
 }
 
 // out[b][k][N-2] = v[b][k] . V0, out[b][k][N-1] = v[b][k] . V1

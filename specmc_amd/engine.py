@@ -13,6 +13,17 @@ Real arithmetic on the mirror-symmetric subspace
     vectors are therefore stored by their imaginary part (the "s-part") and
     every FGMRES inner product equals the reference's ``vdot`` exactly.
 
+States with an x-antisymmetric part (isoviscous model)
+    RBC.step (rbc_galerkin.py:426-436) evolves any complex state.  The engine
+    then carries a second real channel ``theta_o`` = Im theta (x-odd fields)
+    beside ``theta`` = Re theta: the per-mode solves run on both channels
+    with the same factors and the quadratic terms couple them on the half
+    grid (even part f_e g_e + f_o g_o, odd part f_e g_o + f_o g_e).  The
+    stream function of channel b is stored as ``psi_o`` with
+    psi = i psi + (-psi_o) in the reference's convention.  Coefficient
+    arrays whose imaginary part stays within ANTISYM_TOL of their largest
+    entry are mirror states and keep the one-channel path.
+
 Layouts (row-major, run-major)
     spectral vector      [B][Nk][Nz]          (n = Nk*Nz per run)
     padded half grid     [B][Nx+1][Nz]        (x = j*Lx/(2Nx), j = 0..Nx)
@@ -35,6 +46,18 @@ from . import kclock, runtime
 from .basis import shape_ops
 
 KIND_CONST, KIND_DEPTH, KIND_COUPLED = "const", "eta_z", "coupled"
+# an imaginary part of at most this fraction of the largest coefficient is
+# roundoff of a mirror-symmetric state (the rfft of an x-even grid field)
+ANTISYM_TOL = 1e-10
+
+
+def antisym_part(theta):
+    """-> the relative size of the imaginary part of a coefficient array:
+    max |Im theta| / max |theta| (0 for real arrays)."""
+    a = np.asarray(theta)
+    if not np.iscomplexobj(a) or a.size == 0:
+        return 0.0
+    return float(np.abs(a.imag).max()) / max(float(np.abs(a).max()), 1e-300)
 PROFILES = {"harm": 0, "diag": 1, "geom": 2}
 
 MEMO_RTOL = 1e-12          # coupled_model.py:327, 749-765
@@ -390,6 +413,8 @@ class Engine:
         self.counters = torch.zeros(B, 8, dtype=torch.float64, device=dev)
         # state
         self.theta, self.theta_prev = z(B, n), z(B, n)
+        self.general = False           # x-odd channel carried (isoviscous only)
+        self.theta_o = self.psi_o = None
         self.psi, self.psi_prev, self.vsol = z(B, n), z(B, n), z(B, n)
         self.omega = z(B, n)
         self.has_hist = np.zeros(B, dtype=bool)
@@ -485,13 +510,48 @@ class Engine:
 
     # ------------------------------------------------------------ state I/O
     def set_theta(self, th, idx=None):
-        """Upload real coefficient arrays ``th`` (B', Nk, Nz) (runs ``idx``)."""
-        a = torch.from_numpy(np.ascontiguousarray(np.real(th), dtype=np.float64))
-        a = a.reshape(-1, self.n).to(device=self.dev, dtype=self.dtype)
-        if idx is None:
-            self.theta.copy_(a)
-        else:
-            self.theta[torch.as_tensor(idx, device=self.dev)] = a
+        """Upload coefficient arrays ``th`` (B', Nk, Nz) (runs ``idx``).  The
+        imaginary part of an isoviscous state is kept (``theta_o``) when it
+        exceeds ANTISYM_TOL; other imaginary parts are dropped."""
+        dst = (slice(None) if idx is None else torch.as_tensor(idx, device=self.dev))
+        th = np.asarray(th).reshape(-1, self.n)
+        self.theta[dst] = self._dev_real(np.real(th))
+        odd = np.array([self.kind == KIND_CONST and antisym_part(r) > ANTISYM_TOL for r in th])
+        if odd.any():
+            self._ensure_general()
+        if self.general:
+            self.theta_o[dst] = self._dev_real(np.where(odd[:, None], np.imag(th), 0.0))
+
+    def _dev_real(self, a):
+        a = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64))
+        return a.reshape(-1, self.n).to(device=self.dev, dtype=self.dtype)
+
+    def get_theta(self):
+        """-> the (B, Nk, Nz) states: real, or complex once the x-odd channel
+        is carried."""
+        th = self.get(self.theta)
+        return th + 1j * self.get(self.theta_o) if self.general else th
+
+    def _ensure_general(self):
+        """Allocate the x-odd channel of the isoviscous state and its work
+        buffers; every later step of this engine carries both channels (runs
+        without an antisymmetric part carry zeros in it)."""
+        if self.general:
+            return
+        if self.kind != KIND_CONST:
+            raise ValueError("only the isoviscous model carries states with an "
+                             "x-antisymmetric part")
+        B, n, N = self.B, self.n, self.Nz
+        z = lambda *sh: torch.zeros(*sh, dtype=self.dtype, device=self.dev)
+        self.theta_o, self.psi_o, self.omega_o = z(B, n), z(B, n), z(B, n)
+        self.nc_o, self.s1o = z(B, n), z(B, n)
+        self.zvo, self.zvo2 = z(B, 2 * n), z(B, 2 * n)
+        gp = lambda: z(B, self.nrp * N)
+        gu = lambda: z(B, self.nru * N)
+        self.ux_pe, self.uz_po, self.gpo1, self.gpo2, self.eg1o = gp(), gp(), gp(), gp(), gp()
+        self.ux_ue, self.uz_uo, self.th_uo = gu(), gu(), gu()
+        self.bud_o = torch.zeros(B, dtype=torch.float64, device=self.dev)
+        self.general = True
 
     def get(self, t):
         return t.detach().to("cpu", torch.float64).numpy().reshape(self.B, self.Nk, self.Nz)
@@ -520,10 +580,12 @@ class Engine:
         self.Ra[i], self.H[i], self.diss_scale[i] = vals.to(self.dev)
         self.law[i] = torch.from_numpy(self.lawh[i]).to(self.dev)
         self.counters[i].zero_()
-        th = torch.from_numpy(np.ascontiguousarray(np.real(theta), dtype=np.float64))
-        self.theta[i] = th.reshape(self.n).to(device=self.dev, dtype=self.dtype)
+        self.set_theta(np.asarray(theta)[None], [i])
         for t in (self.theta_prev, self.psi, self.psi_prev, self.vsol, self.omega):
             t[i].zero_()
+        if self.general:
+            self.psi_o[i].zero_()
+            self.omega_o[i].zero_()
         self.has_hist[i] = False
         self.dt_prev[i] = 0.0
         self._a0_fact[i] = np.nan
@@ -588,6 +650,8 @@ class Engine:
                     params=np.array(json.dumps(self.params[i], sort_keys=True)),
                     theta=h(self.theta), n_steps=np.array(self.n_steps[i]),
                     n_stokes=np.array(self.n_stokes[i]), counters=h(self.counters))
+        if self.general:
+            snap["theta_odd"] = h(self.theta_o)
         if self.kind != KIND_CONST:
             snap.update(theta_prev=h(self.theta_prev), psi_prev=h(self.psi_prev),
                         dt_prev=np.array(self.dt_prev[i]), has_hist=np.array(self.has_hist[i]))
@@ -613,7 +677,10 @@ class Engine:
             raise ValueError("snapshot precision %s does not match the engine's %s"
                              % (snap["precision"], self.precision))
         params = json.loads(str(snap["params"]))
-        self.admit(i, params, np.asarray(snap["theta"]).reshape(self.Nk, self.Nz))
+        th = np.asarray(snap["theta"]).reshape(self.Nk, self.Nz)
+        if "theta_odd" in snap:
+            th = th + 1j * np.asarray(snap["theta_odd"]).reshape(self.Nk, self.Nz)
+        self.admit(i, params, th)
         d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev)
         self.n_steps[i] = int(snap["n_steps"])
         self.n_stokes[i] = int(snap["n_stokes"])
@@ -694,11 +761,20 @@ class Engine:
                    active=active)
         L.wdot(self.prec, B, nrp * N, self.p(self.uz_p), nrp * N, self.p(self.gp1), nrp * N,
                self.p(self.bud_w["pi"]), self.p(self.bud[:, 3]), 5, self.p(active), s)
+        if self.general:
+            # the x-odd parts add <uz_o theta_o>; the odd part of Phi has zero mean
+            self.zrows(self.theta_o, t["V"], self.zvo, N, N, N, N, active=active)
+            self.xleft(t["So_p"], nrp, self.Nk, self.zvo, N, n, self.gpo1, N, nrp * N, N,
+                       active=active)
+            L.wdot(self.prec, B, nrp * N, self.p(self.uz_po), nrp * N, self.p(self.gpo1),
+                   nrp * N, self.p(self.bud_w["pi"]), self.p(self.bud_o), 1, self.p(active), s)
+            self.bud[:, 3] += self.bud_o
         self.bud[:, 4].zero_()
         if np.any(self.Dih != 0.0):
             eta_c = (self.eta_c if self.kind == KIND_COUPLED else
                      self.eta_c_depth if self.kind == KIND_DEPTH else None)
-            self._phi_grid(self.psi, eta_c, self.gp4, active)
+            self._phi_grid(self.psi, eta_c, self.gp4, active,
+                           out_o=self.eg1o if self.general else None)
             L.wdot(self.prec, B, nrp * N, self.p(self.gp4), nrp * N, None, 0,
                    self.p(self.bud_w["phi"]), self.p(self.bud[:, 4]), 5, self.p(active), s)
         m = self.bud.cpu().numpy()
@@ -727,9 +803,11 @@ class Engine:
         self.zrows(src, self.t["V"], self.s3, N, N, N, N, active=active)
         self.zrows(self.s3, self.t["Vi"], dst, N, N, N, N, active=active)
 
-    def velocity_unpadded(self, psi, active=None):
+    def velocity_unpadded(self, psi, active=None, psi_o=None):
         """ux = grid(D psi), uz = grid(-i k psi) on the unpadded half grid
-        (rbc_galerkin.py:355-372)."""
+        (rbc_galerkin.py:355-372).  With ``psi_o`` (the x-odd channel's
+        stream function) also the other parity parts: ux_ue = Ce(-D psi_o),
+        uz_uo = So(k psi_o)."""
         N, n, nru = self.Nz, self.n, self.nru
         t = self.t
         self.zrows(psi, t["VD"], self.zv, N, N, N, N, active=active)
@@ -738,16 +816,33 @@ class Engine:
                    active=active)
         self.xleft(t["Ce_u"], nru, self.Nk, self.zv2, N, n, self.uz_u, N, nru * N, N,
                    active=active)
+        if psi_o is not None:
+            self.zrows(psi_o, t["VD"], self.zvo, N, N, N, N, alpha=-1.0, active=active)
+            self.zrows(psi_o, t["V"], self.zvo2, N, N, N, N, rscale=t["kx"], active=active)
+            self.xleft(t["Ce_u"], nru, self.Nk, self.zvo, N, n, self.ux_ue, N, nru * N, N,
+                       active=active)
+            self.xleft(t["So_u"], nru, self.Nk, self.zvo2, N, n, self.uz_uo, N, nru * N, N,
+                       active=active)
 
     def cfl(self, safety, dtmax, dtmin, active=None):
         """Per-run CFL dt of the current theta (rbc_galerkin.py:465-471,
         cfl_policy.py:64-74).  Solves Stokes (memoised for the coupled model)."""
         self.stokes(self.theta, active)
-        self.velocity_unpadded(self.psi, active)
+        g = self.general
+        self.velocity_unpadded(self.psi, active, self.psi_o if g else None)
         L, B = self.L, self.B
         nh = self.nru * self.Nz
-        L.maxabs(self.prec, B, nh, self.p(self.ux_u), self.p(self.umax), self.p(active), self.stream())
-        L.maxabs(self.prec, B, nh, self.p(self.uz_u), self.p(self.umax2), self.p(active), self.stream())
+        if g:
+            # full-grid max |u| = max |u_e| + |u_o| over the half grid
+            L.maxabs_pair(self.prec, B, nh, self.p(self.ux_u), self.p(self.ux_ue),
+                          self.p(self.umax), self.p(active), self.stream())
+            L.maxabs_pair(self.prec, B, nh, self.p(self.uz_u), self.p(self.uz_uo),
+                          self.p(self.umax2), self.p(active), self.stream())
+        else:
+            L.maxabs(self.prec, B, nh, self.p(self.ux_u), self.p(self.umax), self.p(active),
+                     self.stream())
+            L.maxabs(self.prec, B, nh, self.p(self.uz_u), self.p(self.umax2), self.p(active),
+                     self.stream())
         torch.maximum(self.umax, self.umax2, out=self.umax)
         hc = self.ops.hcfl
         s = np.broadcast_to(np.asarray(safety, dtype=np.float64), (B,))
@@ -762,12 +857,23 @@ class Engine:
         """(B, 6): Nu_top, Nu_bot, Nu_energy, urms, umax, amp of the current
         theta (rbc_galerkin.py:439-453)."""
         t, N, n = self.t, self.Nz, self.n
+        g = self.general
         self.stokes(self.theta, active)
-        self.velocity_unpadded(self.psi, active)
+        self.velocity_unpadded(self.psi, active, self.psi_o if g else None)
         self.zrows(self.theta, t["V"], self.zv, N, N, N, N, active=active)
         self.xleft(t["Ce_u"], self.nru, self.Nk, self.zv, N, n, self.th_u, N,
                    self.nru * N, N, active=active)
         self.zrows(self.theta, t["D"], self.s1, N, N, N, N, active=active)
+        if g:
+            self.zrows(self.theta_o, t["V"], self.zvo, N, N, N, N, active=active)
+            self.xleft(t["So_u"], self.nru, self.Nk, self.zvo, N, n, self.th_uo, N,
+                       self.nru * N, N, active=active)
+            self.L.diag_general(self.prec, self.B, self.Nx, N, self.p(self.ux_u),
+                                self.p(self.uz_u), self.p(self.th_u), self.p(self.ux_ue),
+                                self.p(self.uz_uo), self.p(self.th_uo), self.p(self.s1), n,
+                                self.p(t["V0"]), self.p(t["V1"]), self.p(t["wq"]),
+                                self.p(self.diag_out), self.p(active), self.stream())
+            return self.diag_out.cpu().numpy().copy()
         self.L.diag(self.prec, self.B, self.Nx, N, self.p(self.ux_u), self.p(self.uz_u),
                     self.p(self.th_u), self.p(self.s1), n, self.p(t["V0"]), self.p(t["V1"]),
                     self.p(t["wq"]), self.p(self.diag_out), self.p(active), self.stream())
@@ -793,15 +899,19 @@ class Engine:
         self.nc = z(B, self.n)
         self.eg1 = z(B, self.nrp * N)
 
-    def _stokes_const(self, theta, active):
-        """omega = Z^{-1}(-i k Ra theta), psi = Z^{-1}(-omega) (rbc_galerkin.py:342-353)."""
+    def _stokes_const(self, theta, active, omega=None, psi=None):
+        """omega = Z^{-1}(-i k Ra theta), psi = Z^{-1}(-omega) (rbc_galerkin.py:342-353).
+        The solves are real-linear: the x-odd channel b gives omega = -omega_o,
+        psi = -psi_o with omega_o, psi_o from the same two solves of b."""
         L, B, Nk, N = self.L, self.B, self.Nk, self.Nz
         s = self.stream()
+        omega = self.omega if omega is None else omega
+        psi = self.psi if psi is None else psi
         L.lu_solve(self.prec, B * Nk, N, self.p(self.z0), self.p(self.z0_piv), self.p(theta),
-                   self.p(self.omega), 1, -1.0, self.p(self.t["kx"]), self.p(self.Ra), Nk, 1,
+                   self.p(omega), 1, -1.0, self.p(self.t["kx"]), self.p(self.Ra), Nk, 1,
                    self.p(active), s)
         L.lu_solve(self.prec, B * Nk, N, self.p(self.z0), self.p(self.z0_piv),
-                   self.p(self.omega), self.p(self.psi), 1, -1.0, None, None, Nk, 1,
+                   self.p(omega), self.p(psi), 1, -1.0, None, None, Nk, 1,
                    self.p(active), s)
 
     def _padded_velocity(self, psi, active):
@@ -813,8 +923,17 @@ class Engine:
                    active=active)
         self.xleft(t["Ce_p"], nrp, self.Nk, self.zv2, N, n, self.uz_p, N, nrp * N, N,
                    active=active)
+        if self.general and psi is self.psi:
+            # x-odd channel: ux_e = synth(-D psi_o), uz_o = synth(k psi_o)
+            self.zrows(self.psi_o, t["VD"], self.zvo, N, N, N, N, alpha=-1.0, active=active)
+            self.zrows(self.psi_o, t["V"], self.zvo2, N, N, N, N, rscale=t["kx"],
+                       active=active)
+            self.xleft(t["Ce_p"], nrp, self.Nk, self.zvo, N, n, self.ux_pe, N, nrp * N, N,
+                       active=active)
+            self.xleft(t["So_p"], nrp, self.Nk, self.zvo2, N, n, self.uz_po, N, nrp * N, N,
+                       active=active)
 
-    def _phi_grid(self, psi, eta_c, out, active):
+    def _phi_grid(self, psi, eta_c, out, active, out_o=None):
         """(Di/Ra) Phi on the padded half grid (dissipation.py:313-362) for the
         (extrapolated) psi and eta coefficients ``eta_c`` (None: eta = 1)."""
         t, N, n, nrp = self.t, self.Nz, self.n, self.nrp
@@ -833,6 +952,30 @@ class Engine:
                    active=active)
         self.xleft(t["So_p"], nrp, self.Nk, self.zv2, N, n, self.gp2, N, nrp * N, N,
                    active=active)
+        if out_o is not None:
+            # x-odd channel (psi = -psi_o): exx_o = synth(-k D psi_o) (odd),
+            # exz_e = synth(-(D D psi_o + k^2 psi_o) / 2) (even); eta = 1
+            if eta_c is not None:
+                raise ValueError("the two-channel dissipation is isoviscous only")
+            L, B, s = self.L, self.B, self.stream()
+            self.zrows(self.psi_o, t["D"], self.s1, N, N, N, N, active=active)
+            self.zrows(self.s1, t["D"], self.s2, N, N, N, N, active=active)
+            L.rowscale(self.prec, B, self.Nk, N, self.p(self.s1), self.p(t["kx"]), -1.0,
+                       None, 0, self.p(self.s1), self.p(active), s)
+            L.rowscale(self.prec, B, self.Nk, N, self.p(self.psi_o), self.p(self.k2), 1.0,
+                       None, 0, self.p(self.s3), self.p(active), s)
+            L.axpby(self.prec, n, B, 0.5, None, self.p(self.s2), 0.5, None,
+                    self.p(self.s3), self.p(self.s2), self.p(active), s)
+            self.zrows(self.s1, t["V"], self.zvo, N, N, N, N, active=active)
+            self.zrows(self.s2, t["V"], self.zvo2, N, N, N, N, alpha=-1.0, active=active)
+            self.xleft(t["So_p"], nrp, self.Nk, self.zvo, N, n, self.gpo1, N, nrp * N, N,
+                       active=active)
+            self.xleft(t["Ce_p"], nrp, self.Nk, self.zvo2, N, n, self.gpo2, N, nrp * N, N,
+                       active=active)
+            L.phi_grid_general(self.prec, B, nrp * N, self.p(self.gp1), self.p(self.gp2),
+                               self.p(self.gpo1), self.p(self.gpo2), self.p(self.diss_scale),
+                               self.p(out), self.p(out_o), self.p(active), s)
+            return
         eta_g = None
         if eta_c is not None:
             self.zrows(eta_c, t["V"], self.zv, N, N, N, N, active=active)
@@ -847,19 +990,32 @@ class Engine:
         """Explicit advection + BE diffusion step (rbc_galerkin.py:396-436)."""
         L, t, B, Nk, N, n, nrp = self.L, self.t, self.B, self.Nk, self.Nz, self.n, self.nrp
         s = self.stream()
+        g = self.general
         self._stokes_const(self.theta, active)
+        if g:
+            self._stokes_const(self.theta_o, active, self.omega_o, self.psi_o)
         self._padded_velocity(self.psi, active)
-        e2 = None
+        e2 = e2o = None
         if np.any(self.Dih != 0.0):
-            self._phi_grid(self.psi, None, self.eg1, active)
-            e2 = self.eg1
+            self._phi_grid(self.psi, None, self.eg1, active, out_o=self.eg1o if g else None)
+            e2, e2o = self.eg1, (self.eg1o if g else None)
         # theta derivatives: thx = synth(i k theta) (odd), thz = synth(D theta)
         self.zrows(self.theta, t["V_VD"], self.zv, N, 2 * N, N, 2 * N, active=active)
         hh = self.H if np.any(self.Hh != 0.0) else None
-        L.advect_prod(self.prec, B, Nk, nrp, N, self.p(self.zv), self.p(t["kx"]),
-                      self.p(self.ux_p), self.p(self.uz_p), self.p(self.uz_p), self.p(hh),
-                      self.p(e2), self.p(self.s1), self.p(t["So_p"]), self.p(t["Ce_p"]),
-                      self.p(t["Ae_p"]), self.p(active), s)
+        if g:
+            self.zrows(self.theta_o, t["V_VD"], self.zvo, N, 2 * N, N, 2 * N, active=active)
+            L.advect_prod_general(self.prec, B, Nk, nrp, N, self.p(self.zv), self.p(self.zvo),
+                                  self.p(t["kx"]), self.p(self.ux_pe), self.p(self.ux_p),
+                                  self.p(self.uz_p), self.p(self.uz_po), self.p(hh),
+                                  self.p(e2), self.p(e2o), self.p(self.s1), self.p(self.s1o),
+                                  self.p(t["So_p"]), self.p(t["Ce_p"]), self.p(t["Ae_p"]),
+                                  self.p(t["Ao_p"]), self.p(active), s)
+            self.zrows(self.s1o, t["Vi"], self.nc_o, N, N, N, N, active=active)
+        else:
+            L.advect_prod(self.prec, B, Nk, nrp, N, self.p(self.zv), self.p(t["kx"]),
+                          self.p(self.ux_p), self.p(self.uz_p), self.p(self.uz_p), self.p(hh),
+                          self.p(e2), self.p(self.s1), self.p(t["So_p"]), self.p(t["Ce_p"]),
+                          self.p(t["Ae_p"]), self.p(active), s)
         self.zrows(self.s1, t["Vi"], self.nc, N, N, N, N, active=active)
         # rhs = theta / dt + Nc ; out = Z_lam^{-1}(-rhs), lam = 1/dt
         dt = np.asarray(dt, dtype=np.float64)
@@ -884,6 +1040,12 @@ class Engine:
                 self.p(self.nc), self.p(self.s1), self.p(active), s)
         L.lu_solve(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv), self.p(self.s1),
                    self.p(self.theta), 1, -1.0, None, None, Nk, 0, self.p(active), s)
+        if g:
+            L.axpby(self.prec, n, B, 1.0, self.p(self.lam), self.p(self.theta_o), 1.0, None,
+                    self.p(self.nc_o), self.p(self.s1o), self.p(active), s)
+            L.lu_solve(self.prec, B * Nk, N, self.p(self.zl), self.p(self.zl_piv),
+                       self.p(self.s1o), self.p(self.theta_o), 1, -1.0, None, None, Nk, 0,
+                       self.p(active), s)
 
     def set_pc_apply(self, mode):
         """Switch the per-mode preconditioner blocks between 'lu' and 'inv';
@@ -1754,6 +1916,8 @@ class Engine:
         self.vsol (and self.omega for const)."""
         if self.kind == KIND_CONST:
             self._stokes_const(theta, active)
+            if self.general and theta is self.theta:
+                self._stokes_const(self.theta_o, active, self.omega_o, self.psi_o)
         elif self.kind == KIND_DEPTH:
             self._stokes_depth(theta, active)
         else:

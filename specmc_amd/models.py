@@ -15,9 +15,13 @@ Nx, Nz, Lx, V, Vi, D, wq, n_steps``.
 The state crosses the boundary as a host numpy array (the reference's
 convention); the device copy is reused as long as the caller hands back the
 array the model returned last, so a march makes no host-to-device copy per
-step.  The stepper runs on the HIP device only: the states must be
-mirror-symmetric (real x-mode coefficients up to roundoff), which every
-scenario of the reference starts from and the box projection maintains.
+step.  The stepper runs on the HIP device only.  The isoviscous model takes
+any state, as RBC does: one with an x-antisymmetric part (imaginary
+coefficients) is carried as two real channels.  The variable-viscosity
+models carry mirror-symmetric states (real x-mode coefficients up to
+roundoff); their ``step`` with ``symmetrize=True`` projects any other state
+onto that subspace first, as the reference's does (convection_modes.py:198,
+implicit_theta.py:527, 610), and their other calls refuse one.
 ``seed`` / ``coeffs`` / ``grid`` are host-side input/output helpers (the
 rfft conventions of rbc_galerkin.py:322-330), not part of the step.
 """
@@ -25,9 +29,10 @@ import numpy as np
 import torch
 
 from . import _numerics as NUM
-from .engine import Engine, KIND_CONST, KIND_DEPTH, KIND_COUPLED
+from .engine import (ANTISYM_TOL, Engine, KIND_CONST, KIND_DEPTH, KIND_COUPLED,
+                     antisym_part)
 
-SYM_TOL = 1e-10
+SYM_TOL = ANTISYM_TOL
 
 
 class _ThetaInfo:
@@ -59,6 +64,7 @@ class _DeviceModel:
         self.dissipation_number = float(dissipation_number)
         self.deta_T = float(deta_T)
         self.symmetrize = bool(symmetrize)
+        self.sym_removed = 0.0       # antisymmetric part removed by the last step's projection
         self.extra = dict(extra)
         params = dict(Ra=self.Ra, deta_T=self.deta_T, Ts=float(arrhenius_Ts),
                       arrhenius_V=float(arrhenius_V), reference_anchor=str(reference_anchor),
@@ -110,23 +116,33 @@ class _DeviceModel:
         return self.coeffs(amp * np.cos(m * np.pi * X) * np.sin(n * np.pi * Z))
 
     # ---- state transfer
-    def _upload(self, that):
+    def _upload(self, that, project=False):
+        """Hand ``that`` to the device.  ``project``: the caller is ``step``,
+        which on a variable-viscosity model with ``symmetrize=True`` starts
+        from project_mirror(that) (convection_modes.py:134-153, 198)."""
         a = np.asarray(that)
         if a.shape != (self.Nk, self.Nz):
             raise ValueError("state must have shape (Nk, Nz) = (%d, %d), got %r"
                              % (self.Nk, self.Nz, a.shape))
         key = a.tobytes()
+        if project and self.kind != KIND_CONST and self.symmetrize:
+            X = self.V @ a.T
+            self.sym_removed = float(np.abs(np.imag(X)).max()
+                                     / max(float(np.abs(X).max()), 1e-300))
         if key == self._dev_key:
             return
-        if np.iscomplexobj(a):
-            scale = max(float(np.abs(a).max()), 1e-300)
-            if float(np.abs(a.imag).max()) > SYM_TOL * scale:
+        anti = antisym_part(a)
+        if anti <= SYM_TOL:
+            a = np.real(a)
+        elif self.kind != KIND_CONST:
+            if not (project and self.symmetrize):
                 raise ValueError(
-                    "the device stepper carries mirror-symmetric states only "
-                    "(real x-mode coefficients); this state has a relative "
-                    "antisymmetric part of %.2e"
-                    % (float(np.abs(a.imag).max()) / scale))
-        self.eng.set_theta(np.real(a)[None])
+                    "a %s model carries mirror-symmetric states (real x-mode "
+                    "coefficients); this state has a relative antisymmetric part "
+                    "of %.2e. Only step() with symmetrize=True accepts it, and "
+                    "projects it as the reference does" % (self.kind, anti))
+            a = np.real(a)       # the device step applies Vi Re(V .) to it
+        self.eng.set_theta(a[None])
         self._dev_key = key
 
     def _resume_from_device(self):
@@ -135,7 +151,7 @@ class _DeviceModel:
         return self._download()
 
     def _download(self):
-        out = self.eng.get(self.eng.theta)[0]
+        out = self.eng.get_theta()[0]
         if self.kind == KIND_CONST:
             out = out.astype(complex)
         self._dev_key = out.tobytes()
@@ -149,7 +165,7 @@ class _DeviceModel:
         return float(self.eng.cfl(safety, dtmax, dtmin)[0])
 
     def step(self, that, dt):
-        self._upload(that)
+        self._upload(that, project=True)
         self.eng.step(float(dt))
         self._after_step()
         return self._download()
@@ -171,6 +187,9 @@ class _DeviceModel:
         psi = 1j * e.get(e.psi)[0]
         if self.kind == KIND_CONST:
             om = 1j * e.get(e.omega)[0]
+            if e.general:       # channel b: psi = -psi_o, omega = -omega_o
+                psi = psi - e.get(e.psi_o)[0]
+                om = om - e.get(e.omega_o)[0]
         else:
             om = -1j * e.get(e.vsol)[0]
         return psi, om
@@ -179,10 +198,34 @@ class _DeviceModel:
         """(ux, uz) on the (Nx, Nz) grid for stream-function coefficients."""
         e = self.eng
         p = np.asarray(psi)
-        a = torch.from_numpy(np.ascontiguousarray(np.imag(p), dtype=np.float64))
-        e.psi.copy_(a.reshape(1, -1).to(device=e.dev, dtype=e.dtype))
-        e.velocity_unpadded(e.psi)
-        return self._mirror(e.ux_u, odd=True), self._mirror(e.uz_u, odd=False)
+        dev = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).reshape(
+            1, -1).to(device=e.dev, dtype=e.dtype)
+        e.psi.copy_(dev(np.imag(p)))
+        # a real part of psi is the stream function of an x-antisymmetric state
+        rel = (float(np.abs(np.real(p)).max()) / max(float(np.abs(p).max()), 1e-300)
+               if np.iscomplexobj(p) and p.size else 1.0 if np.any(p) else 0.0)
+        if rel <= SYM_TOL:
+            e.velocity_unpadded(e.psi)
+            return self._mirror(e.ux_u, odd=True), self._mirror(e.uz_u, odd=False)
+        if self.kind != KIND_CONST:
+            raise ValueError("a %s model carries mirror-symmetric stream functions "
+                             "(imaginary coefficients); this one has a relative real part "
+                             "of %.2e" % (self.kind, rel))
+        e._ensure_general()
+        e.psi_o.copy_(dev(-np.real(p)))
+        e.velocity_unpadded(e.psi, psi_o=e.psi_o)
+        return (self._full(e.ux_ue, e.ux_u), self._full(e.uz_u, e.uz_uo))
+
+    def _full(self, even, odd):
+        """The (Nx, Nz) grid of a field from its x-even and x-odd parts on
+        the half grid: e + o at x_j, e - o at x_{-j}."""
+        h = lambda t: t.detach().to("cpu", torch.float64).numpy().reshape(
+            self.Nx // 2 + 1, self.Nz)
+        e, o = h(even), h(odd)
+        full = np.empty((self.Nx, self.Nz))
+        full[: self.Nx // 2 + 1] = e + o
+        full[self.Nx // 2 + 1:] = (e - o)[1: self.Nx // 2][::-1]
+        return full
 
     def _mirror(self, half, odd):
         h = half.detach().to("cpu", torch.float64).numpy().reshape(self.Nx // 2 + 1, self.Nz)

@@ -68,6 +68,10 @@ SIGNATURES = {
     "smx_phi_grid": [I, I, L, P, P, P, P, P, P, P],
     "smx_diag": [I, I, I, I, P, P, P, P, L, P, P, P, P, P, P],
     "smx_maxabs": [I, I, L, P, P, P, P],
+    "smx_advect_prod_general": [I, I, I, I, I] + [P] * 18,
+    "smx_phi_grid_general": [I, I, L, P, P, P, P, P, P, P, P, P],
+    "smx_diag_general": [I, I, I, I, P, P, P, P, P, P, P, L, P, P, P, P, P, P],
+    "smx_maxabs_pair": [I, I, L, P, P, P, P, P],
     "smx_fg_init": [I, I, L, I, P, P, P, P, P, P, D, D, I, P, P],
     "smx_fg_cycle_end": [I, I, L, I, P, P, P, P, P, P, P, P, P, P],
     "smx_fg_post": [I, I, L, I, P, P, P, P, P, P, P, P, P, P, P, I, P],

@@ -31,8 +31,8 @@ from . import _numerics as NUM
 from .api import model_spec
 from .batch import run_params, _KIND
 from .core.cfl_policy import clamp_safety
-from .engine import (DENSE_FP64_MAXN, Engine, KIND_CONST, KIND_COUPLED, LAG_STORE_BYTES,
-                     RESCUE_RESTART)
+from .engine import (ANTISYM_TOL, DENSE_FP64_MAXN, Engine, KIND_CONST, KIND_COUPLED,
+                     LAG_STORE_BYTES, RESCUE_RESTART, antisym_part)
 
 
 def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=40):
@@ -189,7 +189,7 @@ class Bucket:
             mask = np.zeros(self.S, dtype=np.int32)
             mask[done] = 1
             d = self.eng.diagnostics(torch.as_tensor(mask, device=self.eng.dev))
-        th = self.eng.get(self.eng.theta)
+        th = self.eng.get_theta()
         out = []
         names = ("Nu_top", "Nu_bot", "Nu_energy", "urms", "umax", "amp")
         for i in done:
@@ -253,6 +253,16 @@ class Server:
 
     # ---------------------------------------------------------------- input
     def submit(self, requests):
+        """Queue requests.  A variable-viscosity request must start from a
+        mirror-symmetric state: its march takes the CFL dt of the state as
+        given before the step projects it (convection_modes.py:198), and
+        the device carries those models on the mirror subspace only."""
+        for r in requests:
+            if (self.key_of(r.cfg)[0] != KIND_CONST
+                    and antisym_part(r.theta0) > ANTISYM_TOL):
+                raise ValueError("request %r: a %s model needs a mirror-symmetric initial "
+                                 "state (relative antisymmetric part %.2e)"
+                                 % (r.rid, self.key_of(r.cfg)[0], antisym_part(r.theta0)))
         for r in sorted(requests, key=lambda q: q.arrival):
             self.pending.append(r)
 
