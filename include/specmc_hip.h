@@ -282,17 +282,18 @@ int smx_xprod(int prec, int B, int nf, int nodd, int Nk, int nrow, int Q,
 /* The coupled Schur operator apply (FreeSchurOperator.matvec,
  * lagged_stokes.py:202-220, with mult_eta of lagged_stokes.py:162-193
  * evaluated pseudo-spectrally) for B runs in three launches:
- *   yz[b][k] = W_k v[b][k]     (Wz [Nk][Nz][4Q]: the transposed stacks
- *                               [Vp; 2k^2 Vp G_k; Vp D2 G_k; k Vp D G_k])
- *   tz = smx_xprod(yz, eta)    (4 fields, the first 3 odd in x)
- *   out[b][k] = Mc^T (t0 + t1, k^2 (2 t2 - t0), -4k t3) on rows < Nz-2
+ *   yz[b][k] = W_k v[b][k]     (Wz [Nk][Nz][3Q]: the transposed stacks
+ *                               [Vp + 2k^2 Vp G_k; 2 Vp D2 G_k - Vp;
+ *                                k Vp D G_k])
+ *   tz = smx_xprod(yz, eta)    (3 fields, the first 2 odd in x)
+ *   out[b][k] = Mc^T (u0, k^2 u1, -4k u2) on rows < Nz-2
  *               (Mc [3Q][Nz] = [(D2 Tp)^T; Tp^T; (D Tp)^T]), and the tau rows
  *               out[Nz-2] = V0 . v[b][k], out[Nz-1] = V1 . v[b][k].
  * v and out are [B][Nk][Nz]; eta of run b at eta + b*eta_bs, [nrow][Q];
  * So/Ce [nrow][Nk], Ao/Ae [Nk][nrow] as smx_xprod; kx [Nk].  With a mask,
  * the z-synthesis runs over the active runs only, listed on the device.  The
  * workspace holds yz, tz and that list: at least smx_schur_apply_work(B, Nk,
- * Q) elements of the precision's type, (8 Nk Q + 2) per run. */
+ * Q) elements of the precision's type, (6 Nk Q + 2) per run. */
 long long smx_schur_apply_work(int B, int Nk, int Q);
 /* smx_set_schur_compact(1) (default): with a mask, the z-synthesis of
  * smx_schur_apply runs over the active runs listed on the device (a 64-run

@@ -131,20 +131,25 @@ class ShapeOps:
         """Operators of the three-launch Schur apply (FreeSchurOperator.matvec,
         lagged_stokes.py:202-220), built once per shape in FP64:
 
-        * ``Wz`` (Nk, Nz, 4Q): per mode the transposed stack
-          ``[Vp; 2k^2 Vp G_k; Vp D2 G_k; k Vp D G_k]``, so that one GEMM gives
-          the four z-fields of v and of w = G_k v on the Q product nodes with
-          the convolution factors 2k^2 and k applied;
+        * ``Wz`` (Nk, Nz, 3Q): per mode the transposed stack
+          ``[Vp + 2k^2 Vp G_k; 2 Vp D2 G_k - Vp; k Vp D G_k]``, so that one
+          GEMM gives, on the Q product nodes, the three z-fields whose
+          x-products the operator combines.  The four fields of v and
+          w = G_k v (v, 2k^2 w, D2 w, k D w) enter the operator only as
+          (t0 + t1, 2 t2 - t0, t3) of their products with eta, and the first
+          three have one x-parity, so the sums are taken before the
+          (linear) x-product instead of after it;
         * ``Mc`` (3Q, Nz): ``[(D2 Tp)^T; Tp^T; (D Tp)^T]``, applied to the
-          products ``(t0 + t1, k^2 (2 t2 - t0), -4k t3)``.
+          products ``(u0, k^2 u1, -4k u2)`` of those three fields.
         """
         if self._schur is None:
             N, Q, Nk = self.Nz, self.Q, self.Nk
-            W = np.empty((Nk, N, 4 * Q))
+            W = np.empty((Nk, N, 3 * Q))
             for i, k in enumerate(self.kx):
                 Gk = self.G[i]
-                W[i] = np.concatenate([self.Vp, 2.0 * k * k * (self.Vp @ Gk),
-                                       self.VpD2 @ Gk, k * (self.VpD @ Gk)], 0).T
+                W[i] = np.concatenate([self.Vp + 2.0 * k * k * (self.Vp @ Gk),
+                                       2.0 * (self.VpD2 @ Gk) - self.Vp,
+                                       k * (self.VpD @ Gk)], 0).T
             Mc = np.concatenate([(self.D2 @ self.Tp).T, self.Tp.T, (self.D @ self.Tp).T], 0)
             self._schur = (W, np.ascontiguousarray(Mc))
         return self._schur

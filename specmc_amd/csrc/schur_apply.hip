@@ -3,13 +3,15 @@
 //
 //   1. z-synthesis   yz[b][k] = W_k v[b][k]           per-mode GEMM, K = Nz, over
 //                                                    the active runs (listed)
-//      W_k = [Vp; 2k^2 Vp G_k; Vp D2 G_k; k Vp D G_k] (4Q x Nz): the Green
-//      apply G_k and the synthesis of v, w = G_k v, D2 w, D w on the Q padded
-//      Chebyshev nodes, with the x-convolution factors 2k^2 and k folded in
+//      W_k = [Vp + 2k^2 Vp G_k; 2 Vp D2 G_k - Vp; k Vp D G_k] (3Q x Nz): the
+//      Green apply w = G_k v and the synthesis on the Q padded Chebyshev
+//      nodes of the three combinations of v, 2k^2 w, D2 w and k D w that the
+//      operator uses (the first three share one x-parity, and the x-product
+//      is linear, so they are summed before it)
 //   2. x-products    tz = Ae/Ao (eta . (Ce/So yz))     smx_xprod (mirror half
-//      of the padded x grid, all four fields)
-//   3. combine       out[b][k] = D2Tp (t0 + t1) + Tp k^2 (2 t2 - t0)
-//                                - 4k DTp t3         matrix-core GEMM, K = 3Q
+//      of the padded x grid, all three fields)
+//   3. combine       out[b][k] = D2Tp u0 + Tp k^2 u1 - 4k DTp u2
+//                                                    matrix-core GEMM, K = 3Q
 //      with the tau rows out[Nz-2] = V0 . v, out[Nz-1] = V1 . v
 //      (lagged_stokes.py:212-220; Tp truncates a degree < 2Nz-1 product on the
 //      Q nodes to its first Nz Chebyshev coefficients)
@@ -23,10 +25,10 @@
 #include "specmc_hip.h"
 
 // combine: rows i = b*Nk + k (run-major), columns j < Nz, K = 3Q; A(i, kk) is
-// formed while loading from the four product fields of row i:
-//   kk <  Q : t0 + t1          (against (D2 Tp)^T)
-//   kk < 2Q : k^2 (2 t2 - t0)  (against Tp^T)
-//   kk < 3Q : -4k t3           (against (D Tp)^T)
+// formed while loading from the three product fields of row i:
+//   kk <  Q : u0               (against (D2 Tp)^T)
+//   kk < 2Q : k^2 u1           (against Tp^T)
+//   kk < 3Q : -4k u2           (against (D Tp)^T)
 // 32 x 32 tiles, four wavefronts each owning one 16 x 16 matrix-core block,
 // K chunks of 16 double-buffered through registers.
 #define SC_TM 32
@@ -50,7 +52,7 @@ __global__ void __launch_bounds__(256) k_schur_combine(int Nk, int N, int Q, int
     for (int r = r0; r <= r1; ++r) any |= (active[r] != 0);
     if (!any) return;
   }
-  const int K = 3 * Q, F = 4 * Q;
+  const int K = 3 * Q, F = 3 * Q;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = (w >> 1) * 16, wc = (w & 1) * 16;
   const int lr = lane & 15, lk = lane >> 4;
@@ -75,11 +77,11 @@ __global__ void __launch_bounds__(256) k_schur_combine(int Nk, int N, int Q, int
       T a = T(0);
       if (i < M && c < K) {
         const T *tr = tz + (long)i * F;
-        const int seg = c / Q, q = c - seg * Q;
+        const int seg = c / Q;
         const T kv = kx[i % Nk];
-        if (seg == 0) a = tr[q] + tr[Q + q];
-        else if (seg == 1) a = kv * kv * (T(2) * tr[2 * Q + q] - tr[q]);
-        else a = T(-4) * kv * tr[3 * Q + q];
+        a = tr[c];
+        if (seg == 1) a = kv * kv * a;
+        else if (seg == 2) a = T(-4) * kv * a;
       }
       ra[u] = a;
     }
@@ -150,7 +152,7 @@ extern "C" int smx_set_schur_compact(int on) {
 // elements of at least 4 bytes; proportional to B, so that the capacity of a
 // fixed workspace in runs is its size over the work of one run
 extern "C" long long smx_schur_apply_work(int B, int Nk, int Q) {
-  return (2LL * Nk * 4 * Q + 2) * B;
+  return (2LL * Nk * 3 * Q + 2) * B;
 }
 
 static int schur_apply_run(int prec, int B, int Nk, int N, int nrow, int Q, const void *v,
@@ -158,7 +160,7 @@ static int schur_apply_run(int prec, int B, int Nk, int N, int nrow, int Q, cons
                            const void *Mc, const void *kx, const void *So, const void *Ce,
                            const void *Ao, const void *Ae, const void *V0, const void *V1,
                            void *work, const int *active, void *stream) {
-  const long F = 4L * Q;
+  const long F = 3L * Q;
   const size_t es = prec == 0 ? 8 : 4;
   char *yz = (char *)work;
   char *tz = yz + (size_t)B * Nk * F * es;
@@ -179,8 +181,8 @@ static int schur_apply_run(int prec, int B, int Nk, int N, int nrow, int Q, cons
                      active ? 1 : 0, 1, stream);
   }
   if (rc) return rc;
-  // 2. the four x-products with eta on the padded grid (factors folded in W)
-  rc = smx_xprod(prec, B, 4, 3, Nk, nrow, Q, yz, nullptr, eta, eta_bs, tz, So, Ce, Ao, Ae,
+  // 2. the three x-products with eta on the padded grid (factors folded in W)
+  rc = smx_xprod(prec, B, 3, 2, Nk, nrow, Q, yz, nullptr, eta, eta_bs, tz, So, Ce, Ao, Ae,
                  active, stream);
   if (rc) return rc;
   // 3. truncation, derivatives, combination and tau rows

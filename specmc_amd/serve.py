@@ -53,7 +53,7 @@ def run_bytes(kind, Nx, Nz, precision="fp64", stokes_restart=80, theta_restart=4
     if kind == KIND_COUPLED:
         b += (2 * stokes_restart + 6) * n * w                       # Stokes FGMRES
         b += Nk * N * N * w                                         # BFBT blocks
-        b += (2 * Nk * 4 * Q + Nk * 7 * N + nrp * Q + 2 * nrp * N + 4 * n) * w
+        b += (2 * Nk * 3 * Q + Nk * 7 * N + nrp * Q + 2 * nrp * N + 4 * n) * w
         f32 = precision == "mixed" or n > DENSE_FP64_MAXN
         b += n * n * (4 if f32 else 8) + n * 4                      # lagged factor
         if f32:

@@ -119,15 +119,14 @@ def _twin_schur_apply(o, eta_c, v):
     synthesis, x-products, combination against Mc with the tau rows."""
     W, Mc = o.schur_apply_ops()
     Q, k = o.Q, o.kx[:, None]
-    yz = np.einsum("kn,knc->kc", v, W).reshape(o.Nk, 4, Q)
+    yz = np.einsum("kn,knc->kc", v, W).reshape(o.Nk, 3, Q)
     etaS = o.Ce_p @ (eta_c @ o.Vp.T)
     tz = np.empty_like(yz)
-    for f in range(4):
-        odd = f < 3
+    for f in range(3):
+        odd = f < 2
         g = (o.So_p if odd else o.Ce_p) @ yz[:, f]
         tz[:, f] = (o.Ao_p if odd else o.Ae_p) @ (g * etaS)
-    a = np.concatenate([tz[:, 0] + tz[:, 1], k ** 2 * (2.0 * tz[:, 2] - tz[:, 0]),
-                        -4.0 * k * tz[:, 3]], 1)
+    a = np.concatenate([tz[:, 0], k ** 2 * tz[:, 1], -4.0 * k * tz[:, 2]], 1)
     out = a @ Mc
     out[:, -2] = v @ o.V0
     out[:, -1] = v @ o.V1
