@@ -1526,28 +1526,44 @@ class Engine:
             self.schur_matvec(self.s1, self.s2, mask)
             return self.s2[bt].clone()
 
+        prec = self.prec
+        nrm = torch.empty(nb, dtype=torch.float64, device=self.dev)
+
+        def norms(X):
+            L.rownorm(prec, nb, n, n, self.p(X), self.p(nrm), None, st)
+            return nrm.cpu().numpy().copy()
+
+        def residual(X, out):
+            # out = b - A_free X
+            L.axpby(prec, n, nb, 1.0, None, self.p(b), -1.0, None, self.p(afree(X)),
+                    self.p(out), None, st)
+            return out
+
         b = sv.b[bt].clone()
         x0 = solve(b)
         x = x0.clone()
-        bn = torch.linalg.norm(b, dim=1).clamp_min(1e-300)
-        xn = torch.linalg.norm(x0, dim=1).clamp_min(1e-300)
+        rv = torch.empty_like(b)
+        bn = np.maximum(norms(b), 1e-300)
+        xn = np.maximum(norms(x0), 1e-300)
         kmax = self.max_K if tol is not None else self.K
-        live = torch.ones(nb, dtype=torch.bool, device=self.dev)
-        last = torch.zeros(nb, dtype=torch.float64, device=self.dev)
-        kused = torch.zeros(nb, dtype=torch.int64, device=self.dev)
+        live = np.ones(nb, dtype=bool)
+        live_dev = torch.ones(nb, dtype=torch.int32, device=self.dev)
+        lastc = np.zeros(nb)
+        kused = np.zeros(nb, dtype=np.int64)
         for _ in range(kmax):
-            dx = solve(b - afree(x))
-            c = torch.linalg.norm(dx, dim=1) / xn
-            x = torch.where(live[:, None], x + dx, x)
-            last = torch.where(live, c.to(last.dtype), last)
-            kused += live.to(torch.int64)
+            dx = solve(residual(x, rv))
+            c = norms(dx) / xn
+            # x += dx on the runs still iterating
+            L.axpby(prec, n, nb, 1.0, None, self.p(x), 1.0, None, self.p(dx), self.p(x),
+                    self.p(live_dev), st)
+            lastc = np.where(live, c, lastc)
+            kused += live
             if tol is not None:
                 live = live & ~(c <= tol)
-                if not bool(live.any()):
+                if not live.any():
                     break
-        res = (torch.linalg.norm(b - afree(x), dim=1) / bn).cpu().numpy()
-        lastc = last.cpu().numpy()
-        kused = kused.cpu().numpy()
+                live_dev.copy_(torch.from_numpy(live.astype(np.int32)))
+        res = norms(residual(x, rv)) / bn
         res_direct = None
         for q, r in enumerate(bad):
             # the rescue refreshes the lagged factor at this eta and runs the
@@ -1570,7 +1586,7 @@ class Engine:
                 self.last_route[r] = "bfbt->legacy(weak)"
             else:
                 if res_direct is None:
-                    res_direct = (torch.linalg.norm(b - afree(x0), dim=1) / bn).cpu().numpy()
+                    res_direct = norms(residual(x0, rv)) / bn
                 x[q] = x0[q]
                 self.n_rescue_direct[r] += 1
                 self.last_defect_res[r] = res_direct[q]
