@@ -132,15 +132,21 @@ __global__ void __launch_bounds__(64 * NT * GJM_MPW) k_gj_invert_mfma(double *Am
         const int p = 255 - (int)(wave_max_u64(best) & 0xff);   // the same in every lane
         pj[j] = p;
         const int pl = p & 63, pr = p >> 6;
-        // the pivot row, read out of its lane (uniform values)
+        // the pivot row, read out of its lane (uniform values); pr is uniform,
+        // so only the slot that holds the row is read
         double rp[16];
+        if (RPL == 1 || pr == 0) {
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) {
-          rp[jj] = readlane_t(v[0][jj], pl);
+          for (int jj = 0; jj < 16; ++jj) rp[jj] = readlane_t(v[0][jj], pl);
+        } else {
 #pragma unroll
-          for (int r = 1; r < RPL; ++r) {
-            const double x = readlane_t(v[r][jj], pl);
-            rp[jj] = pr == r ? x : rp[jj];
+          for (int jj = 0; jj < 16; ++jj) {
+            rp[jj] = readlane_t(v[1][jj], pl);
+#pragma unroll
+            for (int r = 2; r < RPL; ++r) {
+              const double x = readlane_t(v[r][jj], pl);
+              rp[jj] = pr == r ? x : rp[jj];
+            }
           }
         }
         const double inv = 1.0 / rp[j];
