@@ -177,3 +177,38 @@ def test_variable_viscosity_step_projects(dev):
         m3 = api.build_model(ConvectionConfig(symmetrize=False, **fields))
     with pytest.raises(ValueError, match="mirror-symmetric"):
         m3.step(th, dt)
+
+
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_served_general_requests_match_solo(dev, concurrent):
+    """Isoviscous requests with and without an antisymmetric part, served
+    through two slots (the third request reuses a retired run's slot), give
+    their solo marches' states, step counts and diagnostics."""
+    from specmc_amd import ConvectionConfig, api
+    from specmc_amd.serve import Request, Server
+    cfg = ConvectionConfig(**CASES["H_Di_16x24"])
+    m = api.build_model(cfg)
+    ths = [general_theta(m, 0.5, 21), m.seed(amp=2e-2), general_theta(m, 0.5, 22)]
+    tstops = [2e-3, 1.5e-3, 1e-3]
+
+    def reqs():
+        return [Request("g%d" % i, cfg, th, ts, arrival=0.0)
+                for i, (th, ts) in enumerate(zip(ths, tstops))]
+
+    solo = {}
+    for r in reqs():
+        mm = api.build_model(cfg)
+        s, info = api.march_adaptive(mm, r.theta0, r.tstop)
+        solo[r.rid] = (s, info["steps"], mm.diagnostics(s))
+    srv = Server(slots=2, concurrent=concurrent, graph_rounds=0)
+    srv.submit(reqs())
+    done = srv.run(max_wall=120)
+    assert len(done) == 3
+    for r in done:
+        s, steps, d = solo[r.rid]
+        err = rel(r.theta, s)
+        print("  %s: steps %d/%d theta rel %.2e" % (r.rid, r.steps, steps, err), flush=True)
+        assert r.steps == steps
+        assert err < 1e-12
+        for k in ("Nu_top", "Nu_energy", "urms", "umax"):
+            assert abs(r.diag[k] - d[k]) <= 1e-12 * max(abs(d[k]), 1.0), (r.rid, k)
